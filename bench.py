@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Particle-updates/s of the constant-velocity bootstrap PF step on one
+MI355X: N = 2^24 particles, FP32 SoA state (x, y, vx, vy), FP64 log-weights,
+systematic resampling when ESS < N/2, observations simulated from a seed
+(synthetic data).
+
+A step is one pass of the hot path over the particle set: the fused move +
+log-likelihood + weight pass, the weight merge (ESS, log-evidence), and, when
+the device decides so, the exact-count resampling and in-place gather. The
+timed region brackets exactly --steps steps with inputs resident in HBM.
+
+Prints one JSON line (the metric contract) with:
+  value       particle-updates/s (N * steps / seconds)
+  roofline    achieved HBM GB/s of the dominant kernel (the fused move,
+              48 algorithmic bytes per particle) from HIP events on the
+              library stream, against 8 TB/s
+  cpu_baseline  the oracle's OpenMP restatement of the reference path on the
+              host cores, on a bounded sample of the same workload
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+MOVE_BYTES = 48       # state r/w 16 + 16 (FP32 x4), log-weight r/w 8 + 8
+RESAMPLE_BYTES = 48   # log-weights 8, ancestor w/r 4 + 4, state r/w 16 + 16
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=195)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=1 << 24)
+    ap.add_argument("--scheme", default="systematic")
+    ap.add_argument("--threshold", type=float, default=0.5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def run_steps(va, f, scheme, thr, t0, k, ox, oy):
+    for t in range(t0, t0 + k):
+        f.move(t, float(ox[t]), float(oy[t]))
+        f.resample_async(scheme, thr)
+    return t0 + k
+
+
+def time_config(va, torch, n, scheme, thr, warmup, steps, ox, oy, seed=1):
+    f = va.CvFilter(n, "float32", seed=seed)
+    f.init(float(ox[0]), float(oy[0]))
+    f.resample_async(scheme, thr)
+    t = run_steps(va, f, scheme, thr, 1, warmup, ox, oy)
+    torch.cuda.synchronize()
+    r0 = f.resample_count()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    t = run_steps(va, f, scheme, thr, t, steps, ox, oy)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    nres = f.resample_count() - r0
+    return f, t, ms, nres
+
+
+def move_kernel_ms(va, torch, f, t, reps, ox, oy):
+    """Average duration of the fused move (+ its one-block weight merge),
+    HIP events on the library stream."""
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    f.move(t, float(ox[t]), float(oy[t]))
+    torch.cuda.synchronize()
+    e0.record()
+    for i in range(reps):
+        tt = t + 1 + i
+        f.move(tt, float(ox[tt]), float(oy[tt]))
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def cpu_baseline(budget_s):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    from vsmc_amd.models import cv_simulate
+    threads = min(os.cpu_count() or 1, 16)
+    n, T = 1 << 18, 10
+    _, ox, oy = cv_simulate(200, 20161)
+    # grow the sample until it takes a meaningful share of the budget
+    secs = ol.pf_run(n, T, 1, ol.SYSTEMATIC, 0.5, ox, oy, threads=threads)["seconds"]
+    while secs < budget_s / 4 and n < (1 << 24):
+        n *= 2
+        secs = ol.pf_run(n, T, 1, ol.SYSTEMATIC, 0.5, ox, oy, threads=threads)["seconds"]
+    return {"value": n * T / secs, "unit": "particle-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle Sampler restatement, N={n}, T={T}, FP64 state, "
+                      f"systematic, ESS<N/2; OpenMP eval_sp loop, sequential "
+                      f"weights/resampling as in the reference; {secs:.2f}s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU (single-GPU build); --gpus {a.gpus} is not "
+              f"supported", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import vsmc_amd as va
+    from vsmc_amd.models import cv_simulate
+
+    torch.cuda.set_device(0)
+    stream = torch.cuda.current_stream()
+    va.check(va.lib().vsmch_set_stream(stream.cuda_stream))
+    scheme = va.SCHEMES[a.scheme]
+    T = a.warmup + a.steps + 64
+    _, ox, oy = cv_simulate(T, 20161)
+
+    f, t, ms, nres = time_config(va, torch, a.n, scheme, a.threshold, a.warmup, a.steps,
+                                 ox, oy)
+    reps = min(20, T - t - 2)
+    mk_ms = move_kernel_ms(va, torch, f, t, reps, ox, oy)
+    f.close()
+    torch.cuda.synchronize()
+
+    secs = ms / 1e3
+    value = a.n * a.steps / secs
+    bytes_alg = a.n * (MOVE_BYTES * a.steps + RESAMPLE_BYTES * nres)
+    achieved = MOVE_BYTES * a.n / (mk_ms / 1e3) / 1e9
+    out = {
+        "metric": "particle-updates/s at N=2^24 (%HBM roofline)",
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 state / f64 weights",
+        "data": "synthetic (constant-velocity track simulated from seed 20161)",
+        "config": {"workload": "cv_bootstrap_pf", "n": a.n, "state": "fp32 SoA x4",
+                   "weights": "fp64", "scheme": a.scheme, "threshold": a.threshold,
+                   "model": "CV dt=0.1 sd 0.02, Student-t obs df=10 scale 0.1",
+                   "parallelism": "single GPU"},
+        "step_hbm_frac": bytes_alg / secs / 1e9 / HBM_PEAK_GBS,
+        "resampled_steps": nres,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_cv_step<float> + weight merge",
+                     "kernel_ms": mk_ms},
+    }
+    if not a.no_extra:
+        extra = {}
+        for name, sch, thr in (("systematic_always", va.SYSTEMATIC, float("inf")),
+                               ("stratified", va.STRATIFIED, a.threshold),
+                               ("multinomial", va.MULTINOMIAL, a.threshold),
+                               ("residual", va.RESIDUAL, a.threshold)):
+            g, _, ms2, nres2 = time_config(va, torch, a.n, sch, thr, 2, 20, ox, oy)
+            g.close()
+            extra[name] = {"ms_per_step": ms2 / 20, "resampled_steps": nres2,
+                           "updates_per_s": a.n * 20 / (ms2 / 1e3),
+                           "step_hbm_frac": a.n * (MOVE_BYTES * 20 + RESAMPLE_BYTES * nres2)
+                           / (ms2 / 1e3) / 1e9 / HBM_PEAK_GBS}
+        out["extra"] = extra
+    out["cpu_baseline"] = None if a.no_cpu else cpu_baseline(a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

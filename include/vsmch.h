@@ -1,0 +1,224 @@
+/*
+ * vsmch.h -- C ABI of libvsmch, the MI355X (gfx950) backend of vSMC's
+ * per-step particle path.
+ *
+ * Conventions
+ *  - Every array argument is a DEVICE pointer (hipMalloc'd) unless the
+ *    parameter name ends in _host. Sizes are plain integers.
+ *  - Every function returns a vsmch_status; vsmch_last_error() describes
+ *    the last failure. The reference C API returns void
+ *    (include/vsmc/vsmc.h:286-310); status codes replace its asserts.
+ *  - Work is queued on the library stream (vsmch_set_stream; default: the
+ *    null stream). Functions that return host values synchronise it.
+ *
+ * Each entry point cites the reference interface it replaces.
+ */
+#ifndef VSMCH_H
+#define VSMCH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int vsmch_status;
+#define VSMCH_OK 0
+#define VSMCH_ERR_INVALID 1
+#define VSMCH_ERR_HIP 2
+#define VSMCH_ERR_ALLOC 3
+#define VSMCH_ERR_UNSUPPORTED 4
+#define VSMCH_ERR_WEIGHTS 5
+
+/* Stream kinds: Philox4x32-10 (include/vsmc/rng/philox.hpp:309) and
+ * Threefry4x64-20 (include/vsmc/rng/threefry.hpp:389). */
+#define VSMCH_PHILOX4X32 0
+#define VSMCH_THREEFRY4X64 1
+
+/* Same order as vsmc_resample_scheme (include/vsmc/vsmc.h:277-284) and
+ * vsmc::ResampleScheme (include/vsmc/internal/defines.hpp:49-56). */
+typedef enum {
+    VSMCH_RESAMPLE_MULTINOMIAL = 0,
+    VSMCH_RESAMPLE_STRATIFIED = 1,
+    VSMCH_RESAMPLE_SYSTEMATIC = 2,
+    VSMCH_RESAMPLE_RESIDUAL = 3,
+    VSMCH_RESAMPLE_RESIDUAL_STRATIFIED = 4,
+    VSMCH_RESAMPLE_RESIDUAL_SYSTEMATIC = 5
+} vsmch_resample_scheme;
+
+const char *vsmch_last_error(void);
+int vsmch_version(void);
+/* Library stream (a hipStream_t). */
+vsmch_status vsmch_set_stream(void *stream);
+vsmch_status vsmch_synchronize(void);
+
+/* ------------------------------------------------------------- streams */
+
+/* Words [word_offset, word_offset + n) of the engine seeded with key0:
+ * u32 words for Philox, u64 words for Threefry. Replaces
+ * CounterEngine::operator()(n, r) (include/vsmc/rng/counter.hpp:357-390)
+ * and vsmc_rng_rand (include/vsmc/vsmc.h:58). */
+vsmch_status vsmch_rng_words(int kind, uint64_t key0, uint64_t word_offset,
+    size_t n, void *out);
+
+/* U01Distribution<double> over the same words: u * 2^-32 (Philox),
+ * (u >> 11) * 2^-53 (Threefry). Replaces u01_distribution
+ * (include/vsmc/rng/u01_distribution.hpp:218-233). */
+vsmch_status vsmch_u01_fill(int kind, uint64_t key0, uint64_t word_offset,
+    size_t n, double *out);
+
+/* Normals. form 0: batch normal_distribution(rng, n, r, mean, sd)
+ * (include/vsmc/rng/normal_distribution.hpp:222-267); form 1: n scalar
+ * NormalDistribution draws (:199-216). */
+vsmch_status vsmch_normal_fill(int kind, uint64_t key0, uint64_t word_offset,
+    size_t n, double mean, double sd, int form, double *out);
+
+/* --------------------------------------------------- resample transforms */
+
+/* resample_trans_u01_rep: counts of M weights against N uniforms, exactly
+ * as the sequential merge of include/vsmc/resample/transform.hpp:42-71
+ * (bit-exact, including its FP64 accumulation order). Replaces
+ * vsmc_resample_trans_u01_rep (include/vsmc/vsmc.h:286-287). */
+vsmch_status vsmch_resample_trans_u01_rep(int m, int n, const double *weight,
+    const double *u01, int *replication);
+
+/* resample_trans_u01_index (transform.hpp:75-95); replaces
+ * vsmc_resample_trans_u01_index (include/vsmc/vsmc.h:288-289). */
+vsmch_status vsmch_resample_trans_u01_index(int m, int n,
+    const double *weight, const double *u01, int *index);
+
+/* resample_trans_rep_index (transform.hpp:99-134); replaces
+ * vsmc_resample_trans_rep_index (include/vsmc/vsmc.h:290-291). */
+vsmch_status vsmch_resample_trans_rep_index(int m, int n,
+    const int *replication, int *index);
+
+/* resample_trans_index_rep (transform.hpp:138-149); replaces
+ * vsmc_resample_trans_index_rep (include/vsmc/vsmc.h:292-293). */
+vsmch_status vsmch_resample_trans_index_rep(int m, int n, const int *index,
+    int *replication);
+
+/* resample_trans_residual (transform.hpp:154-168); the reference returns
+ * R = N - sum(integ) (include/vsmc/vsmc.h:294-295), here *r_host. */
+vsmch_status vsmch_resample_trans_residual(int m, int n, const double *weight,
+    double *resid, int *integ, int *r_host);
+
+/* Resample<Scheme>()(M, N, rng, W, rep) with the Philox4x32 stream
+ * (key0, *word_offset_host) as the resampling engine; on return
+ * *word_offset_host has advanced by the words the reference engine would
+ * have consumed. Replaces vsmc_resample / vsmc_resample_<scheme>
+ * (include/vsmc/vsmc.h:297-310; the reference's vsmc_rng is a build-
+ * dependent engine, here the stream is explicit). */
+vsmch_status vsmch_resample(int m, int n, uint32_t key0,
+    uint64_t *word_offset_host, const double *weight, int *replication,
+    vsmch_resample_scheme scheme);
+
+/* Test hooks (no reference counterpart): the exact sequential prefix sums
+ * out[i] = (((w0 + w1) + w2) + ... + wi) and the U01SequenceSorted values
+ * (include/vsmc/rng/u01_sequence.hpp:130-164) of a Philox stream. */
+vsmch_status vsmch_exact_cumsum(size_t n, const double *w, double *out);
+vsmch_status vsmch_sorted_uniforms(size_t n, uint32_t key0,
+    uint64_t word_offset, double *out);
+
+/* -------------------------------------------------------- particle set */
+
+/* One particle system on the device: N particles, an SoA state of `dim`
+ * components (FP32 or FP64), FP64 log-weights, the Philox streams of
+ * RNGSetVector (particle i: key seed_key(seed, i); resampling engine:
+ * seed_key(seed, N), include/vsmc/rng/rng_set.hpp:88-115,
+ * include/vsmc/core/particle.hpp:60-67). Replaces Particle<T> + Weight +
+ * StateMatrix<ColMajor, Dim, T> for device states. */
+typedef struct vsmch_particle vsmch_particle;
+
+typedef struct {
+    int64_t n;            /* particles */
+    int dim;              /* state components */
+    int state_bytes;      /* 4 (float) or 8 (double) */
+    uint32_t seed;        /* Seed::instance().set(seed) before construction */
+} vsmch_particle_config;
+
+vsmch_status vsmch_particle_create(vsmch_particle **p,
+    const vsmch_particle_config *cfg);
+vsmch_status vsmch_particle_destroy(vsmch_particle *p);
+/* Device pointers of the state (component d at state + d * n) and of the
+ * log-weights. */
+vsmch_status vsmch_particle_state(vsmch_particle *p, void **state);
+vsmch_status vsmch_particle_log_weight(vsmch_particle *p, double **logw);
+/* Key of particle i's stream and of the resampling engine; the resampling
+ * engine's next unread word. */
+vsmch_status vsmch_particle_keys(vsmch_particle *p, uint32_t *key_first_host,
+    uint32_t *resample_key_host, uint64_t *resample_word_host);
+
+/* Weight::set_equal (include/vsmc/core/weight.hpp:99-103). */
+vsmch_status vsmch_weight_set_equal(vsmch_particle *p);
+/* Weight::set_log (weight.hpp:144-149) from a device array. */
+vsmch_status vsmch_weight_set_log(vsmch_particle *p, const double *logw);
+/* Weight::add_log (weight.hpp:168-173) from a device array. */
+vsmch_status vsmch_weight_add_log(vsmch_particle *p, const double *inc);
+/* Weight::ess (weight.hpp:76); also the log-evidence accumulated by
+ * set_log/add_log (log sum_i W_prev,i exp(inc_i),
+ * user_guide/tex/smc.tex:169-182). Synchronises. */
+vsmch_status vsmch_weight_ess(vsmch_particle *p, double *ess_host);
+vsmch_status vsmch_weight_log_zconst(vsmch_particle *p, double *logz_host);
+/* Normalised linear weights W (Weight::data, weight.hpp:81). */
+vsmch_status vsmch_weight_read(vsmch_particle *p, double *W);
+
+/* Particle::resample(op, threshold) (include/vsmc/core/particle.hpp:
+ * 170-197): if ESS < threshold * N, draws counts with `scheme` from the
+ * resampling engine, turns them into ancestors (transform.hpp:99-134),
+ * gathers the state in place (state_matrix.hpp:356-364) and sets equal
+ * weights. *resampled_host (may be NULL) receives the decision.
+ * rep_out / idx_out (device, may be NULL) receive counts and ancestors. */
+vsmch_status vsmch_particle_resample(vsmch_particle *p, int scheme,
+    double threshold, int *resampled_host, int *rep_out, int *idx_out);
+
+/* Number of resampling events so far (the length of the true entries of
+ * Sampler::resampled_history, include/vsmc/core/sampler.hpp:702-707). */
+vsmch_status vsmch_particle_resample_count(vsmch_particle *p,
+    int64_t *count_host);
+
+/* Monitor::eval (include/vsmc/core/monitor.hpp:242-268): r_d = sum_i W_i
+ * f_d(x_i) for f the identity on components [c0, c0 + dim). */
+vsmch_status vsmch_monitor_state_moments(vsmch_particle *p, int c0, int dim,
+    double *r_host);
+
+/* ------------------------------------------- constant-velocity PF model */
+
+/* The constant-velocity bootstrap PF of user_guide/cpp/pf_seq.cpp with the
+ * constants of example/pf/include/pf.hpp (state x, y, vx, vy; Student-t or
+ * Gaussian position observations). obs_model 0: Student-t without the
+ * normalising constants (the reference's log_likelihood), 1: with them,
+ * 2: Gaussian with constants. */
+typedef struct {
+    double dt;
+    double sd_pos, sd_vel;    /* process noise */
+    double sd_pos0, sd_vel0;  /* prior */
+    int obs_model;
+    double obs_scale, obs_df;
+} vsmch_cv_model;
+
+/* PFInit::eval_sp for all particles, then Weight::set_log
+ * (user_guide/cpp/pf_seq.cpp:70-86). obs_* are host scalars. */
+vsmch_status vsmch_cv_init(vsmch_particle *p, const vsmch_cv_model *m,
+    double obs_x, double obs_y);
+/* PFMove::eval_sp for all particles at step `iter` (>= 1), then
+ * Weight::add_log (pf_seq.cpp:106-126), fused into one pass. */
+vsmch_status vsmch_cv_move(vsmch_particle *p, const vsmch_cv_model *m,
+    int64_t iter, double obs_x, double obs_y);
+
+/* The whole Sampler loop for T steps (initialize + iterate(T - 1),
+ * include/vsmc/core/sampler.hpp:443-476, :735-785) for the model above:
+ * per step ESS before resampling, resampled flag, monitor "pos" (the
+ * weighted mean of x, y after resampling) and the cumulative log-evidence,
+ * all kept on the device and copied to the host arrays at the end (each
+ * may be NULL). obs_x/obs_y: host arrays of length T. */
+vsmch_status vsmch_cv_run(vsmch_particle *p, const vsmch_cv_model *m,
+    int64_t T, const double *obs_x_host, const double *obs_y_host,
+    int scheme, double threshold, double *ess_host, int *resampled_host,
+    double *pos_host, double *logz_host);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VSMCH_H */

@@ -1,0 +1,136 @@
+"""Device particle set (Weight, Particle::resample, Monitor) and the whole
+constant-velocity PF loop against the oracle's Sampler restatement."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _t(a, dtype=None):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda", dtype=dtype)
+
+
+def _np(t):
+    return t.cpu().numpy()
+
+
+def c0_obs():
+    with open(os.path.join(GOLD, "cv_obs_c0.json")) as f:
+        g = json.load(f)
+    return (np.array([float.fromhex(x) for x in g["obs_x"]]),
+            np.array([float.fromhex(x) for x in g["obs_y"]]))
+
+
+@pytest.mark.parametrize("n", [1, 3, 4096, 100003])
+def test_weights(dev, n):
+    import vsmc_amd as va
+    rs = np.random.default_rng(n)
+    p = va.Particle(n, 2, "float64", seed=5)
+    lw = rs.normal(0, 3, n)
+    p.set_log(_t(lw))
+    W, ess, dz = ol.weight_set_log(lw)
+    np.testing.assert_allclose(_np(p.weights()), W, rtol=1e-13, atol=1e-300)
+    np.testing.assert_allclose(p.ess(), ess, rtol=1e-12)
+    np.testing.assert_allclose(p.log_zconst(), dz, rtol=1e-12, atol=1e-12)
+    inc = rs.normal(0, 1, n)
+    p.add_log(_t(inc))
+    W2, ess2, dz2 = ol.weight_add_log(W, inc)
+    np.testing.assert_allclose(_np(p.weights()), W2, rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(p.ess(), ess2, rtol=1e-11)
+    np.testing.assert_allclose(p.log_zconst(), dz + dz2, rtol=1e-12, atol=1e-12)
+    p.set_equal()
+    assert abs(p.ess() - n) < 1e-9 * n
+    p.close()
+
+
+@pytest.mark.parametrize("scheme", range(6))
+def test_particle_resample(dev, scheme):
+    """Particle::resample on the device: decision, counts from the device
+    weights with the resampling engine (key seed_key(seed, N)), ancestors,
+    in-place gather of the SoA state, equal weights afterwards."""
+    import torch
+    import vsmc_amd as va
+    n = 50000
+    rs = np.random.default_rng(scheme)
+    p = va.Particle(n, 3, "float32", seed=11)
+    x = torch.as_tensor(rs.normal(size=(3, n)), dtype=torch.float32, device="cuda")
+    p.set_state(x)
+    p.set_log(_t(rs.normal(0, 2, n)))
+    W = _np(p.weights())
+    _, rkey, rword = p.keys()
+    assert rword == 0
+    resampled, rep, idx = p.resample(scheme, 0.5, want_rep=True)
+    assert resampled
+    rep, idx = _np(rep), _np(idx)
+    erep, used = ol.resample(scheme, W, n, rkey)
+    if scheme in (ol.MULTINOMIAL, ol.RESIDUAL):
+        assert int((rep != erep).sum()) <= 2
+    else:
+        assert (rep == erep).all()
+    assert (idx == ol.trans_rep_index(rep)).all()
+    assert (_np(p.state()) == _np(x)[:, idx]).all()
+    assert abs(p.ess() - n) < 1e-6 * n
+    _, _, rword = p.keys()
+    if scheme not in (ol.MULTINOMIAL, ol.RESIDUAL):
+        assert rword == used
+    # threshold never: nothing happens
+    r2, _, _ = p.resample(scheme, -math.inf)
+    assert not r2
+    p.close()
+
+
+def test_monitor_moments(dev):
+    import torch
+    import vsmc_amd as va
+    n = 70001
+    rs = np.random.default_rng(2)
+    p = va.Particle(n, 4, "float64", seed=1)
+    x = rs.normal(size=(4, n))
+    p.set_state(torch.as_tensor(x, device="cuda"))
+    p.set_log(_t(rs.normal(0, 1, n)))
+    W = _np(p.weights())
+    r = p.moments(1, 2)
+    np.testing.assert_allclose(r, x[1:3] @ W, rtol=1e-11)
+
+
+@pytest.mark.parametrize("scheme", [ol.STRATIFIED, ol.SYSTEMATIC, ol.MULTINOMIAL,
+                                    ol.RESIDUAL])
+def test_pf_c0_vs_oracle(dev, scheme):
+    """CPU workload (N = 10^4, T = 100, FP64, ESS < N/2): the device run and
+    the oracle agree step by step: resampled flags equal, ESS and the
+    monitor "pos" within 1e-9 relative, log-evidence within 1e-9."""
+    import vsmc_amd as va
+    ox, oy = c0_obs()
+    N, T, seed = 10000, 100, 20161
+    f = va.CvFilter(N, "float64", seed=seed)
+    g = f.run(ox, oy, scheme, 0.5)
+    e = ol.pf_run(N, T, seed, scheme, 0.5, ox, oy)
+    print("resampled steps", int(g["resampled"].sum()), int(e["resampled"].sum()), flush=True)
+    assert (g["resampled"] == e["resampled"]).all()
+    np.testing.assert_allclose(g["ess"], e["ess"], rtol=1e-9)
+    np.testing.assert_allclose(g["pos"], e["pos"], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(g["logz"], e["logz"], rtol=1e-9, atol=1e-9)
+    f.close()
+
+
+def test_pf_fp32_large(dev):
+    """FP32 state, FP64 weights at N = 2^20: finite, tracks the FP64 run's
+    log-evidence within Monte Carlo error."""
+    import vsmc_amd as va
+    ox, oy = c0_obs()
+    f = va.CvFilter(1 << 20, "float32", seed=3)
+    g = f.run(ox[:50], oy[:50], ol.SYSTEMATIC, 0.5)
+    f.close()
+    h = va.CvFilter(1 << 20, "float64", seed=3)
+    k = h.run(ox[:50], oy[:50], ol.SYSTEMATIC, 0.5)
+    h.close()
+    assert np.isfinite(g["logz"]).all()
+    assert abs(g["logz"][-1] - k["logz"][-1]) < 2.0

@@ -1,0 +1,241 @@
+"""vsmc_amd -- MI355X (gfx950) backend of vSMC's per-step particle path.
+
+The compute lives in the in-tree shared library ``libvsmch.so`` (HIP
+kernels behind the C ABI of ``include/vsmch.h``); this module loads it with
+ctypes and offers thin Python wrappers over device tensors (torch is used
+only to allocate device memory). There is no CPU fallback: every call goes
+to the HIP library, and loading fails loudly if it is missing.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvsmch.so")
+
+PHILOX4X32 = 0
+THREEFRY4X64 = 1
+
+# vsmc::ResampleScheme order (include/vsmc/internal/defines.hpp:49-56)
+MULTINOMIAL = 0
+STRATIFIED = 1
+SYSTEMATIC = 2
+RESIDUAL = 3
+RESIDUAL_STRATIFIED = 4
+RESIDUAL_SYSTEMATIC = 5
+SCHEMES = {
+    "multinomial": MULTINOMIAL,
+    "stratified": STRATIFIED,
+    "systematic": SYSTEMATIC,
+    "residual": RESIDUAL,
+    "residual_stratified": RESIDUAL_STRATIFIED,
+    "residual_systematic": RESIDUAL_SYSTEMATIC,
+}
+
+# Every C symbol of include/vsmch.h with (restype, argtypes).
+_c = ctypes
+_P = _c.c_void_p
+SIGNATURES = {
+    "vsmch_last_error": (_c.c_char_p, []),
+    "vsmch_version": (_c.c_int, []),
+    "vsmch_set_stream": (_c.c_int, [_P]),
+    "vsmch_synchronize": (_c.c_int, []),
+    "vsmch_rng_words": (_c.c_int, [_c.c_int, _c.c_uint64, _c.c_uint64, _c.c_size_t, _P]),
+    "vsmch_u01_fill": (_c.c_int, [_c.c_int, _c.c_uint64, _c.c_uint64, _c.c_size_t, _P]),
+    "vsmch_normal_fill": (_c.c_int, [_c.c_int, _c.c_uint64, _c.c_uint64, _c.c_size_t,
+                                     _c.c_double, _c.c_double, _c.c_int, _P]),
+    "vsmch_resample_trans_u01_rep": (_c.c_int, [_c.c_int, _c.c_int, _P, _P, _P]),
+    "vsmch_resample_trans_u01_index": (_c.c_int, [_c.c_int, _c.c_int, _P, _P, _P]),
+    "vsmch_resample_trans_rep_index": (_c.c_int, [_c.c_int, _c.c_int, _P, _P]),
+    "vsmch_resample_trans_index_rep": (_c.c_int, [_c.c_int, _c.c_int, _P, _P]),
+    "vsmch_resample_trans_residual": (_c.c_int, [_c.c_int, _c.c_int, _P, _P, _P,
+                                                 _c.POINTER(_c.c_int)]),
+    "vsmch_resample": (_c.c_int, [_c.c_int, _c.c_int, _c.c_uint32,
+                                  _c.POINTER(_c.c_uint64), _P, _P, _c.c_int]),
+    "vsmch_exact_cumsum": (_c.c_int, [_c.c_size_t, _P, _P]),
+    "vsmch_sorted_uniforms": (_c.c_int, [_c.c_size_t, _c.c_uint32, _c.c_uint64, _P]),
+    "vsmch_particle_create": (_c.c_int, [_c.POINTER(_P), _P]),
+    "vsmch_particle_destroy": (_c.c_int, [_P]),
+    "vsmch_particle_state": (_c.c_int, [_P, _c.POINTER(_P)]),
+    "vsmch_particle_log_weight": (_c.c_int, [_P, _c.POINTER(_P)]),
+    "vsmch_particle_keys": (_c.c_int, [_P, _c.POINTER(_c.c_uint32),
+                                       _c.POINTER(_c.c_uint32), _c.POINTER(_c.c_uint64)]),
+    "vsmch_weight_set_equal": (_c.c_int, [_P]),
+    "vsmch_weight_set_log": (_c.c_int, [_P, _P]),
+    "vsmch_weight_add_log": (_c.c_int, [_P, _P]),
+    "vsmch_weight_ess": (_c.c_int, [_P, _c.POINTER(_c.c_double)]),
+    "vsmch_weight_log_zconst": (_c.c_int, [_P, _c.POINTER(_c.c_double)]),
+    "vsmch_weight_read": (_c.c_int, [_P, _P]),
+    "vsmch_particle_resample": (_c.c_int, [_P, _c.c_int, _c.c_double,
+                                           _c.POINTER(_c.c_int), _P, _P]),
+    "vsmch_particle_resample_count": (_c.c_int, [_P, _c.POINTER(_c.c_int64)]),
+    "vsmch_monitor_state_moments": (_c.c_int, [_P, _c.c_int, _c.c_int,
+                                               _c.POINTER(_c.c_double)]),
+    "vsmch_cv_init": (_c.c_int, [_P, _P, _c.c_double, _c.c_double]),
+    "vsmch_cv_move": (_c.c_int, [_P, _P, _c.c_int64, _c.c_double, _c.c_double]),
+    "vsmch_cv_run": (_c.c_int, [_P, _P, _c.c_int64, _P, _P, _c.c_int, _c.c_double,
+                                _P, _P, _P, _P]),
+}
+
+
+class ParticleConfig(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("dim", ctypes.c_int),
+                ("state_bytes", ctypes.c_int), ("seed", ctypes.c_uint32)]
+
+
+class CvModel(ctypes.Structure):
+    """vsmch_cv_model: constant-velocity PF constants (example/pf/include/pf.hpp)."""
+    _fields_ = [("dt", ctypes.c_double), ("sd_pos", ctypes.c_double),
+                ("sd_vel", ctypes.c_double), ("sd_pos0", ctypes.c_double),
+                ("sd_vel0", ctypes.c_double), ("obs_model", ctypes.c_int),
+                ("obs_scale", ctypes.c_double), ("obs_df", ctypes.c_double)]
+
+
+def cv_model(dt=0.1, sd_pos=0.02, sd_vel=0.02, sd_pos0=2.0, sd_vel0=1.0,
+             obs_model=0, obs_scale=0.1, obs_df=10.0):
+    return CvModel(dt, sd_pos, sd_vel, sd_pos0, sd_vel0, obs_model, obs_scale, obs_df)
+
+
+class VsmchError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Loads libvsmch.so (built by vsmc_amd.build); raises if it is absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise VsmchError(
+                f"{LIB_PATH} is missing: build it with `python -m vsmc_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().vsmch_last_error()
+        raise VsmchError(f"vsmch status {rc}: {msg.decode() if msg else ''}")
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream_sync():
+    check(lib().vsmch_synchronize())
+
+
+# ------------------------------------------------------------ streams
+
+def rng_words(kind, key, offset, n, device="cuda"):
+    import torch
+    dt = torch.int32 if kind == PHILOX4X32 else torch.int64
+    out = torch.empty(n, dtype=dt, device=device)
+    check(lib().vsmch_rng_words(kind, key, offset, n, _ptr(out)))
+    _stream_sync()
+    return out
+
+
+def u01_fill(kind, key, offset, n, out=None, device="cuda"):
+    import torch
+    if out is None:
+        out = torch.empty(n, dtype=torch.float64, device=device)
+    check(lib().vsmch_u01_fill(kind, key, offset, n, _ptr(out)))
+    return out
+
+
+def normal_fill(kind, key, offset, n, mean=0.0, sd=1.0, form=0, out=None, device="cuda"):
+    import torch
+    if out is None:
+        out = torch.empty(n, dtype=torch.float64, device=device)
+    check(lib().vsmch_normal_fill(kind, key, offset, n, mean, sd, form, _ptr(out)))
+    return out
+
+
+# ------------------------------------------------- resample transforms
+
+def resample_trans_u01_rep(weight, u01, m=None, n=None):
+    """resample_trans_u01_rep (include/vsmc/resample/transform.hpp:42-71)."""
+    import torch
+    m = weight.numel() if m is None else m
+    n = u01.numel() if n is None else n
+    rep = torch.empty(max(m, 1), dtype=torch.int32, device=weight.device)
+    check(lib().vsmch_resample_trans_u01_rep(m, n, _ptr(weight), _ptr(u01), _ptr(rep)))
+    _stream_sync()
+    return rep[:m]
+
+
+def resample_trans_u01_index(weight, u01):
+    import torch
+    m, n = weight.numel(), u01.numel()
+    idx = torch.empty(max(n, 1), dtype=torch.int32, device=weight.device)
+    check(lib().vsmch_resample_trans_u01_index(m, n, _ptr(weight), _ptr(u01), _ptr(idx)))
+    _stream_sync()
+    return idx[:n]
+
+
+def resample_trans_rep_index(rep, n=None):
+    import torch
+    m = rep.numel()
+    n = m if n is None else n
+    idx = torch.empty(max(n, 1), dtype=torch.int32, device=rep.device)
+    check(lib().vsmch_resample_trans_rep_index(m, n, _ptr(rep), _ptr(idx)))
+    _stream_sync()
+    return idx[:n]
+
+
+def resample_trans_index_rep(idx, m):
+    import torch
+    rep = torch.empty(max(m, 1), dtype=torch.int32, device=idx.device)
+    check(lib().vsmch_resample_trans_index_rep(m, idx.numel(), _ptr(idx), _ptr(rep)))
+    _stream_sync()
+    return rep[:m]
+
+
+def resample_trans_residual(weight, n):
+    import torch
+    m = weight.numel()
+    resid = torch.empty(max(m, 1), dtype=torch.float64, device=weight.device)
+    integ = torch.empty(max(m, 1), dtype=torch.int32, device=weight.device)
+    R = ctypes.c_int(0)
+    check(lib().vsmch_resample_trans_residual(m, n, _ptr(weight), _ptr(resid),
+                                              _ptr(integ), ctypes.byref(R)))
+    return resid[:m], integ[:m], R.value
+
+
+def resample(scheme, weight, n, key, word_offset=0, m=None):
+    """Resample<Scheme>()(M, N, rng, W, rep) with the Philox stream (key,
+    word_offset); returns (rep, new word offset)."""
+    import torch
+    m = weight.numel() if m is None else m
+    rep = torch.empty(max(m, 1), dtype=torch.int32, device=weight.device)
+    off = ctypes.c_uint64(word_offset)
+    check(lib().vsmch_resample(m, n, key, ctypes.byref(off), _ptr(weight), _ptr(rep),
+                               scheme))
+    return rep[:m], off.value
+
+
+def exact_cumsum(w):
+    import torch
+    out = torch.empty_like(w)
+    check(lib().vsmch_exact_cumsum(w.numel(), _ptr(w), _ptr(out)))
+    _stream_sync()
+    return out
+
+
+def sorted_uniforms(n, key, word_offset=0, device="cuda"):
+    import torch
+    out = torch.empty(max(n, 1), dtype=torch.float64, device=device)
+    check(lib().vsmch_sorted_uniforms(n, key, word_offset, _ptr(out)))
+    _stream_sync()
+    return out[:n]
+
+
+from .sampler import Particle, CvFilter  # noqa: E402,F401

@@ -1,0 +1,437 @@
+// capi.hip -- the extern "C" entry points declared in include/vsmch.h.
+#include "particle.hpp"
+#include "../../include/vsmch.h"
+#include "../../include/vsmch/device_rng.hpp"
+
+#include <cmath>
+#include <vector>
+
+using namespace vsmch;
+
+namespace {
+
+hipStream_t g_stream = nullptr;
+
+Workspace &ws_current()
+{
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    Workspace &ws = default_workspace(dev);
+    ws.stream = g_stream;
+    return ws;
+}
+
+bool valid_kind(int k) { return k == VSMCH_PHILOX4X32 || k == VSMCH_THREEFRY4X64; }
+
+CvModelDev cv_dev(const vsmch_cv_model *m)
+{
+    CvModelDev d{};
+    d.dt = m->dt;
+    d.sd_pos = m->sd_pos;
+    d.sd_vel = m->sd_vel;
+    d.sd_pos0 = m->sd_pos0;
+    d.sd_vel0 = m->sd_vel0;
+    d.obs_scale = m->obs_scale;
+    d.inv_scale = 1 / m->obs_scale;
+    d.obs_df = m->obs_df;
+    d.obs_model = m->obs_model;
+    const double two_pi = 6.283185307179586476925286766559005768394;
+    if (m->obs_model == 1) {
+        const double nu = m->obs_df;
+        const double k = std::lgamma(0.5 * (nu + 1)) - std::lgamma(0.5 * nu) -
+            0.5 * std::log(nu * 3.141592653589793238462643383279502884197) -
+            std::log(m->obs_scale);
+        d.ll_const = 2 * k;
+    } else if (m->obs_model == 2) {
+        d.ll_const = -(2 * (std::log(m->obs_scale) + 0.5 * std::log(two_pi)));
+    }
+    return d;
+}
+
+} // namespace
+
+struct vsmch_particle {
+    ParticleSet *p;
+};
+
+extern "C" {
+
+const char *vsmch_last_error(void) { return get_error(); }
+
+int vsmch_version(void) { return 10000; }
+
+vsmch_status vsmch_set_stream(void *stream)
+{
+    g_stream = static_cast<hipStream_t>(stream);
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_synchronize(void)
+{
+    VSMCH_HIP(hipStreamSynchronize(g_stream));
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_rng_words(int kind, uint64_t key0, uint64_t word_offset,
+    size_t n, void *out)
+{
+    if (!valid_kind(kind) || (n > 0 && out == nullptr)) {
+        set_error("vsmch_rng_words: invalid argument");
+        return VSMCH_ERR_INVALID;
+    }
+    return stream_fill(kind, key0, word_offset, static_cast<long long>(n), out,
+        0, g_stream);
+}
+
+vsmch_status vsmch_u01_fill(int kind, uint64_t key0, uint64_t word_offset,
+    size_t n, double *out)
+{
+    if (!valid_kind(kind) || (n > 0 && out == nullptr)) {
+        set_error("vsmch_u01_fill: invalid argument");
+        return VSMCH_ERR_INVALID;
+    }
+    return stream_fill(kind, key0, word_offset, static_cast<long long>(n), out,
+        1, g_stream);
+}
+
+vsmch_status vsmch_normal_fill(int kind, uint64_t key0, uint64_t word_offset,
+    size_t n, double mean, double sd, int form, double *out)
+{
+    if (!valid_kind(kind) || (form != 0 && form != 1) ||
+        (n > 0 && out == nullptr)) {
+        set_error("vsmch_normal_fill: invalid argument");
+        return VSMCH_ERR_INVALID;
+    }
+    return normal_fill(kind, key0, word_offset, static_cast<long long>(n),
+        mean, sd, form, out, g_stream);
+}
+
+vsmch_status vsmch_resample_trans_u01_rep(int m, int n, const double *weight,
+    const double *u01, int *replication)
+{
+    if (m < 0 || n < 0 || (m > 0 && (!weight || !replication)) ||
+        (n > 0 && m > 1 && !u01)) {
+        set_error("vsmch_resample_trans_u01_rep: invalid argument");
+        return VSMCH_ERR_INVALID;
+    }
+    return trans_u01_rep(ws_current(), m, n, weight, u01, replication);
+}
+
+vsmch_status vsmch_resample_trans_u01_index(int m, int n,
+    const double *weight, const double *u01, int *index)
+{
+    // transform.hpp:75-95: index[j] is the particle u_j lands in, i.e. the
+    // counts expanded in source order.
+    if (m < 0 || n < 0 || (m > 0 && n > 0 && (!weight || !index))) {
+        set_error("vsmch_resample_trans_u01_index: invalid argument");
+        return VSMCH_ERR_INVALID;
+    }
+    if (m == 0 || n == 0)
+        return VSMCH_OK;
+    Workspace &ws = ws_current();
+    if (!ws.ensure(m > n ? m : n))
+        return VSMCH_ERR_ALLOC;
+    int *rep = nullptr;
+    VSMCH_HIP(hipMalloc(&rep, sizeof(int) * m));
+    int rc = trans_u01_rep(ws, m, n, weight, u01, rep);
+    if (rc == VSMCH_OK)
+        rc = trans_rep_index(ws, m, n, rep, index, nullptr, 0, 0, 4, false,
+            true);
+    (void)hipStreamSynchronize(g_stream);
+    (void)hipFree(rep);
+    return rc;
+}
+
+vsmch_status vsmch_resample_trans_rep_index(int m, int n,
+    const int *replication, int *index)
+{
+    if (m < 0 || n < 0 || (m > 0 && n > 0 && (!replication || !index))) {
+        set_error("vsmch_resample_trans_rep_index: invalid argument");
+        return VSMCH_ERR_INVALID;
+    }
+    return trans_rep_index(ws_current(), m, n, replication, index, nullptr, 0,
+        0, 4, false, false);
+}
+
+vsmch_status vsmch_resample_trans_index_rep(int m, int n, const int *index,
+    int *replication)
+{
+    if (m < 0 || n < 0 || (m > 0 && n > 0 && (!replication || !index))) {
+        set_error("vsmch_resample_trans_index_rep: invalid argument");
+        return VSMCH_ERR_INVALID;
+    }
+    return trans_index_rep(ws_current(), m, n, index, replication);
+}
+
+vsmch_status vsmch_resample_trans_residual(int m, int n, const double *weight,
+    double *resid, int *integ, int *r_host)
+{
+    if (m < 0 || n < 0 || !r_host || (m > 0 && (!weight || !resid || !integ))) {
+        set_error("vsmch_resample_trans_residual: invalid argument");
+        return VSMCH_ERR_INVALID;
+    }
+    long long R = 0;
+    const int rc = trans_residual(ws_current(), m, n, weight, resid, integ, &R);
+    *r_host = static_cast<int>(R);
+    return rc;
+}
+
+vsmch_status vsmch_resample(int m, int n, uint32_t key0,
+    uint64_t *word_offset_host, const double *weight, int *replication,
+    vsmch_resample_scheme scheme)
+{
+    if (m < 0 || n < 0 || !word_offset_host || scheme < 0 || scheme > 5 ||
+        (m > 0 && (!weight || !replication))) {
+        set_error("vsmch_resample: invalid argument");
+        return VSMCH_ERR_INVALID;
+    }
+    if (m == 0) {
+        // ResampleSystematic still constructs its sequence (one word)
+        return VSMCH_OK;
+    }
+    Workspace &ws = ws_current();
+    int rc = resample_scheme(ws, scheme, m, n, key0, *word_offset_host, weight,
+        kWeightArray, replication, false);
+    if (rc != VSMCH_OK)
+        return rc;
+    unsigned long long used = 0;
+    rc = read_words_used(ws, &used);
+    if (rc != VSMCH_OK)
+        return rc;
+    *word_offset_host += used;
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_exact_cumsum(size_t n, const double *w, double *out)
+{
+    if (n > 0 && (!w || !out)) {
+        set_error("vsmch_exact_cumsum: invalid argument");
+        return VSMCH_ERR_INVALID;
+    }
+    return exact_cumsum(ws_current(), static_cast<long long>(n), w, out);
+}
+
+vsmch_status vsmch_sorted_uniforms(size_t n, uint32_t key0,
+    uint64_t word_offset, double *out)
+{
+    if (n > 0 && !out) {
+        set_error("vsmch_sorted_uniforms: invalid argument");
+        return VSMCH_ERR_INVALID;
+    }
+    return sorted_uniforms(ws_current(), static_cast<long long>(n), key0,
+        word_offset, out);
+}
+
+// ------------------------------------------------------------ particles
+
+vsmch_status vsmch_particle_create(vsmch_particle **out,
+    const vsmch_particle_config *cfg)
+{
+    if (!out || !cfg) {
+        set_error("vsmch_particle_create: null argument");
+        return VSMCH_ERR_INVALID;
+    }
+    ParticleSet *p = nullptr;
+    const int rc = particle_create(&p, cfg->n, cfg->dim, cfg->state_bytes,
+        cfg->seed, g_stream);
+    if (rc != VSMCH_OK)
+        return rc;
+    *out = new vsmch_particle{p};
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_particle_destroy(vsmch_particle *p)
+{
+    if (!p)
+        return VSMCH_OK;
+    (void)hipStreamSynchronize(p->p->ws.stream);
+    particle_destroy(p->p);
+    delete p;
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_particle_state(vsmch_particle *p, void **state)
+{
+    if (!p || !state)
+        return VSMCH_ERR_INVALID;
+    *state = p->p->state;
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_particle_log_weight(vsmch_particle *p, double **logw)
+{
+    if (!p || !logw)
+        return VSMCH_ERR_INVALID;
+    *logw = p->p->logw;
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_particle_keys(vsmch_particle *p, uint32_t *key_first_host,
+    uint32_t *resample_key_host, uint64_t *resample_word_host)
+{
+    if (!p)
+        return VSMCH_ERR_INVALID;
+    PDev h{};
+    const int rc = read_pdev(p->p, &h);
+    if (rc != VSMCH_OK)
+        return rc;
+    if (key_first_host)
+        *key_first_host = seed_key(p->p->seed, 0);
+    if (resample_key_host)
+        *resample_key_host = h.rkey;
+    if (resample_word_host)
+        *resample_word_host = h.rword;
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_weight_set_equal(vsmch_particle *p)
+{
+    if (!p)
+        return VSMCH_ERR_INVALID;
+    return weight_update(p->p, nullptr, 2);
+}
+
+vsmch_status vsmch_weight_set_log(vsmch_particle *p, const double *logw)
+{
+    if (!p || !logw)
+        return VSMCH_ERR_INVALID;
+    return weight_update(p->p, logw, 0);
+}
+
+vsmch_status vsmch_weight_add_log(vsmch_particle *p, const double *inc)
+{
+    if (!p || !inc)
+        return VSMCH_ERR_INVALID;
+    return weight_update(p->p, inc, 1);
+}
+
+vsmch_status vsmch_weight_ess(vsmch_particle *p, double *ess_host)
+{
+    if (!p || !ess_host)
+        return VSMCH_ERR_INVALID;
+    PDev h{};
+    const int rc = read_pdev(p->p, &h);
+    *ess_host = h.ess;
+    return rc;
+}
+
+vsmch_status vsmch_weight_log_zconst(vsmch_particle *p, double *logz_host)
+{
+    if (!p || !logz_host)
+        return VSMCH_ERR_INVALID;
+    PDev h{};
+    const int rc = read_pdev(p->p, &h);
+    *logz_host = h.logz;
+    return rc;
+}
+
+vsmch_status vsmch_weight_read(vsmch_particle *p, double *W)
+{
+    if (!p || !W)
+        return VSMCH_ERR_INVALID;
+    return weight_read(p->p, W);
+}
+
+vsmch_status vsmch_particle_resample(vsmch_particle *p, int scheme,
+    double threshold, int *resampled_host, int *rep_out, int *idx_out)
+{
+    if (!p || scheme < 0 || scheme > 5)
+        return VSMCH_ERR_INVALID;
+    ParticleSet *ps = p->p;
+    int rc = resample_step(ps, scheme, threshold, 0, nullptr, nullptr, rep_out,
+        idx_out);
+    if (rc != VSMCH_OK)
+        return rc;
+    if (resampled_host) {
+        PDev h{};
+        rc = read_pdev(ps, &h);
+        *resampled_host = h.resampled;
+    }
+    return rc;
+}
+
+vsmch_status vsmch_particle_resample_count(vsmch_particle *p,
+    int64_t *count_host)
+{
+    if (!p || !count_host)
+        return VSMCH_ERR_INVALID;
+    PDev h{};
+    const int rc = read_pdev(p->p, &h);
+    *count_host = h.nresampled;
+    return rc;
+}
+
+vsmch_status vsmch_monitor_state_moments(vsmch_particle *p, int c0, int dim,
+    double *r_host)
+{
+    if (!p || !r_host)
+        return VSMCH_ERR_INVALID;
+    int rc = moments(p->p, c0, dim, p->p->mout);
+    if (rc != VSMCH_OK)
+        return rc;
+    VSMCH_HIP(hipMemcpyAsync(r_host, p->p->mout, sizeof(double) * dim,
+        hipMemcpyDeviceToHost, p->p->ws.stream));
+    VSMCH_HIP(hipStreamSynchronize(p->p->ws.stream));
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_cv_init(vsmch_particle *p, const vsmch_cv_model *m,
+    double obs_x, double obs_y)
+{
+    if (!p || !m)
+        return VSMCH_ERR_INVALID;
+    return cv_step(p->p, cv_dev(m), 0, obs_x, obs_y, false);
+}
+
+vsmch_status vsmch_cv_move(vsmch_particle *p, const vsmch_cv_model *m,
+    int64_t iter, double obs_x, double obs_y)
+{
+    if (!p || !m || iter < 1)
+        return VSMCH_ERR_INVALID;
+    return cv_step(p->p, cv_dev(m), iter, obs_x, obs_y, false);
+}
+
+vsmch_status vsmch_cv_run(vsmch_particle *p, const vsmch_cv_model *m,
+    int64_t T, const double *obs_x_host, const double *obs_y_host,
+    int scheme, double threshold, double *ess_host, int *resampled_host,
+    double *pos_host, double *logz_host)
+{
+    if (!p || !m || T < 1 || !obs_x_host || !obs_y_host || scheme < 0 ||
+        scheme > 5)
+        return VSMCH_ERR_INVALID;
+    ParticleSet *ps = p->p;
+    hipStream_t st = ps->ws.stream;
+    double *ess_d = nullptr, *pos_d = nullptr, *logz_d = nullptr;
+    int *rs_d = nullptr;
+    VSMCH_HIP(hipMalloc(&ess_d, sizeof(double) * T));
+    VSMCH_HIP(hipMalloc(&pos_d, sizeof(double) * 2 * T));
+    VSMCH_HIP(hipMalloc(&logz_d, sizeof(double) * T));
+    VSMCH_HIP(hipMalloc(&rs_d, sizeof(int) * T));
+    int rc = cv_run(ps, cv_dev(m), T, obs_x_host, obs_y_host, scheme,
+        threshold, ess_d, rs_d, pos_d, logz_d);
+    if (rc == VSMCH_OK) {
+        if (ess_host)
+            (void)hipMemcpyAsync(ess_host, ess_d, sizeof(double) * T,
+                hipMemcpyDeviceToHost, st);
+        if (resampled_host)
+            (void)hipMemcpyAsync(resampled_host, rs_d, sizeof(int) * T,
+                hipMemcpyDeviceToHost, st);
+        if (pos_host)
+            (void)hipMemcpyAsync(pos_host, pos_d, sizeof(double) * 2 * T,
+                hipMemcpyDeviceToHost, st);
+        if (logz_host)
+            (void)hipMemcpyAsync(logz_host, logz_d, sizeof(double) * T,
+                hipMemcpyDeviceToHost, st);
+        if (hipStreamSynchronize(st) != hipSuccess) {
+            set_error("vsmch_cv_run: stream error");
+            rc = VSMCH_ERR_HIP;
+        }
+    }
+    (void)hipFree(ess_d);
+    (void)hipFree(pos_d);
+    (void)hipFree(logz_d);
+    (void)hipFree(rs_d);
+    return rc;
+}
+
+} // extern "C"

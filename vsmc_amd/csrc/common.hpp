@@ -1,0 +1,184 @@
+// common.hpp -- shared device primitives of libvsmch (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+namespace vsmch {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;          // threads per tile block (4 waves)
+constexpr int kItems = 16;           // elements per thread
+constexpr int kTile = kBlock * kItems;  // 4096 elements per tile
+constexpr int kWaves = kBlock / kWave;
+
+// ----------------------------------------------------- wave / block scans
+// __shfl_up for trivially copyable types of 4-byte multiples.
+template <typename T>
+__device__ inline T shfl_up_any(T v, int d)
+{
+    static_assert(sizeof(T) % 4 == 0, "4-byte multiple");
+    constexpr int n = sizeof(T) / 4;
+    union {
+        T t;
+        int w[n];
+    } a, b;
+    a.t = v;
+#pragma unroll
+    for (int i = 0; i < n; ++i)
+        b.w[i] = __shfl_up(a.w[i], d, kWave);
+    return b.t;
+}
+
+template <typename T>
+__device__ inline T shfl_any(T v, int src)
+{
+    static_assert(sizeof(T) % 4 == 0, "4-byte multiple");
+    constexpr int n = sizeof(T) / 4;
+    union {
+        T t;
+        int w[n];
+    } a, b;
+    a.t = v;
+#pragma unroll
+    for (int i = 0; i < n; ++i)
+        b.w[i] = __shfl(a.w[i], src, kWave);
+    return b.t;
+}
+
+template <typename T>
+__device__ inline T shfl_down_any(T v, int d)
+{
+    static_assert(sizeof(T) % 4 == 0, "4-byte multiple");
+    constexpr int n = sizeof(T) / 4;
+    union {
+        T t;
+        int w[n];
+    } a, b;
+    a.t = v;
+#pragma unroll
+    for (int i = 0; i < n; ++i)
+        b.w[i] = __shfl_down(a.w[i], d, kWave);
+    return b.t;
+}
+
+// Inclusive wave scan with an associative operator. Lanes read lane - d
+// through ds_bpermute (__shfl_up); lanes below d keep their value.
+template <typename T, typename Op>
+__device__ inline T wave_inclusive_scan(T v, Op op)
+{
+    const int lane = threadIdx.x % kWave;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        T u = shfl_up_any(v, d);
+        if (lane >= d)
+            v = op(u, v);
+    }
+    return v;
+}
+
+// Block-wide exclusive scan (kBlock threads). Returns the exclusive prefix
+// of this thread; *total receives the block aggregate. `lds` needs kWaves
+// entries. Contains __syncthreads (all threads must call).
+template <typename T, typename Op>
+__device__ inline T block_exclusive_scan(T v, Op op, T identity, T *lds,
+    T *total)
+{
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    T inc = wave_inclusive_scan(v, op);
+    if (lane == kWave - 1)
+        lds[wave] = inc;
+    __syncthreads();
+    T wpre = identity;
+    T agg = identity;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        if (w < wave)
+            wpre = op(wpre, lds[w]);
+        agg = op(agg, lds[w]);
+    }
+    T ex = shfl_up_any(inc, 1);
+    if (lane == 0)
+        ex = identity;
+    __syncthreads();
+    *total = agg;
+    return op(wpre, ex);
+}
+
+template <typename T, typename Op>
+__device__ inline T wave_reduce(T v, Op op)
+{
+#pragma unroll
+    for (int d = kWave / 2; d >= 1; d >>= 1)
+        v = op(v, shfl_down_any(v, d));
+    return shfl_any(v, 0);
+}
+
+// Block-wide reduction, result broadcast to all threads.
+template <typename T, typename Op>
+__device__ inline T block_reduce(T v, Op op, T *lds)
+{
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    v = wave_reduce(v, op);
+    if (lane == 0)
+        lds[wave] = v;
+    __syncthreads();
+    T r = lds[0];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w)
+        r = op(r, lds[w]);
+    __syncthreads();
+    return r;
+}
+
+struct OpAddD {
+    __device__ double operator()(double a, double b) const { return a + b; }
+};
+struct OpAddL {
+    __device__ long long operator()(long long a, long long b) const
+    {
+        return a + b;
+    }
+};
+struct OpMinL {
+    __device__ long long operator()(long long a, long long b) const
+    {
+        return a < b ? a : b;
+    }
+};
+struct OpMaxD {
+    __device__ double operator()(double a, double b) const
+    {
+        return a > b ? a : b;
+    }
+};
+
+// Load a tile of kTile doubles from `src` (n valid, tile-relative base) into
+// blocked order (thread t owns elements t*kItems .. t*kItems+kItems-1),
+// staging through LDS so the global reads are coalesced.
+// LDS index with one pad slot per kItems entries, so that the blocked reads
+// (stride kItems) do not all hit the same banks.
+__device__ inline int lds_pad(int e) { return e + (e / kItems); }
+constexpr int kTileLds = kTile + kTile / kItems;
+
+template <typename Src>
+__device__ inline void load_tile_blocked(Src src, long long base,
+    long long n, double (&w)[kItems], double *lds_tile, double pad)
+{
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const int e = j * kBlock + threadIdx.x;
+        const long long g = base + e;
+        lds_tile[lds_pad(e)] = g < n ? src(g) : pad;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+        w[j] = lds_tile[lds_pad(threadIdx.x * kItems + j)];
+    __syncthreads();
+}
+
+} // namespace vsmch

@@ -1,0 +1,350 @@
+// fold.hpp -- exact emulation of a sequential FP64 left fold on gfx950.
+//
+// resample_trans_u01_rep (include/vsmc/resample/transform.hpp:61-69) and
+// resample_trans_residual (:164) accumulate weights one by one:
+// s_i = fl(s_{i-1} + w_i). A parallel scan rounds differently, and at
+// N = 2^24 that moves tens of replication counts. This file reproduces the
+// sequential values exactly, in parallel:
+//
+//  * While s stays in one binade [2^e, 2^(e+1)), write s = a * u with
+//    u = 2^(e-52) and a an integer in [2^52, 2^53). Adding w >= 0 rounds
+//    a + w/u to the nearest integer, ties to even, so one step is the map
+//    a -> a + d[a mod 2] with d[0], d[1] depending only on (w, e). These
+//    maps compose: (d0, d1) then (c0, c1) is p -> d[p] + c[(p + d[p]) mod 2].
+//  * A step whose result leaves the binade (a + d >= 2^53 under the finer
+//    rounding, which happens exactly when the real sum rounds to >= 2^(e+1))
+//    is done as one real FP64 add from the exact previous value.
+//  * A tile (4096 elements) first predicts, from an approximate prefix, where
+//    the binade changes; its summary is a short list of (binade, map) runs
+//    and crossing weights. One workgroup walks the tile summaries with exact
+//    values (64 single-run tiles per wave step), verifying every prediction;
+//    a tile whose prediction fails is evaluated element by element.
+//  * Each tile then recomputes its own exact prefix from its exact start.
+// Weights must be >= 0 and not NaN; +inf is allowed (sums become +inf).
+#pragma once
+
+#include "common.hpp"
+
+namespace vsmch {
+namespace fold {
+
+constexpr long long kCap = 1LL << 54;
+constexpr long long kTwo53 = 1LL << 53;
+constexpr int kSegMax = 15;       // crossings a tile summary can hold
+constexpr int kEmpty = -100000;   // binade tag of an empty run
+
+struct FMap {
+    long long d0;
+    long long d1;
+};
+
+__host__ __device__ inline FMap ident() { return FMap{0, 0}; }
+
+__host__ __device__ inline long long sat(long long x)
+{
+    return x >= kCap ? kCap : x;
+}
+
+// m1 applied first, then m2.
+__host__ __device__ inline FMap compose(FMap m1, FMap m2)
+{
+    FMap r;
+    r.d0 = m1.d0 >= kCap ? kCap : sat(m1.d0 + ((m1.d0 & 1) ? m2.d1 : m2.d0));
+    r.d1 = m1.d1 >= kCap ? kCap :
+                           sat(m1.d1 + (((m1.d1 + 1) & 1) ? m2.d1 : m2.d0));
+    return r;
+}
+
+struct OpCompose {
+    __device__ FMap operator()(FMap a, FMap b) const { return compose(a, b); }
+};
+
+// Binade exponent of x > 0 (finite); subnormals share the lowest normal ulp.
+__host__ __device__ inline int binade(double x)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    int eb = static_cast<int>((b >> 52) & 0x7FF);
+    return (eb == 0 ? 1 : eb) - 1023;
+}
+
+// Integer a with x = a * 2^(binade(x) - 52).
+__host__ __device__ inline long long mant(double x)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const long long frac = static_cast<long long>(b & ((1ULL << 52) - 1));
+    return ((b >> 52) & 0x7FF) == 0 ? frac : (frac | (1LL << 52));
+}
+
+__host__ __device__ inline double from_mant(long long a, int e)
+{
+    return ldexp(static_cast<double>(a), e - 52);
+}
+
+// Map of adding w (>= 0) while the sum stays in binade e.
+__host__ __device__ inline FMap elem_map(double w, int e)
+{
+    const double q = ldexp(w, 52 - e);
+    if (!(q < static_cast<double>(kCap)))
+        return FMap{kCap, kCap};
+    const long long k = static_cast<long long>(q);
+    const double f = q - static_cast<double>(k);
+    if (f < 0.5)
+        return FMap{k, k};
+    if (f > 0.5)
+        return FMap{k + 1, k + 1};
+    // tie: pick the even result for each parity of a
+    return FMap{(k & 1) ? k + 1 : k, (k & 1) ? k : k + 1};
+}
+
+struct TileSeg {
+    int ncross;              // crossings in the tile; -1: evaluate exactly
+    int e[kSegMax + 1];      // binade of each run (kEmpty if empty)
+    FMap map[kSegMax + 1];   // composed map of each run
+    double cross[kSegMax];   // weight of each crossing element
+};
+
+// Applies a tile summary to the exact start x. Returns false when a
+// prediction does not hold (the caller then evaluates the tile exactly).
+__device__ inline bool apply_segments(const TileSeg &sg, double &x)
+{
+    if (sg.ncross < 0)
+        return false;
+    for (int r = 0; r <= sg.ncross; ++r) {
+        if (sg.e[r] != kEmpty) {
+            if (!(x > 0.0) || !(x < INFINITY))
+                return false;
+            if (binade(x) != sg.e[r])
+                return false;
+            const long long a0 = mant(x);
+            const long long a = a0 + ((a0 & 1) ? sg.map[r].d1 : sg.map[r].d0);
+            if (a >= kTwo53)
+                return false;
+            x = from_mant(a, sg.e[r]);
+        }
+        if (r < sg.ncross)
+            x = x + sg.cross[r];
+    }
+    return true;
+}
+
+struct ExactShared {
+    FMap scan[kWaves];
+    long long red[kWaves];
+    double b0;
+    double b1;
+};
+
+// Exact inclusive fold of one tile held in blocked order, from the exact
+// start x0. All threads of the block must call it (it synchronises).
+__device__ inline void tile_exact(const double (&w)[kItems], double x0,
+    double (&s)[kItems], ExactShared &sh)
+{
+    const int base = threadIdx.x * kItems;
+    int pos = 0;
+    double cur = x0;
+    while (pos < kTile) {
+        if (!(cur < INFINITY)) {
+#pragma unroll
+            for (int j = 0; j < kItems; ++j)
+                if (base + j >= pos)
+                    s[j] = cur;
+            break;
+        }
+        if (cur == 0.0) {
+            long long lf = kTile;
+#pragma unroll
+            for (int j = 0; j < kItems; ++j)
+                if (base + j >= pos && w[j] != 0.0 && lf == kTile)
+                    lf = base + j;
+            const long long k = block_reduce(lf, OpMinL(), sh.red);
+#pragma unroll
+            for (int j = 0; j < kItems; ++j) {
+                const int idx = base + j;
+                if (idx >= pos && idx < k)
+                    s[j] = 0.0;
+                if (idx == k) {
+                    s[j] = w[j];
+                    sh.b0 = w[j];
+                }
+            }
+            if (k >= kTile)
+                break;
+            __syncthreads();
+            cur = sh.b0;
+            __syncthreads();
+            pos = static_cast<int>(k) + 1;
+            continue;
+        }
+        const int e = binade(cur);
+        const long long a0 = mant(cur);
+        const int p0 = static_cast<int>(a0 & 1);
+        FMap agg = ident();
+#pragma unroll
+        for (int j = 0; j < kItems; ++j)
+            if (base + j >= pos)
+                agg = compose(agg, elem_map(w[j], e));
+        FMap total;
+        FMap run = block_exclusive_scan(agg, OpCompose(), ident(), sh.scan,
+            &total);
+        long long lc = kTile;
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const int idx = base + j;
+            if (idx >= pos) {
+                run = compose(run, elem_map(w[j], e));
+                const long long a = a0 + (p0 ? run.d1 : run.d0);
+                if (a >= kTwo53) {
+                    if (lc == kTile)
+                        lc = idx;
+                } else {
+                    s[j] = from_mant(a, e);
+                }
+            }
+        }
+        const long long c = block_reduce(lc, OpMinL(), sh.red);
+        if (c >= kTile)
+            break;
+#pragma unroll
+        for (int j = 0; j < kItems; ++j)
+            if (base + j == c - 1)
+                sh.b0 = s[j];
+        __syncthreads();
+        const double prev = (c == pos) ? cur : sh.b0;
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            if (base + j == c) {
+                s[j] = prev + w[j];
+                sh.b1 = s[j];
+            }
+        }
+        __syncthreads();
+        cur = sh.b1;
+        __syncthreads();
+        pos = static_cast<int>(c) + 1;
+    }
+}
+
+// Segmented composition of (flag, map) pairs: a set flag starts a run.
+struct SegPair {
+    long long flag;
+    FMap m;
+};
+struct OpSeg {
+    __device__ SegPair operator()(SegPair a, SegPair b) const
+    {
+        SegPair r;
+        r.flag = a.flag | b.flag;
+        r.m = b.flag ? b.m : compose(a.m, b.m);
+        return r;
+    }
+};
+
+struct SegShared {
+    double tile[kTileLds];
+    double dscan[kWaves];
+    long long lscan[kWaves];
+    FMap fscan[kWaves];
+    int fflag[kWaves];
+    SegPair pscan[kWaves];
+    unsigned char cross[kTile + 1];
+    TileSeg seg;
+};
+
+// Tile summary from the approximate prefix tpre (start of this tile).
+// w holds the tile in blocked order, nvalid elements valid.
+__device__ inline void tile_segments(const double (&w)[kItems], int nvalid,
+    double tpre, TileSeg *out, SegShared &sh)
+{
+    const int base = threadIdx.x * kItems;
+    double tsum = 0;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+        tsum += w[j];
+    double dtot;
+    const double ex = block_exclusive_scan(tsum, OpAddD(), 0.0, sh.dscan, &dtot);
+    double prev = tpre + ex;
+    // crossing flags and binades of the state before each element
+    bool cr[kItems];
+    int eb[kItems];
+    int ncl = 0;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const double t = prev + w[j];
+        const bool valid = base + j < nvalid;
+        bool c = false;
+        if (valid) {
+            if (!(prev > 0.0) || !(prev < INFINITY) || !(t < INFINITY))
+                c = true;
+            else
+                c = binade(t) != binade(prev);
+        }
+        cr[j] = c;
+        eb[j] = (prev > 0.0 && prev < INFINITY) ? binade(prev) : kEmpty;
+        sh.cross[base + j] = c ? 1 : 0;
+        ncl += c ? 1 : 0;
+        prev = t;
+    }
+    if (threadIdx.x == 0)
+        sh.cross[kTile] = 1;
+    long long ltot;
+    long long rank = block_exclusive_scan(static_cast<long long>(ncl), OpAddL(),
+        0LL, sh.lscan, &ltot);
+    if (threadIdx.x == 0) {
+        sh.seg.ncross = ltot > kSegMax ? -1 : static_cast<int>(ltot);
+        for (int r = 0; r <= kSegMax; ++r) {
+            sh.seg.e[r] = kEmpty;
+            sh.seg.map[r] = ident();
+        }
+        for (int r = 0; r < kSegMax; ++r)
+            sh.seg.cross[r] = 0.0;
+    }
+    __syncthreads();
+    // segmented scan: a run starts at element 0, at a crossing, and right
+    // after a crossing; crossing elements carry the identity
+    SegPair agg{0, ident()};
+    SegPair pj[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const int idx = base + j;
+        const bool prevc = idx == 0 ? true : (sh.cross[idx - 1] != 0);
+        SegPair p;
+        p.flag = (prevc || cr[j]) ? 1 : 0;
+        p.m = (cr[j] || idx >= nvalid || eb[j] == kEmpty) ? ident() :
+                                                            elem_map(w[j], eb[j]);
+        pj[j] = p;
+        agg = OpSeg()(agg, p);
+    }
+    SegPair stot;
+    SegPair run = block_exclusive_scan(agg, OpSeg(), SegPair{0, ident()},
+        sh.pscan, &stot);
+    if (ltot <= kSegMax) {
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const int idx = base + j;
+            run = OpSeg()(run, pj[j]);
+            if (idx < nvalid) {
+                if (cr[j]) {
+                    sh.seg.cross[rank] = w[j];
+                    ++rank;
+                } else {
+                    const bool last = (idx + 1 >= nvalid) || sh.cross[idx + 1];
+                    if (last) {
+                        sh.seg.e[rank] = eb[j];
+                        sh.seg.map[rank] = run.m;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // copy the summary out
+    const int words = static_cast<int>(sizeof(TileSeg) / 4);
+    const int *src = reinterpret_cast<const int *>(&sh.seg);
+    int *dst = reinterpret_cast<int *>(out);
+    for (int i = threadIdx.x; i < words; i += kBlock)
+        dst[i] = src[i];
+    __syncthreads();
+}
+
+} // namespace fold
+} // namespace vsmch

@@ -1,0 +1,156 @@
+// internal.hpp -- host-side API between the C ABI (capi.hip) and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+
+#include "fold.hpp"
+
+namespace vsmch {
+
+// Device-resident parameters of one resampling event. Kernels read them
+// through a pointer so that a captured step needs no host round trip.
+struct ResampleDev {
+    double lmax;          // weights from log-weights: W = exp(l - lmax) * inv_sum
+    double inv_sum;
+    double inv_resid;     // 1 / exact sum of residuals
+    double u0;            // systematic start U * delta
+    double delta;         // fl(1 / ndraw)
+    double resid_total;   // exact sequential sum of residuals
+    long long ndraw;      // number of uniforms (N, or R for residual schemes)
+    long long nsample;    // N (residual schemes add integer parts)
+    long long M;          // number of weights
+    long long integ_sum;  // sum of integer parts (residual schemes)
+    unsigned long long word0;  // resampling-stream word at the event start
+    unsigned long long words_used;  // words the event consumed
+    unsigned int key;     // resampling-stream key
+    int active;           // 0: step did not resample, kernels return early
+    int err;              // input validation flags
+    int equal;            // weights are 1/N (log-weights ignored)
+};
+
+// Grow-only device workspace shared by the resampling passes.
+struct Workspace {
+    hipStream_t stream = nullptr;
+    long long cap_n = 0;
+    long long cap_tiles = 0;
+    double *tsum = nullptr;      // per-tile approximate sums
+    double *tpre = nullptr;      // approximate exclusive tile prefix
+    fold::TileSeg *segs = nullptr;
+    double *xstart = nullptr;    // exact tile starts (+ total at [ntiles])
+    double *xgstart = nullptr;   // exact group starts
+    int *gmode = nullptr;        // 1: group resolved through its group map
+    struct GroupSeg *groups = nullptr;
+    long long *tagg = nullptr;   // per-tile (zeros, extras, sources) of rep
+    long long *toff = nullptr;   // exclusive offsets of tagg (+ totals)
+    int *zdst = nullptr;         // childless slots in dst order
+    int *cE = nullptr;           // extra-copy prefix of sources with extras
+    int *csrc = nullptr;         // those sources
+    double *v = nullptr;         // materialised uniforms (running max)
+    double *aux = nullptr;       // scratch N doubles
+    int *rep = nullptr;          // scratch counts
+    int *integ_dummy = nullptr;
+    ResampleDev *prm = nullptr;
+    long long *scal = nullptr;   // small device scalars
+    std::string err;
+
+    bool ensure(long long n);
+    void release();
+};
+
+Workspace &default_workspace(int device);
+
+// GroupSeg: 64 consecutive tiles that share one binade and have no crossing.
+struct GroupSeg {
+    int flat;
+    int e;
+    fold::FMap map;
+};
+
+constexpr int kGroup = 64;
+
+// Status codes (mirrors vsmch_status in include/vsmch.h)
+enum {
+    kOk = 0,
+    kErrInvalid = 1,
+    kErrHip = 2,
+    kErrAlloc = 3,
+    kErrUnsupported = 4,
+    kErrWeights = 5,
+};
+
+void set_error(const std::string &msg);
+const char *get_error();
+
+#define VSMCH_HIP(expr)                                                        \
+    do {                                                                       \
+        hipError_t e_ = (expr);                                                \
+        if (e_ != hipSuccess) {                                                \
+            ::vsmch::set_error(std::string(#expr) + ": " +                     \
+                hipGetErrorString(e_));                                        \
+            return ::vsmch::kErrHip;                                           \
+        }                                                                      \
+    } while (0)
+
+// ---- host entry points implemented in resample.hip
+
+// Schemes, enum order of include/vsmc/internal/defines.hpp (ResampleScheme)
+enum Scheme {
+    kMultinomial = 0,
+    kStratified = 1,
+    kSystematic = 2,
+    kResidual = 3,
+    kResidualStratified = 4,
+    kResidualSystematic = 5,
+};
+
+// Weight source kinds for the resampling passes.
+enum WeightKind {
+    kWeightArray = 0,    // W given (linear, non-negative)
+    kWeightLog = 1,      // W = exp(l - lmax) * inv_sum from prm
+};
+
+// Replication counts of the transform with explicit uniforms:
+// resample_trans_u01_rep(M, N, W, u01, rep) semantics. Device pointers.
+int trans_u01_rep(Workspace &ws, long long M, long long N, const double *W,
+    const double *u01, int *rep);
+
+// rep -> index (resample_trans_rep_index semantics); optional state gather
+// applied in place on an SoA state of `dim` components of elem_bytes each,
+// component stride `ld` elements. idx may be null when only gathering.
+// force_expand: lay out the expanded list (src repeated rep[src] times)
+// even when M == N (resample_trans_u01_index).
+int trans_rep_index(Workspace &ws, long long M, long long N, const int *rep,
+    int *idx, void *state, int dim, long long ld, int elem_bytes,
+    bool device_active, bool force_expand = false);
+
+int trans_index_rep(Workspace &ws, long long M, long long N, const int *idx,
+    int *rep);
+
+int trans_residual(Workspace &ws, long long M, long long N, const double *W,
+    double *resid, int *integ, long long *R_out);
+
+// Full scheme with the resampling stream (key, word offset): counts for M
+// weights and N draws. weight_kind selects the weight source; for
+// kWeightLog the caller has set prm->lmax / inv_sum on the device and
+// `weights` points at the log-weights. When device_active is true the
+// kernels honour prm->active (set on the device) and the host does not
+// synchronise; otherwise the host sets everything.
+int resample_scheme(Workspace &ws, int scheme, long long M, long long N,
+    unsigned int key, unsigned long long word0, const double *weights,
+    int weight_kind, int *rep, bool device_active);
+
+// Words the last resample_scheme call consumed (read back to the host).
+int read_words_used(Workspace &ws, unsigned long long *out);
+
+// Sorted uniforms of the multinomial scheme (U01SequenceSorted) for
+// ndraw draws from (key, word0): out[j] (device). For tests.
+int sorted_uniforms(Workspace &ws, long long ndraw, unsigned int key,
+    unsigned long long word0, double *out);
+
+// Exact sequential prefix sums (for tests): out[i] = fl(...fl(w0 + w1)...)
+int exact_cumsum(Workspace &ws, long long n, const double *w, double *out);
+
+} // namespace vsmch

@@ -1,0 +1,544 @@
+// particle.hip -- device particle set: weights, fused PF move, monitor.
+//
+// Reference: Weight (include/vsmc/core/weight.hpp:99-229), Particle::
+// resample (include/vsmc/core/particle.hpp:170-197), Monitor::eval
+// (include/vsmc/core/monitor.hpp:242-268), Sampler::do_iter / do_resample
+// (include/vsmc/core/sampler.hpp:746-778), and the PF model of
+// user_guide/cpp/pf_seq.cpp / example/pf/include/pf.hpp.
+//
+// Log-weights are kept as l_i with an offset c such that the normalised
+// weight is W_i = exp(l_i - c); after set_equal the flag `equal` stands for
+// W_i = 1/N without touching l. One pass over the particles writes l and
+// produces per-block (max, sum exp, sum exp^2) partials; a one-block kernel
+// merges them into lmax, the sum, the ESS and the log-evidence increment.
+#include "particle.hpp"
+#include "../../include/vsmch/device_rng.hpp"
+
+namespace vsmch {
+
+constexpr int kPB = 256;              // threads per particle block
+constexpr int kPItems = 16;           // particles per thread
+constexpr int kPTile = kPB * kPItems; // particles per block
+
+struct Part3 {
+    double m, s, q;
+};
+
+// Online merge of (max, sum exp(x - max), sum exp(2 (x - max))).
+__device__ inline Part3 part_merge(Part3 a, Part3 b)
+{
+    if (!(a.m > -INFINITY))
+        return b;
+    if (!(b.m > -INFINITY))
+        return a;
+    if (a.m >= b.m) {
+        const double r = exp(b.m - a.m);
+        return Part3{a.m, a.s + b.s * r, a.q + b.q * r * r};
+    }
+    const double r = exp(a.m - b.m);
+    return Part3{b.m, b.s + a.s * r, b.q + a.q * r * r};
+}
+
+struct OpPart {
+    __device__ Part3 operator()(Part3 a, Part3 b) const { return part_merge(a, b); }
+};
+
+__device__ inline Part3 part_of(double l)
+{
+    return Part3{l, (l > -INFINITY) ? 1.0 : 0.0, (l > -INFINITY) ? 1.0 : 0.0};
+}
+
+__device__ inline void block_partials(Part3 p, Part3 *out)
+{
+    __shared__ Part3 lds[kWaves];
+    p = block_reduce(p, OpPart(), lds);
+    if (threadIdx.x == 0)
+        out[blockIdx.x] = p;
+}
+
+// log W_i of the current weights.
+__device__ inline double logw_of(const double *l, long long i, const PDev *pd)
+{
+    return pd->equal ? pd->log_equal : l[i] - pd->c;
+}
+
+// mode 0: set_log (l = src), mode 1: add_log (l = log W + src)
+__global__ void __launch_bounds__(kPB) k_weight_pass(double *l,
+    const double *src, long long n, int mode, const PDev *pd, Part3 *part)
+{
+    const long long base = static_cast<long long>(blockIdx.x) * kPTile;
+    Part3 acc{-INFINITY, 0.0, 0.0};
+    for (int j = 0; j < kPItems; ++j) {
+        const long long i = base + j * kPB + threadIdx.x;
+        if (i < n) {
+            const double v = mode == 0 ? src[i] : logw_of(l, i, pd) + src[i];
+            l[i] = v;
+            acc = part_merge(acc, part_of(v));
+        }
+    }
+    block_partials(acc, part);
+}
+
+// Merges block partials; mode 0: set_log, 1: add_log, 2: set_equal.
+// Also prepares the resampling parameters of this step (weights from l)
+// and records the history entry `t` when hist is non-null.
+__global__ void __launch_bounds__(kPB) k_weight_finalize(const Part3 *part,
+    long long nblocks, long long n, int mode, PDev *pd)
+{
+    __shared__ Part3 lds[kWaves];
+    Part3 acc{-INFINITY, 0.0, 0.0};
+    if (mode != 2) {
+        for (long long b = threadIdx.x; b < nblocks; b += kPB)
+            acc = part_merge(acc, part[b]);
+        acc = block_reduce(acc, OpPart(), lds);
+    }
+    if (threadIdx.x != 0)
+        return;
+    if (mode == 2) {
+        pd->equal = 1;
+        pd->log_equal = -log(static_cast<double>(n));
+        pd->ess = static_cast<double>(n);
+        pd->lmax = pd->log_equal;
+        pd->sum = 1.0;
+        pd->c = pd->log_equal;
+        return;
+    }
+    pd->equal = 0;
+    pd->lmax = acc.m;
+    pd->sum = acc.s;
+    pd->c = acc.m + log(acc.s);
+    // ESS = 1 / sum W^2 = S^2 / Q (weight.hpp:43-47)
+    pd->ess = acc.s * acc.s / acc.q;
+    // evidence increment: log sum_i W_prev,i exp(inc_i) = c for add_log
+    // (l = log W_prev + inc); set_log assumes equal previous weights.
+    const double inc = mode == 0 ? pd->c - log(static_cast<double>(n)) : pd->c;
+    pd->logz_inc = inc;
+    pd->logz += inc;
+}
+
+// Resampling decision and event parameters (Particle::resample,
+// particle.hpp:172-187; Sampler::do_resample records ESS first,
+// sampler.hpp:772-778).
+__global__ void k_decide(PDev *pd, ResampleDev *prm, long long n,
+    double threshold, long long t, double *ess_hist, int *rs_hist)
+{
+    const double ess = pd->ess;
+    const int active = ess < threshold * static_cast<double>(n) ? 1 : 0;
+    if (ess_hist)
+        ess_hist[t] = ess;
+    if (rs_hist)
+        rs_hist[t] = active;
+    pd->resampled = active;
+    prm->active = active;
+    prm->M = n;
+    prm->nsample = n;
+    prm->ndraw = n;
+    prm->key = pd->rkey;
+    prm->word0 = pd->rword;
+    prm->err = 0;
+    prm->integ_sum = 0;
+    prm->words_used = 0;
+    prm->equal = pd->equal;
+    if (pd->equal) {
+        prm->lmax = pd->log_equal;
+        prm->inv_sum = 1.0;
+    } else {
+        prm->lmax = pd->lmax;
+        prm->inv_sum = 1 / pd->sum;
+    }
+}
+
+// After the event: equal weights, advance the resampling stream.
+__global__ void k_after_resample(PDev *pd, const ResampleDev *prm, long long n)
+{
+    if (!prm->active)
+        return;
+    pd->nresampled += 1;
+    pd->rword += prm->words_used;
+    pd->equal = 1;
+    pd->log_equal = -log(static_cast<double>(n));
+    pd->c = pd->log_equal;
+    pd->ess = static_cast<double>(n);
+}
+
+// Weighted moments r_d = sum_i W_i x_d(i) of components [c0, c0 + dim)
+// (dim <= 8); per-block partials of sum exp(l - lmax) x and the merge.
+template <typename Real>
+__global__ void __launch_bounds__(kPB) k_moments(const Real *state,
+    const double *l, long long n, int c0, int dim, const PDev *pd,
+    double *part)
+{
+    const long long base = static_cast<long long>(blockIdx.x) * kPTile;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int j = 0; j < kPItems; ++j) {
+        const long long i = base + j * kPB + threadIdx.x;
+        if (i < n) {
+            const double w = exp(logw_of(l, i, pd));
+            for (int d = 0; d < dim; ++d)
+                acc[d] += w * static_cast<double>(state[(c0 + d) * n + i]);
+        }
+    }
+    __shared__ double lds[kWaves];
+    for (int d = 0; d < dim; ++d) {
+        const double v = block_reduce(acc[d], OpAddD(), lds);
+        if (threadIdx.x == 0)
+            part[blockIdx.x * 8 + d] = v;
+    }
+}
+
+__global__ void __launch_bounds__(kPB) k_moments_finalize(const double *part,
+    long long nblocks, int dim, double *out)
+{
+    __shared__ double lds[kWaves];
+    for (int d = 0; d < dim; ++d) {
+        double acc = 0.0;
+        for (long long b = threadIdx.x; b < nblocks; b += kPB)
+            acc += part[b * 8 + d];
+        acc = block_reduce(acc, OpAddD(), lds);
+        if (threadIdx.x == 0)
+            out[d] = acc;
+    }
+}
+
+// Normalised linear weights W_i = exp(log W_i).
+__global__ void k_weight_read(const double *l, long long n, const PDev *pd,
+    double *W)
+{
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        W[i] = pd->equal ? 1.0 / static_cast<double>(n) : exp(l[i] - pd->c);
+}
+
+// ------------------------------------------------------------ CV model
+
+template <typename Real>
+__device__ inline Real radd(Real a, Real b);
+template <>
+__device__ inline double radd<double>(double a, double b) { return __dadd_rn(a, b); }
+template <>
+__device__ inline float radd<float>(float a, float b) { return __fadd_rn(a, b); }
+template <typename Real>
+__device__ inline Real rmul(Real a, Real b);
+template <>
+__device__ inline double rmul<double>(double a, double b) { return __dmul_rn(a, b); }
+template <>
+__device__ inline float rmul<float>(float a, float b) { return __fmul_rn(a, b); }
+
+// log-likelihood of example/pf/include/pf.hpp:78-87 (Student-t with the
+// reference's dropped constants), optionally with the constants, or
+// Gaussian. Evaluated in Real.
+template <typename Real>
+__device__ inline Real cv_loglik(const CvModelDev &m, Real x, Real y, Real ox,
+    Real oy)
+{
+    if (m.obs_model == 2) {
+        const Real rx = (x - ox) / static_cast<Real>(m.obs_scale);
+        const Real ry = (y - oy) / static_cast<Real>(m.obs_scale);
+        return static_cast<Real>(-0.5) * (rx * rx + ry * ry) +
+            static_cast<Real>(m.ll_const);
+    }
+    const Real scale = static_cast<Real>(m.inv_scale);
+    const Real nu = static_cast<Real>(m.obs_df);
+    Real lx = rmul(scale, static_cast<Real>(x - ox));
+    Real ly = rmul(scale, static_cast<Real>(y - oy));
+    lx = log(radd(static_cast<Real>(1), rmul(lx, lx) / nu));
+    ly = log(radd(static_cast<Real>(1), rmul(ly, ly) / nu));
+    const Real c = rmul(static_cast<Real>(-0.5), radd(nu, static_cast<Real>(1)));
+    Real ll = rmul(c, radd(lx, ly));
+    if (m.obs_model == 1)
+        ll = ll + static_cast<Real>(m.ll_const);
+    return ll;
+}
+
+// PFInit::eval_sp (user_guide/cpp/pf_seq.cpp:70-81): 4 normals from the
+// first block of the particle's stream: x, y from one Box-Muller pair with
+// sd_pos0, vx, vy from the next with sd_vel0; then the log-likelihood.
+// PFMove::eval_sp (:111-122) at step t uses block t + 1:
+// x += sd_pos z0 + dt vx, y += sd_pos z1 + dt vy, vx += sd_vel z2,
+// vy += sd_vel z3, then log-likelihood; weights by add_log.
+// Also accumulates the monitor moments of the state *before* the move
+// (the previous step's post-resampling "pos" record) when mon != null.
+template <typename Real, bool kInit>
+__global__ void __launch_bounds__(kPB) k_cv_step(Real *state, double *l,
+    long long n, uint32_t seed, long long t, Real ox, Real oy, CvModelDev m,
+    const PDev *pd, Part3 *part, double *mon)
+{
+    const long long base = static_cast<long long>(blockIdx.x) * kPTile;
+    Part3 acc{-INFINITY, 0.0, 0.0};
+    double mx = 0.0, my = 0.0;
+    Real *X = state;
+    Real *Y = state + n;
+    Real *VX = state + 2 * n;
+    Real *VY = state + 3 * n;
+    const double lw_equal = pd->log_equal;
+    const int equal = pd->equal;
+    const double c = pd->c;
+    for (int j = 0; j < kPItems; ++j) {
+        const long long i = base + j * kPB + threadIdx.x;
+        if (i >= n)
+            continue;
+        const uint32_t key = seed_key(seed, static_cast<unsigned long long>(i));
+        const u32x4 wd = philox_block(key, static_cast<unsigned long long>(t) + 1);
+        const normal_pair<Real> p0 = box_muller<Real>(wd.v[0], wd.v[1]);
+        const normal_pair<Real> p1 = box_muller<Real>(wd.v[2], wd.v[3]);
+        Real x, y, vx, vy;
+        double lold = 0.0;
+        if (kInit) {
+            x = rmul(static_cast<Real>(m.sd_pos0), p0.z0);
+            y = rmul(static_cast<Real>(m.sd_pos0), p0.z1);
+            vx = rmul(static_cast<Real>(m.sd_vel0), p1.z0);
+            vy = rmul(static_cast<Real>(m.sd_vel0), p1.z1);
+        } else {
+            x = X[i];
+            y = Y[i];
+            vx = VX[i];
+            vy = VY[i];
+            lold = equal ? lw_equal : l[i] - c;
+            if (mon) {
+                const double w = exp(lold);
+                mx += w * static_cast<double>(x);
+                my += w * static_cast<double>(y);
+            }
+            const Real dt = static_cast<Real>(m.dt);
+            x = radd(x, radd(rmul(static_cast<Real>(m.sd_pos), p0.z0), rmul(dt, vx)));
+            y = radd(y, radd(rmul(static_cast<Real>(m.sd_pos), p0.z1), rmul(dt, vy)));
+            vx = radd(vx, rmul(static_cast<Real>(m.sd_vel), p1.z0));
+            vy = radd(vy, rmul(static_cast<Real>(m.sd_vel), p1.z1));
+        }
+        X[i] = x;
+        Y[i] = y;
+        VX[i] = vx;
+        VY[i] = vy;
+        const double ll = static_cast<double>(cv_loglik<Real>(m, x, y, ox, oy));
+        const double lnew = kInit ? ll : lold + ll;
+        l[i] = lnew;
+        acc = part_merge(acc, part_of(lnew));
+    }
+    block_partials(acc, part);
+    if (mon) {
+        __shared__ double lds[kWaves];
+        mx = block_reduce(mx, OpAddD(), lds);
+        my = block_reduce(my, OpAddD(), lds);
+        if (threadIdx.x == 0) {
+            mon[blockIdx.x * 8 + 0] = mx;
+            mon[blockIdx.x * 8 + 1] = my;
+        }
+    }
+}
+
+// ------------------------------------------------------------ host side
+
+static long long pblocks(long long n)
+{
+    const long long b = (n + kPTile - 1) / kPTile;
+    return b < 1 ? 1 : b;
+}
+
+int particle_create(ParticleSet **out, long long n, int dim, int bytes,
+    uint32_t seed, hipStream_t st)
+{
+    if (n < 1 || dim < 1 || dim > 64 || (bytes != 4 && bytes != 8)) {
+        set_error("particle_create: invalid configuration");
+        return kErrInvalid;
+    }
+    ParticleSet *p = new ParticleSet();
+    p->n = n;
+    p->dim = dim;
+    p->bytes = bytes;
+    p->seed = seed;
+    p->ws.stream = st;
+    const long long nb = pblocks(n);
+    bool ok = hipMalloc(&p->state, static_cast<size_t>(n) * dim * bytes) == hipSuccess;
+    ok = ok && hipMalloc(&p->logw, sizeof(double) * n) == hipSuccess;
+    ok = ok && hipMalloc(&p->scratch, sizeof(double) * n) == hipSuccess;
+    ok = ok && hipMalloc(&p->rep, sizeof(int) * n) == hipSuccess;
+    ok = ok && hipMalloc(&p->part, sizeof(Part3) * nb) == hipSuccess;
+    ok = ok && hipMalloc(&p->mpart, sizeof(double) * 8 * nb) == hipSuccess;
+    ok = ok && hipMalloc(&p->mout, sizeof(double) * 8) == hipSuccess;
+    ok = ok && hipMalloc(&p->pd, sizeof(PDev)) == hipSuccess;
+    ok = ok && p->ws.ensure(n);
+    if (!ok) {
+        particle_destroy(p);
+        set_error("particle_create: allocation failed");
+        return kErrAlloc;
+    }
+    PDev h{};
+    h.equal = 1;
+    h.log_equal = -log(static_cast<double>(n));
+    h.c = h.log_equal;
+    h.ess = static_cast<double>(n);
+    h.rkey = seed_key(seed, static_cast<unsigned long long>(n));
+    h.rword = 0;
+    if (hipMemcpy(p->pd, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(p->state, 0, static_cast<size_t>(n) * dim * bytes) != hipSuccess ||
+        hipMemset(p->logw, 0, sizeof(double) * n) != hipSuccess) {
+        particle_destroy(p);
+        set_error("particle_create: initialisation failed");
+        return kErrHip;
+    }
+    *out = p;
+    return kOk;
+}
+
+void particle_destroy(ParticleSet *p)
+{
+    if (!p)
+        return;
+    void *ps[] = {p->state, p->logw, p->scratch, p->rep, p->part, p->mpart,
+        p->mout, p->pd};
+    for (void *q : ps)
+        if (q)
+            (void)hipFree(q);
+    p->ws.release();
+    delete p;
+}
+
+int weight_update(ParticleSet *p, const double *src, int mode)
+{
+    hipStream_t st = p->ws.stream;
+    const long long nb = pblocks(p->n);
+    if (mode != 2)
+        hipLaunchKernelGGL(k_weight_pass, dim3(nb), dim3(kPB), 0, st, p->logw,
+            src, p->n, mode, p->pd, static_cast<Part3 *>(p->part));
+    hipLaunchKernelGGL(k_weight_finalize, dim3(1), dim3(kPB), 0, st, static_cast<const Part3 *>(p->part),
+        nb, p->n, mode, p->pd);
+    VSMCH_HIP(hipGetLastError());
+    return kOk;
+}
+
+int read_pdev(ParticleSet *p, PDev *h)
+{
+    VSMCH_HIP(hipMemcpyAsync(h, p->pd, sizeof(PDev), hipMemcpyDeviceToHost,
+        p->ws.stream));
+    VSMCH_HIP(hipStreamSynchronize(p->ws.stream));
+    return kOk;
+}
+
+int weight_read(ParticleSet *p, double *W)
+{
+    hipLaunchKernelGGL(k_weight_read, dim3((p->n + 255) / 256), dim3(256), 0,
+        p->ws.stream, p->logw, p->n, p->pd, W);
+    VSMCH_HIP(hipGetLastError());
+    return kOk;
+}
+
+// One resampling event driven from the device decision (no host sync).
+int resample_step(ParticleSet *p, int scheme, double threshold, long long t,
+    double *ess_hist, int *rs_hist, int *rep_out, int *idx_out)
+{
+    hipStream_t st = p->ws.stream;
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(1), 0, st, p->pd, p->ws.prm,
+        p->n, threshold, t, ess_hist, rs_hist);
+    VSMCH_HIP(hipGetLastError());
+    int *rep = rep_out ? rep_out : p->rep;
+    int rc = resample_scheme(p->ws, scheme, p->n, p->n, 0, 0, p->logw,
+        kWeightLog, rep, true);
+    if (rc != kOk)
+        return rc;
+    rc = trans_rep_index(p->ws, p->n, p->n, rep, idx_out, p->state, p->dim,
+        p->n, p->bytes, true, false);
+    if (rc != kOk)
+        return rc;
+    hipLaunchKernelGGL(k_after_resample, dim3(1), dim3(1), 0, st, p->pd,
+        p->ws.prm, p->n);
+    VSMCH_HIP(hipGetLastError());
+    return kOk;
+}
+
+int moments(ParticleSet *p, int c0, int dim, double *out_dev)
+{
+    if (dim < 1 || dim > 8 || c0 < 0 || c0 + dim > p->dim) {
+        set_error("moments: components out of range (at most 8)");
+        return kErrInvalid;
+    }
+    hipStream_t st = p->ws.stream;
+    const long long nb = pblocks(p->n);
+    if (p->bytes == 4)
+        hipLaunchKernelGGL(k_moments<float>, dim3(nb), dim3(kPB), 0, st,
+            static_cast<const float *>(p->state), p->logw, p->n, c0, dim,
+            p->pd, p->mpart);
+    else
+        hipLaunchKernelGGL(k_moments<double>, dim3(nb), dim3(kPB), 0, st,
+            static_cast<const double *>(p->state), p->logw, p->n, c0, dim,
+            p->pd, p->mpart);
+    hipLaunchKernelGGL(k_moments_finalize, dim3(1), dim3(kPB), 0, st,
+        p->mpart, nb, dim, out_dev);
+    VSMCH_HIP(hipGetLastError());
+    return kOk;
+}
+
+int cv_step(ParticleSet *p, const CvModelDev &m, long long t, double ox,
+    double oy, bool with_monitor)
+{
+    if (p->dim != 4) {
+        set_error("cv model needs dim == 4");
+        return kErrInvalid;
+    }
+    hipStream_t st = p->ws.stream;
+    const long long nb = pblocks(p->n);
+    double *mon = with_monitor ? p->mpart : nullptr;
+    if (p->bytes == 4) {
+        if (t == 0)
+            hipLaunchKernelGGL((k_cv_step<float, true>), dim3(nb), dim3(kPB), 0,
+                st, static_cast<float *>(p->state), p->logw, p->n, p->seed, t,
+                static_cast<float>(ox), static_cast<float>(oy), m, p->pd,
+                static_cast<Part3 *>(p->part), static_cast<double *>(nullptr));
+        else
+            hipLaunchKernelGGL((k_cv_step<float, false>), dim3(nb), dim3(kPB),
+                0, st, static_cast<float *>(p->state), p->logw, p->n, p->seed,
+                t, static_cast<float>(ox), static_cast<float>(oy), m, p->pd,
+                static_cast<Part3 *>(p->part), mon);
+    } else {
+        if (t == 0)
+            hipLaunchKernelGGL((k_cv_step<double, true>), dim3(nb), dim3(kPB),
+                0, st, static_cast<double *>(p->state), p->logw, p->n, p->seed,
+                t, ox, oy, m, p->pd, static_cast<Part3 *>(p->part), static_cast<double *>(nullptr));
+        else
+            hipLaunchKernelGGL((k_cv_step<double, false>), dim3(nb), dim3(kPB),
+                0, st, static_cast<double *>(p->state), p->logw, p->n, p->seed,
+                t, ox, oy, m, p->pd, static_cast<Part3 *>(p->part), mon);
+    }
+    hipLaunchKernelGGL(k_weight_finalize, dim3(1), dim3(kPB), 0, st, static_cast<const Part3 *>(p->part),
+        nb, p->n, t == 0 ? 0 : 1, p->pd);
+    VSMCH_HIP(hipGetLastError());
+    return kOk;
+}
+
+__global__ void k_store_scalar(const PDev *pd, double *logz_hist, long long t)
+{
+    logz_hist[t] = pd->logz;
+}
+
+int cv_run(ParticleSet *p, const CvModelDev &m, long long T,
+    const double *ox, const double *oy, int scheme, double threshold,
+    double *ess_d, int *rs_d, double *pos_d, double *logz_d)
+{
+    hipStream_t st = p->ws.stream;
+    const long long nb = pblocks(p->n);
+    int rc;
+    for (long long t = 0; t < T; ++t) {
+        // the move of step t also yields the "pos" record of step t - 1
+        rc = cv_step(p, m, t, ox[t], oy[t], pos_d != nullptr && t > 0);
+        if (rc != kOk)
+            return rc;
+        if (pos_d && t > 0)
+            hipLaunchKernelGGL(k_moments_finalize, dim3(1), dim3(kPB), 0, st,
+                p->mpart, nb, 2, pos_d + 2 * (t - 1));
+        if (logz_d)
+            hipLaunchKernelGGL(k_store_scalar, dim3(1), dim3(1), 0, st, p->pd,
+                logz_d, t);
+        rc = resample_step(p, scheme, threshold, t, ess_d, rs_d, nullptr,
+            nullptr);
+        if (rc != kOk)
+            return rc;
+    }
+    if (pos_d && T > 0) {
+        rc = moments(p, 0, 2, pos_d + 2 * (T - 1));
+        if (rc != kOk)
+            return rc;
+    }
+    VSMCH_HIP(hipGetLastError());
+    return kOk;
+}
+
+} // namespace vsmch

@@ -1,0 +1,73 @@
+// particle.hpp -- device particle set shared by particle.hip and capi.hip.
+#pragma once
+
+#include "internal.hpp"
+
+namespace vsmch {
+
+// Device-resident weight and stream state of one particle set.
+struct PDev {
+    double lmax;       // max of l
+    double sum;        // sum exp(l - lmax)
+    double c;          // log W_i = l_i - c (when !equal)
+    double ess;
+    double logz;       // accumulated log-evidence
+    double logz_inc;
+    double log_equal;  // -log N
+    unsigned long long rword;  // resampling stream: next unread word
+    unsigned int rkey;         // resampling stream key
+    long long nresampled;      // resampling events so far
+    int equal;                 // weights are 1/N (l ignored)
+    int resampled;             // last decision
+};
+
+// Model constants of the constant-velocity PF, precomputed on the host.
+struct CvModelDev {
+    double dt;
+    double sd_pos, sd_vel;
+    double sd_pos0, sd_vel0;
+    double obs_scale, inv_scale, obs_df;
+    double ll_const;   // normalising constant added for obs_model 1, 2
+    int obs_model;
+    int pad;
+};
+
+struct ParticleSet {
+    long long n = 0;
+    int dim = 0;
+    int bytes = 0;
+    uint32_t seed = 0;
+    void *state = nullptr;
+    double *logw = nullptr;
+    double *scratch = nullptr;
+    int *rep = nullptr;
+    void *part = nullptr;
+    double *mpart = nullptr;
+    double *mout = nullptr;
+    PDev *pd = nullptr;
+    Workspace ws;
+};
+
+int particle_create(ParticleSet **out, long long n, int dim, int bytes,
+    uint32_t seed, hipStream_t st);
+void particle_destroy(ParticleSet *p);
+int weight_update(ParticleSet *p, const double *src, int mode);
+int read_pdev(ParticleSet *p, PDev *h);
+int weight_read(ParticleSet *p, double *W);
+int resample_step(ParticleSet *p, int scheme, double threshold, long long t,
+    double *ess_hist, int *rs_hist, int *rep_out, int *idx_out);
+int moments(ParticleSet *p, int c0, int dim, double *out_dev);
+int cv_step(ParticleSet *p, const CvModelDev &m, long long t, double ox,
+    double oy, bool with_monitor);
+int cv_run(ParticleSet *p, const CvModelDev &m, long long T,
+    const double *ox, const double *oy, int scheme, double threshold,
+    double *ess_d, int *rs_d, double *pos_d, double *logz_d);
+
+// rng.hip
+int stream_fill(int kind, unsigned long long key, unsigned long long offset,
+    long long n, void *out, int mode, hipStream_t st);
+int normal_fill(int kind, unsigned long long key, unsigned long long offset,
+    long long n, double mean, double sd, int form, double *out,
+    hipStream_t st);
+
+} // namespace vsmch

@@ -1,0 +1,1477 @@
+// resample.hip -- replication counts, rep -> index and in-place gather.
+//
+// Reference: include/vsmc/resample/transform.hpp (resample_trans_*),
+// include/vsmc/resample/{multinomial,stratified,systematic,residual,
+// residual_stratified,residual_systematic}.hpp, include/vsmc/rng/
+// u01_sequence.hpp (the uniform sequences), include/vsmc/core/
+// state_matrix.hpp:246-262 (copy).
+//
+// Counts: with s_i the sequential cumulative weight (exact, fold.hpp) and
+// v_j the running maximum of the uniforms, the merge loop of
+// transform.hpp:61-70 leaves j = G(s_i) = #{j : v_j <= s_i} after particle
+// i, so rep_i = G(s_i) - G(s_{i-1}) and rep_{M-1} = N - G(s_{M-2}).
+// G comes from closed forms for systematic and stratified uniforms and from
+// a search over the materialised sequence otherwise.
+#include "internal.hpp"
+#include "../../include/vsmch/device_rng.hpp"
+
+#include <mutex>
+#include <vector>
+
+namespace vsmch {
+
+using fold::FMap;
+using fold::TileSeg;
+
+// ------------------------------------------------------------ sources
+
+struct WArray {
+    const double *w;
+    __device__ double operator()(long long i) const { return w[i]; }
+    __device__ long long integ(long long) const { return 0; }
+};
+
+// W_i = inv_sum * exp(l_i - lmax) (Weight::normalize, weight.hpp:208-222)
+struct WLog {
+    const double *l;
+    const ResampleDev *p;
+    __device__ double operator()(long long i) const
+    {
+        if (p->equal)
+            return 1.0 / static_cast<double>(p->M);
+        return p->inv_sum * exp(l[i] - p->lmax);
+    }
+    __device__ long long integ(long long) const { return 0; }
+};
+
+// resid_i = modf(N * W_i) (transform.hpp:159-161)
+template <class B>
+struct WResidRaw {
+    B b;
+    double coeff;
+    __device__ double operator()(long long i) const
+    {
+        double ip;
+        return modf(coeff * b(i), &ip);
+    }
+    __device__ long long integ(long long i) const
+    {
+        double ip;
+        modf(coeff * b(i), &ip);
+        return static_cast<long long>(ip);
+    }
+};
+
+// normalised residual inv_resid * resid_i (transform.hpp:163-164); zero
+// when no residual draws remain.
+template <class B>
+struct WResid {
+    B b;
+    double coeff;
+    const ResampleDev *p;
+    __device__ double operator()(long long i) const
+    {
+        if (p->ndraw <= 0)
+            return 0.0;
+        double ip;
+        const double r = modf(coeff * b(i), &ip);
+        return p->inv_resid * r;
+    }
+    __device__ long long integ(long long i) const
+    {
+        double ip;
+        modf(coeff * b(i), &ip);
+        return static_cast<long long>(ip);
+    }
+};
+
+// Sequence terms of U01SequenceSorted (u01_sequence.hpp:149-151), negated:
+// a_k = -(log(U_k) / (n - k)) >= 0, so the fold of a is -lmax.
+struct WSortedTerms {
+    const ResampleDev *p;
+    __device__ double operator()(long long k) const
+    {
+        const double u = u01_u32d(philox_word(p->key, p->word0 + k));
+        return -(log(u) / static_cast<double>(p->ndraw - k));
+    }
+    __device__ long long integ(long long) const { return 0; }
+};
+
+// U01SequenceSystematic: u_j = u0 + j * delta (u01_sequence.hpp:246), no FMA.
+struct USys {
+    const ResampleDev *p;
+    __device__ double u(long long j) const
+    {
+        return __dadd_rn(p->u0, __dmul_rn(static_cast<double>(j), p->delta));
+    }
+    __device__ long long ndraw() const { return p->ndraw; }
+    __device__ long long count(double x) const
+    {
+        const long long n = p->ndraw;
+        if (n <= 0)
+            return 0;
+        if (!(x < INFINITY))
+            return n;
+        const double est = (x - p->u0) * static_cast<double>(n);
+        long long g;
+        if (!(est > 0.0))
+            g = 0;
+        else if (est >= static_cast<double>(n))
+            g = n;
+        else
+            g = static_cast<long long>(est) + 1;
+        while (g > 0 && u(g - 1) > x)
+            --g;
+        while (g < n && u(g) <= x)
+            ++g;
+        return g;
+    }
+};
+
+// U01SequenceStratified: u_j = U_j * delta + j * delta (u01_sequence.hpp:200),
+// no FMA; v_j = max(u_{j-1}, u_j) is the running maximum because
+// u_{j-2} < fl(j * delta) <= u_j.
+struct UStrat {
+    const ResampleDev *p;
+    __device__ double u(long long j) const
+    {
+        const double U = u01_u32d(philox_word(p->key, p->word0 + j));
+        return __dadd_rn(__dmul_rn(U, p->delta),
+            __dmul_rn(static_cast<double>(j), p->delta));
+    }
+    __device__ double v(long long j) const
+    {
+        const double a = u(j);
+        if (j == 0)
+            return a;
+        const double b = u(j - 1);
+        return b > a ? b : a;
+    }
+    __device__ long long ndraw() const { return p->ndraw; }
+    __device__ long long count(double x) const
+    {
+        const long long n = p->ndraw;
+        if (n <= 0)
+            return 0;
+        if (!(x < INFINITY))
+            return n;
+        const double est = x * static_cast<double>(n);
+        long long g;
+        if (!(est > 0.0))
+            g = 0;
+        else if (est >= static_cast<double>(n))
+            g = n;
+        else
+            g = static_cast<long long>(est);
+        while (g > 0 && v(g - 1) > x)
+            --g;
+        while (g < n && v(g) <= x)
+            ++g;
+        return g;
+    }
+};
+
+// Materialised non-decreasing sequence v (running maximum of the uniforms).
+struct UArr {
+    const double *v;
+    const ResampleDev *p;
+    __device__ long long ndraw() const { return p->ndraw; }
+    __device__ long long count(double x) const
+    {
+        long long lo = 0, hi = p->ndraw;
+        while (lo < hi) {
+            const long long mid = (lo + hi) >> 1;
+            if (v[mid] <= x)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        return lo;
+    }
+};
+
+__device__ inline long long eff_n(const long long *ndev, long long nhost)
+{
+    return ndev ? *ndev : nhost;
+}
+
+__device__ inline long long ceil_div(long long a, long long b)
+{
+    return (a + b - 1) / b;
+}
+
+// ---------------------------------------------------------- fold passes
+
+template <class WSrc>
+__global__ void __launch_bounds__(kBlock) k_tile_sum(WSrc src, long long nh,
+    const long long *ndev, const ResampleDev *prm, double *tsum, int *err)
+{
+    if (!prm->active)
+        return;
+    const long long n = eff_n(ndev, nh);
+    const long long base = static_cast<long long>(blockIdx.x) * kTile;
+    if (base >= n)
+        return;
+    __shared__ double lds[kWaves];
+    double acc = 0.0;
+    int bad = 0;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const long long g = base + j * kBlock + threadIdx.x;
+        if (g < n) {
+            const double x = src(g);
+            if (!(x >= 0.0))
+                bad = 1;
+            acc += x;
+        }
+    }
+    const double tot = block_reduce(acc, OpAddD(), lds);
+    if (threadIdx.x == 0)
+        tsum[blockIdx.x] = tot;
+    if (bad)
+        atomicOr(err, 1);
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_tiles(const double *tsum,
+    long long nh, const long long *ndev, const ResampleDev *prm, double *tpre)
+{
+    if (!prm->active)
+        return;
+    const long long n = eff_n(ndev, nh);
+    const long long ntiles = ceil_div(n, kTile);
+    __shared__ double lds[kWaves];
+    double carry = 0.0;
+    for (long long c = 0; c < ntiles; c += kBlock) {
+        const long long t = c + threadIdx.x;
+        const double v = t < ntiles ? tsum[t] : 0.0;
+        double tot;
+        const double ex = block_exclusive_scan(v, OpAddD(), 0.0, lds, &tot);
+        if (t < ntiles)
+            tpre[t] = carry + ex;
+        carry += tot;
+    }
+}
+
+template <class WSrc>
+__global__ void __launch_bounds__(kBlock) k_segments(WSrc src, long long nh,
+    const long long *ndev, const ResampleDev *prm, const double *tpre,
+    TileSeg *segs)
+{
+    if (!prm->active)
+        return;
+    const long long n = eff_n(ndev, nh);
+    const long long base = static_cast<long long>(blockIdx.x) * kTile;
+    if (base >= n)
+        return;
+    __shared__ fold::SegShared sh;
+    double w[kItems];
+    load_tile_blocked(src, base, n, w, sh.tile, 0.0);
+    const long long rem = n - base;
+    const int nvalid = rem < kTile ? static_cast<int>(rem) : kTile;
+    fold::tile_segments(w, nvalid, blockIdx.x == 0 ? 0.0 : tpre[blockIdx.x],
+        &segs[blockIdx.x], sh);
+}
+
+// One wave per group of 64 tiles: flat when every tile is a single run in
+// the same binade.
+__global__ void __launch_bounds__(kWave) k_groups(long long nh,
+    const long long *ndev, const ResampleDev *prm, const TileSeg *segs,
+    GroupSeg *groups)
+{
+    if (!prm->active)
+        return;
+    const long long n = eff_n(ndev, nh);
+    const long long ntiles = ceil_div(n, kTile);
+    const long long t = static_cast<long long>(blockIdx.x) * kGroup + threadIdx.x;
+    if (static_cast<long long>(blockIdx.x) * kGroup >= ntiles)
+        return;
+    const bool valid = t < ntiles;
+    int nc = 0, e = fold::kEmpty;
+    FMap m = fold::ident();
+    if (valid) {
+        nc = segs[t].ncross;
+        e = segs[t].e[0];
+        m = segs[t].map[0];
+    }
+    const int e0 = __shfl(e, 0, kWave);
+    const bool ok = !valid || (nc == 0 && e == e0 && e != fold::kEmpty);
+    const unsigned long long bal = __ballot(ok);
+    const FMap P = wave_inclusive_scan(m, fold::OpCompose());
+    const FMap tot = shfl_any(P, kWave - 1);
+    if (threadIdx.x == 0) {
+        GroupSeg gs;
+        gs.flat = (bal == ~0ull) ? 1 : 0;
+        gs.e = e0;
+        gs.map = tot;
+        groups[blockIdx.x] = gs;
+    }
+}
+
+struct ResolveShared {
+    double tile[kTileLds];
+    fold::ExactShared ex;
+    double bx;
+    int bL;
+    int bok;
+};
+
+// Batched advance over up to 64 items (groups or tiles) whose summaries are
+// single maps in binade binade(x): returns how many were applied, writes the
+// start of each into starts[], and updates x. Called by wave 0 only.
+template <class Get>
+__device__ inline int wave_advance(double x, long long first, long long count,
+    Get get, double *starts, int *modes, double *xnew)
+{
+    const int lane = threadIdx.x;
+    const bool xpos = x > 0.0 && x < INFINITY;
+    const int ex = xpos ? fold::binade(x) : fold::kEmpty;
+    const long long it = first + lane;
+    bool ok = false;
+    FMap m = fold::ident();
+    if (xpos && lane < count) {
+        int e;
+        FMap mm;
+        if (get(it, e, mm) && e == ex) {
+            ok = true;
+            m = mm;
+        }
+    }
+    const unsigned long long bal = __ballot(ok);
+    const int L = (~bal == 0ull) ? kWave : (__ffsll(~bal) - 1);
+    const FMap P = wave_inclusive_scan(m, fold::OpCompose());
+    const long long a0 = xpos ? fold::mant(x) : 0;
+    const long long a = a0 + ((a0 & 1) ? P.d1 : P.d0);
+    const bool over = lane < L && a >= fold::kTwo53;
+    const unsigned long long bo = __ballot(over);
+    int L2 = L;
+    if (bo != 0ull) {
+        const int f = __ffsll(bo) - 1;
+        L2 = f < L2 ? f : L2;
+    }
+    const double val = (lane < L2) ? fold::from_mant(a, ex) : 0.0;
+    double prevv = shfl_up_any(val, 1);
+    if (lane == 0)
+        prevv = x;
+    if (lane < L2) {
+        starts[it] = prevv;
+        if (modes)
+            modes[it] = 1;
+    }
+    const double last = shfl_any(val, L2 > 0 ? L2 - 1 : 0);
+    *xnew = L2 > 0 ? last : x;
+    return L2;
+}
+
+template <class WSrc>
+__global__ void __launch_bounds__(kBlock) k_resolve(WSrc src, long long nh,
+    const long long *ndev, const ResampleDev *prm, const TileSeg *segs,
+    const GroupSeg *groups, double *xstart, double *xgstart, int *gmode)
+{
+    if (!prm->active)
+        return;
+    const long long n = eff_n(ndev, nh);
+    const long long ntiles = ceil_div(n, kTile);
+    const long long ngroups = ceil_div(ntiles, kGroup);
+    __shared__ ResolveShared sh;
+    double x = 0.0;
+    long long g = 0;
+    while (g < ngroups) {
+        if (threadIdx.x < kWave) {
+            double xn;
+            auto get = [&](long long gg, int &e, FMap &m) {
+                const GroupSeg gs = groups[gg];
+                e = gs.e;
+                m = gs.map;
+                return gs.flat != 0;
+            };
+            const int L = wave_advance(x, g, ngroups - g, get, xgstart, gmode,
+                &xn);
+            if (threadIdx.x == 0) {
+                sh.bL = L;
+                sh.bx = xn;
+            }
+        }
+        __syncthreads();
+        const int L = sh.bL;
+        x = sh.bx;
+        __syncthreads();
+        if (L > 0) {
+            g += L;
+            continue;
+        }
+        // group g walked tile by tile
+        if (threadIdx.x == 0) {
+            gmode[g] = 0;
+            xgstart[g] = x;
+        }
+        long long t = g * kGroup;
+        const long long tend = (t + kGroup < ntiles) ? t + kGroup : ntiles;
+        while (t < tend) {
+            if (threadIdx.x < kWave) {
+                double xn;
+                auto get = [&](long long tt, int &e, FMap &m) {
+                    const int nc = segs[tt].ncross;
+                    e = segs[tt].e[0];
+                    m = segs[tt].map[0];
+                    return nc == 0;
+                };
+                const int L2 = wave_advance(x, t, tend - t, get, xstart,
+                    static_cast<int *>(nullptr), &xn);
+                if (threadIdx.x == 0) {
+                    sh.bL = L2;
+                    sh.bx = xn;
+                }
+            }
+            __syncthreads();
+            const int L2 = sh.bL;
+            x = sh.bx;
+            __syncthreads();
+            if (L2 > 0) {
+                t += L2;
+                continue;
+            }
+            if (threadIdx.x == 0) {
+                xstart[t] = x;
+                double y = x;
+                sh.bok = fold::apply_segments(segs[t], y) ? 1 : 0;
+                sh.bx = y;
+            }
+            __syncthreads();
+            const int ok = sh.bok;
+            double y = sh.bx;
+            __syncthreads();
+            if (!ok) {
+                const long long base = t * kTile;
+                double w[kItems], s[kItems];
+                load_tile_blocked(src, base, n, w, sh.tile, 0.0);
+                fold::tile_exact(w, x, s, sh.ex);
+                const long long rem = n - base;
+                const int last = (rem < kTile ? static_cast<int>(rem) : kTile) - 1;
+#pragma unroll
+                for (int j = 0; j < kItems; ++j)
+                    if (static_cast<int>(threadIdx.x) * kItems + j == last)
+                        sh.bx = s[j];
+                __syncthreads();
+                y = sh.bx;
+                __syncthreads();
+            }
+            x = y;
+            ++t;
+        }
+        ++g;
+    }
+    if (threadIdx.x == 0) {
+        xstart[ntiles] = x;
+        xgstart[ngroups] = x;
+    }
+}
+
+// Exact start of tile t after the resolver.
+__device__ inline double tile_start(long long t, const double *xstart,
+    const double *xgstart, const int *gmode, const TileSeg *segs)
+{
+    const long long g = t / kGroup;
+    if (gmode[g]) {
+        const double x = xgstart[g];
+        const long long t0 = g * kGroup;
+        if (t == t0)
+            return x;
+        FMap P = fold::ident();
+        for (long long tt = t0; tt < t; ++tt)
+            P = fold::compose(P, segs[tt].map[0]);
+        const long long a0 = fold::mant(x);
+        return fold::from_mant(a0 + ((a0 & 1) ? P.d1 : P.d0), fold::binade(x));
+    }
+    return xstart[t];
+}
+
+struct FoldBufs {
+    const double *xstart;
+    const double *xgstart;
+    const int *gmode;
+    const TileSeg *segs;
+};
+
+// Replication counts (and residual integer parts) for one tile.
+template <class WSrc, class USrc>
+__global__ void __launch_bounds__(kBlock) k_count(WSrc src, USrc us,
+    long long nh, const long long *ndev, const ResampleDev *prm, FoldBufs fb,
+    int *rep)
+{
+    if (!prm->active)
+        return;
+    const long long M = eff_n(ndev, nh);
+    const long long base = static_cast<long long>(blockIdx.x) * kTile;
+    if (base >= M)
+        return;
+    __shared__ double tile[kTileLds];
+    __shared__ fold::ExactShared ex;
+    __shared__ long long lastG[kBlock];
+    __shared__ double x0s;
+    double w[kItems], s[kItems];
+    load_tile_blocked(src, base, M, w, tile, 0.0);
+    if (threadIdx.x == 0)
+        x0s = tile_start(blockIdx.x, fb.xstart, fb.xgstart, fb.gmode, fb.segs);
+    __syncthreads();
+    const double x0 = x0s;
+    fold::tile_exact(w, x0, s, ex);
+    const long long ib = base + static_cast<long long>(threadIdx.x) * kItems;
+    const long long nd = us.ndraw();
+    long long G[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const long long i = ib + j;
+        G[j] = 0;
+        if (i < M)
+            G[j] = (i == M - 1) ? nd : us.count(s[j]);
+    }
+    lastG[threadIdx.x] = G[kItems - 1];
+    __syncthreads();
+    long long gp;
+    if (threadIdx.x == 0)
+        gp = (blockIdx.x == 0) ? 0 : us.count(x0);
+    else
+        gp = lastG[threadIdx.x - 1];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const long long i = ib + j;
+        if (i < M) {
+            const long long r = G[j] - gp + src.integ(i);
+            gp = G[j];
+            rep[i] = static_cast<int>(r);
+        }
+    }
+}
+
+// Writes f(s_i) for the exact prefix s (tests; sorted uniforms).
+struct EpiRaw {
+    double *out;
+    __device__ void operator()(long long i, double s) const { out[i] = s; }
+};
+// u_j = 1 - exp(lmax_j) with lmax_j = -s_j (u01_sequence.hpp:151)
+struct EpiSorted {
+    double *out;
+    __device__ void operator()(long long i, double s) const
+    {
+        out[i] = 1.0 - exp(-s);
+    }
+};
+
+template <class WSrc, class Epi>
+__global__ void __launch_bounds__(kBlock) k_prefix(WSrc src, long long nh,
+    const long long *ndev, const ResampleDev *prm, FoldBufs fb, Epi epi)
+{
+    if (!prm->active)
+        return;
+    const long long n = eff_n(ndev, nh);
+    const long long base = static_cast<long long>(blockIdx.x) * kTile;
+    if (base >= n)
+        return;
+    __shared__ double tile[kTileLds];
+    __shared__ fold::ExactShared ex;
+    __shared__ double x0s;
+    double w[kItems], s[kItems];
+    load_tile_blocked(src, base, n, w, tile, 0.0);
+    if (threadIdx.x == 0)
+        x0s = tile_start(blockIdx.x, fb.xstart, fb.xgstart, fb.gmode, fb.segs);
+    __syncthreads();
+    fold::tile_exact(w, x0s, s, ex);
+    const long long ib = base + static_cast<long long>(threadIdx.x) * kItems;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+        if (ib + j < n)
+            epi(ib + j, s[j]);
+}
+
+// ------------------------------------------------- running maximum (v)
+
+__global__ void __launch_bounds__(kBlock) k_tile_max(const double *u,
+    long long nh, const long long *ndev, const ResampleDev *prm, double *tmax)
+{
+    if (!prm->active)
+        return;
+    const long long n = eff_n(ndev, nh);
+    const long long base = static_cast<long long>(blockIdx.x) * kTile;
+    if (base >= n)
+        return;
+    __shared__ double lds[kWaves];
+    double m = -INFINITY;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const long long g = base + j * kBlock + threadIdx.x;
+        if (g < n) {
+            const double x = u[g];
+            m = x > m ? x : m;
+        }
+    }
+    m = block_reduce(m, OpMaxD(), lds);
+    if (threadIdx.x == 0)
+        tmax[blockIdx.x] = m;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_max(const double *tmax,
+    long long nh, const long long *ndev, const ResampleDev *prm, double *tpre)
+{
+    if (!prm->active)
+        return;
+    const long long n = eff_n(ndev, nh);
+    const long long ntiles = ceil_div(n, kTile);
+    __shared__ double lds[kWaves];
+    double carry = -INFINITY;
+    for (long long c = 0; c < ntiles; c += kBlock) {
+        const long long t = c + threadIdx.x;
+        const double v = t < ntiles ? tmax[t] : -INFINITY;
+        double tot;
+        const double ex = block_exclusive_scan(v, OpMaxD(), -__builtin_huge_val(), lds, &tot);
+        if (t < ntiles)
+            tpre[t] = carry > ex ? carry : ex;
+        carry = carry > tot ? carry : tot;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_apply_max(const double *u,
+    long long nh, const long long *ndev, const ResampleDev *prm,
+    const double *tpre, double *v)
+{
+    if (!prm->active)
+        return;
+    const long long n = eff_n(ndev, nh);
+    const long long base = static_cast<long long>(blockIdx.x) * kTile;
+    if (base >= n)
+        return;
+    __shared__ double tile[kTileLds];
+    __shared__ double lds[kWaves];
+    double w[kItems];
+    load_tile_blocked(WArray{u}, base, n, w, tile, -INFINITY);
+    double m = -INFINITY;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+        m = w[j] > m ? w[j] : m;
+    double tot;
+    double run = block_exclusive_scan(m, OpMaxD(), -__builtin_huge_val(), lds, &tot);
+    const double tp = tpre[blockIdx.x];
+    run = run > tp ? run : tp;
+    const long long ib = base + static_cast<long long>(threadIdx.x) * kItems;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        run = w[j] > run ? w[j] : run;
+        if (ib + j < n)
+            v[ib + j] = run;
+    }
+}
+
+// ------------------------------------------------------ rep -> index
+
+// Per-tile aggregates of rep: (childless slots, extra copies, sources).
+// Expand mode (M != N): (unused, copies, sources with copies).
+template <bool kExpand>
+__global__ void __launch_bounds__(kBlock) k_ridx_agg(const int *rep,
+    long long M, const ResampleDev *prm, long long *tagg)
+{
+    if (prm && !prm->active)
+        return;
+    const long long base = static_cast<long long>(blockIdx.x) * kTile;
+    __shared__ long long lds[kWaves];
+    long long z = 0, e = 0, c = 0;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const long long g = base + j * kBlock + threadIdx.x;
+        if (g < M) {
+            const long long r = rep[g];
+            if (kExpand) {
+                e += r;
+                c += r > 0;
+            } else {
+                z += r == 0;
+                e += r > 1 ? r - 1 : 0;
+                c += r > 1;
+            }
+        }
+    }
+    z = block_reduce(z, OpAddL(), lds);
+    e = block_reduce(e, OpAddL(), lds);
+    c = block_reduce(c, OpAddL(), lds);
+    if (threadIdx.x == 0) {
+        tagg[3 * blockIdx.x + 0] = z;
+        tagg[3 * blockIdx.x + 1] = e;
+        tagg[3 * blockIdx.x + 2] = c;
+    }
+}
+
+struct L3 {
+    long long a, b, c;
+};
+struct OpAddL3 {
+    __device__ L3 operator()(L3 x, L3 y) const
+    {
+        return L3{x.a + y.a, x.b + y.b, x.c + y.c};
+    }
+};
+
+__global__ void __launch_bounds__(kBlock) k_ridx_scan(const long long *tagg,
+    long long ntiles, const ResampleDev *prm, long long *toff)
+{
+    if (prm && !prm->active)
+        return;
+    __shared__ L3 lds[kWaves];
+    L3 carry{0, 0, 0};
+    for (long long c0 = 0; c0 < ntiles; c0 += kBlock) {
+        const long long t = c0 + threadIdx.x;
+        L3 v{0, 0, 0};
+        if (t < ntiles)
+            v = L3{tagg[3 * t], tagg[3 * t + 1], tagg[3 * t + 2]};
+        L3 tot;
+        const L3 ex = block_exclusive_scan(v, OpAddL3(), L3{0, 0, 0}, lds, &tot);
+        if (t < ntiles) {
+            toff[3 * t + 0] = carry.a + ex.a;
+            toff[3 * t + 1] = carry.b + ex.b;
+            toff[3 * t + 2] = carry.c + ex.c;
+        }
+        carry = OpAddL3()(carry, tot);
+    }
+    if (threadIdx.x == 0) {
+        toff[3 * ntiles + 0] = carry.a;
+        toff[3 * ntiles + 1] = carry.b;
+        toff[3 * ntiles + 2] = carry.c;
+    }
+}
+
+template <bool kExpand>
+__global__ void __launch_bounds__(kBlock) k_ridx_scatter(const int *rep,
+    long long M, const ResampleDev *prm, const long long *toff, int *zdst,
+    int *cE, int *csrc, int *idx)
+{
+    if (prm && !prm->active)
+        return;
+    const long long base = static_cast<long long>(blockIdx.x) * kTile;
+    __shared__ L3 lds[kWaves];
+    __shared__ int tile[kTile];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const long long g = base + j * kBlock + threadIdx.x;
+        tile[j * kBlock + threadIdx.x] = g < M ? rep[g] : 0;
+    }
+    __syncthreads();
+    int r[kItems];
+    L3 loc{0, 0, 0};
+    const long long ib = base + static_cast<long long>(threadIdx.x) * kItems;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        r[j] = tile[threadIdx.x * kItems + j];
+        const bool valid = ib + j < M;
+        const long long rr = valid ? r[j] : 0;
+        if (kExpand) {
+            loc.b += rr;
+            loc.c += rr > 0;
+        } else {
+            loc.a += valid && rr == 0;
+            loc.b += rr > 1 ? rr - 1 : 0;
+            loc.c += rr > 1;
+        }
+    }
+    L3 tot;
+    L3 ex = block_exclusive_scan(loc, OpAddL3(), L3{0, 0, 0}, lds, &tot);
+    long long zr = toff[3 * blockIdx.x + 0] + ex.a;
+    long long er = toff[3 * blockIdx.x + 1] + ex.b;
+    long long cr = toff[3 * blockIdx.x + 2] + ex.c;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const long long i = ib + j;
+        if (i >= M)
+            break;
+        const long long rr = r[j];
+        if (kExpand) {
+            if (rr > 0) {
+                cE[cr] = static_cast<int>(er);
+                csrc[cr] = static_cast<int>(i);
+                ++cr;
+                er += rr;
+            }
+        } else {
+            if (rr == 0) {
+                zdst[zr] = static_cast<int>(i);
+                ++zr;
+            } else {
+                if (idx)
+                    idx[i] = static_cast<int>(i);
+                if (rr > 1) {
+                    cE[cr] = static_cast<int>(er);
+                    csrc[cr] = static_cast<int>(i);
+                    ++cr;
+                    er += rr - 1;
+                }
+            }
+        }
+    }
+}
+
+__device__ inline long long upper_bound_i(const int *a, long long n, long long x)
+{
+    long long lo = 0, hi = n;
+    while (lo < hi) {
+        const long long mid = (lo + hi) >> 1;
+        if (a[mid] <= x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Source of each rank k (the k-th childless slot, or the k-th output in
+// expand mode): the source whose copies cover k. Optionally gathers an SoA
+// state in place: x[d][dst] = x[d][src] (sources are never destinations).
+template <bool kExpand, typename E>
+__global__ void __launch_bounds__(kBlock) k_expand(const ResampleDev *prm,
+    const long long *toff, long long ntiles, const int *zdst, const int *cE,
+    const int *csrc, int *idx, E *state, int dim, long long ld)
+{
+    if (prm && !prm->active)
+        return;
+    const long long K = kExpand ? toff[3 * ntiles + 1] : toff[3 * ntiles + 0];
+    const long long C = toff[3 * ntiles + 2];
+    const long long k0 = static_cast<long long>(blockIdx.x) * kTile;
+    if (k0 >= K)
+        return;
+    if (C <= 0)
+        return;
+    const long long kl = (k0 + kTile < K ? k0 + kTile : K) - 1;
+    __shared__ int win[kTile + 1];
+    __shared__ long long clo_s, chi_s;
+    if (threadIdx.x == 0) {
+        long long lo = upper_bound_i(cE, C, k0) - 1;
+        long long hi = upper_bound_i(cE, C, kl) - 1;
+        clo_s = lo < 0 ? 0 : lo;
+        chi_s = hi < 0 ? 0 : hi;
+    }
+    __syncthreads();
+    const long long clo = clo_s, chi = chi_s;
+    // each source in the window covers at least one rank, so the window
+    // holds at most kTile entries; search globally if counts break that
+    const long long wn = chi - clo + 1;
+    const bool in_lds = wn >= 1 && wn <= kTile;
+    if (in_lds)
+        for (long long q = threadIdx.x; q < wn; q += kBlock)
+            win[q] = cE[clo + q];
+    __syncthreads();
+    for (int j = 0; j < kItems; ++j) {
+        const long long k = k0 + j * kBlock + threadIdx.x;
+        if (k > kl)
+            break;
+        long long c = in_lds ? clo + upper_bound_i(win, wn, k) - 1 :
+                               upper_bound_i(cE, C, k) - 1;
+        c = c < 0 ? 0 : (c >= C ? C - 1 : c);
+        const int src = csrc[c];
+        const long long dst = kExpand ? k : zdst[k];
+        if (idx)
+            idx[dst] = src;
+        if (state) {
+            for (int d = 0; d < dim; ++d)
+                state[d * ld + dst] = state[d * ld + src];
+        }
+    }
+}
+
+__global__ void k_index_rep(const int *idx, long long N, int *rep)
+{
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < N)
+        atomicAdd(&rep[idx[i]], 1);
+}
+
+template <typename T>
+__global__ void k_fill(T *p, long long n, T v, const ResampleDev *prm)
+{
+    if (prm && !prm->active)
+        return;
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+// ------------------------------------------------------- event setup
+
+struct HostParams {
+    long long M, N;
+    unsigned int key;
+    unsigned long long word0;
+    int scheme;
+    int set_weights;   // 1: overwrite active / counts (host-driven)
+};
+
+// Host-driven event: fill the device parameters (no host->device copy).
+__global__ void k_setup(ResampleDev *p, HostParams h)
+{
+    p->M = h.M;
+    p->nsample = h.N;
+    p->ndraw = h.N;
+    p->key = h.key;
+    p->word0 = h.word0;
+    p->active = 1;
+    p->err = 0;
+    p->words_used = 0;
+    p->integ_sum = 0;
+    p->equal = 0;
+}
+
+// Draw parameters once ndraw is known: delta = fl(1 / ndraw) and the
+// systematic start u0 = U * delta drawn first (u01_sequence.hpp:182-187,
+// :231-236).
+__global__ void k_draw_params(ResampleDev *p, int scheme)
+{
+    if (!p->active)
+        return;
+    const long long nd = p->ndraw;
+    p->delta = 1 / static_cast<double>(nd);
+    if (scheme == kSystematic || scheme == kResidualSystematic) {
+        const double U = u01_u32d(philox_word(p->key, p->word0));
+        p->u0 = U * p->delta;
+    } else {
+        p->u0 = 0.0;
+    }
+}
+
+// Residual: exact total of residuals -> 1/total; R = N - sum(integ).
+__global__ void k_resid_params(ResampleDev *p, const double *xstart_total)
+{
+    if (!p->active)
+        return;
+    p->resid_total = *xstart_total;
+    p->inv_resid = 1 / p->resid_total;
+    p->ndraw = p->nsample - p->integ_sum;
+}
+
+template <class WSrc>
+__global__ void __launch_bounds__(kBlock) k_integ_sum(WSrc src, long long M,
+    ResampleDev *p)
+{
+    if (!p->active)
+        return;
+    __shared__ long long lds[kWaves];
+    long long acc = 0;
+    const long long base = static_cast<long long>(blockIdx.x) * kTile;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const long long g = base + j * kBlock + threadIdx.x;
+        if (g < M)
+            acc += src.integ(g);
+    }
+    acc = block_reduce(acc, OpAddL(), lds);
+    if (threadIdx.x == 0)
+        atomicAdd(reinterpret_cast<unsigned long long *>(&p->integ_sum),
+            static_cast<unsigned long long>(acc));
+}
+
+// Words of the resampling stream one event consumes: the lazy sequences
+// draw one word per new index up to the first failing comparison
+// (transform.hpp:61-69), systematic one word in its constructor.
+__global__ void k_words_used(ResampleDev *p, const int *rep, int scheme,
+    int resid, const double *W, int wkind, const double *lw)
+{
+    if (!p->active)
+        return;
+    const long long M = p->M;
+    const long long nd = p->ndraw;
+    unsigned long long used = 0;
+    if (scheme == kSystematic || scheme == kResidualSystematic) {
+        used = 1;
+    } else if (M >= 2 && nd >= 1) {
+        long long last = rep[M - 1];
+        if (resid) {
+            double w = wkind == kWeightLog ?
+                (p->equal ? 1.0 / static_cast<double>(M) :
+                            p->inv_sum * exp(lw[M - 1] - p->lmax)) :
+                W[M - 1];
+            double ip;
+            modf(static_cast<double>(p->nsample) * w, &ip);
+            last -= static_cast<long long>(ip);
+        }
+        const long long jend = nd - last;
+        used = static_cast<unsigned long long>(jend + 1 < nd ? jend + 1 : nd);
+    }
+    p->words_used = used;
+}
+
+// ------------------------------------------------------------ host side
+
+static std::mutex g_err_mu;
+static std::string g_err;
+
+void set_error(const std::string &msg)
+{
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_err = msg;
+}
+
+const char *get_error()
+{
+    static thread_local std::string copy;
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    copy = g_err;
+    return copy.c_str();
+}
+
+bool Workspace::ensure(long long n)
+{
+    const long long tiles = (n + kTile - 1) / kTile + 1;
+    if (n <= cap_n && tiles <= cap_tiles && prm)
+        return true;
+    release();
+    const long long nn = n < 1 ? 1 : n;
+    const long long groups = tiles / kGroup + 2;
+    bool ok = true;
+    auto A = [&](void **p, size_t bytes) {
+        if (hipMalloc(p, bytes ? bytes : 8) != hipSuccess)
+            ok = false;
+    };
+    A((void **)&tsum, sizeof(double) * tiles);
+    A((void **)&tpre, sizeof(double) * tiles);
+    A((void **)&segs, sizeof(TileSeg) * tiles);
+    A((void **)&xstart, sizeof(double) * (tiles + 1));
+    A((void **)&xgstart, sizeof(double) * (groups + 1));
+    A((void **)&gmode, sizeof(int) * (groups + 1));
+    A((void **)&groups, sizeof(GroupSeg) * (groups + 1));
+    A((void **)&tagg, sizeof(long long) * 3 * tiles);
+    A((void **)&toff, sizeof(long long) * 3 * (tiles + 1));
+    A((void **)&zdst, sizeof(int) * nn);
+    A((void **)&cE, sizeof(int) * nn);
+    A((void **)&csrc, sizeof(int) * nn);
+    A((void **)&v, sizeof(double) * nn);
+    A((void **)&aux, sizeof(double) * nn);
+    A((void **)&rep, sizeof(int) * nn);
+    A((void **)&prm, sizeof(ResampleDev));
+    A((void **)&scal, sizeof(long long) * 16);
+    if (!ok) {
+        release();
+        set_error("workspace allocation failed");
+        return false;
+    }
+    cap_n = nn;
+    cap_tiles = tiles;
+    return true;
+}
+
+void Workspace::release()
+{
+    void *ps[] = {tsum, tpre, segs, xstart, xgstart, gmode, groups, tagg, toff,
+        zdst, cE, csrc, v, aux, rep, prm, scal};
+    for (void *p : ps)
+        if (p)
+            (void)hipFree(p);
+    tsum = tpre = xstart = xgstart = v = aux = nullptr;
+    segs = nullptr;
+    gmode = zdst = cE = csrc = rep = nullptr;
+    groups = nullptr;
+    tagg = toff = scal = nullptr;
+    prm = nullptr;
+    cap_n = cap_tiles = 0;
+}
+
+Workspace &default_workspace(int device)
+{
+    static std::mutex mu;
+    static std::vector<Workspace *> ws;
+    std::lock_guard<std::mutex> lk(mu);
+    if (device < 0)
+        device = 0;
+    if (static_cast<int>(ws.size()) <= device)
+        ws.resize(device + 1, nullptr);
+    if (!ws[device])
+        ws[device] = new Workspace();
+    return *ws[device];
+}
+
+static inline unsigned int grid_tiles(long long n)
+{
+    const long long t = (n + kTile - 1) / kTile;
+    return static_cast<unsigned int>(t < 1 ? 1 : t);
+}
+
+// Runs the fold passes (tile sums, approximate scan, segments, groups,
+// resolver) over n weights from src; n may live on the device (ndev).
+template <class WSrc>
+static int run_fold(Workspace &ws, WSrc src, long long nmax,
+    const long long *ndev)
+{
+    hipStream_t st = ws.stream;
+    const unsigned int gt = grid_tiles(nmax);
+    const unsigned int gg = static_cast<unsigned int>((gt + kGroup - 1) / kGroup);
+    int *err = reinterpret_cast<int *>(&ws.prm->err);
+    hipLaunchKernelGGL(k_tile_sum<WSrc>, dim3(gt), dim3(kBlock), 0, st, src,
+        nmax, ndev, ws.prm, ws.tsum, err);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, st, ws.tsum,
+        nmax, ndev, ws.prm, ws.tpre);
+    hipLaunchKernelGGL(k_segments<WSrc>, dim3(gt), dim3(kBlock), 0, st, src,
+        nmax, ndev, ws.prm, ws.tpre, ws.segs);
+    hipLaunchKernelGGL(k_groups, dim3(gg), dim3(kWave), 0, st, nmax, ndev,
+        ws.prm, ws.segs, ws.groups);
+    hipLaunchKernelGGL(k_resolve<WSrc>, dim3(1), dim3(kBlock), 0, st, src,
+        nmax, ndev, ws.prm, ws.segs, ws.groups, ws.xstart, ws.xgstart,
+        ws.gmode);
+    VSMCH_HIP(hipGetLastError());
+    return kOk;
+}
+
+static FoldBufs fold_bufs(Workspace &ws)
+{
+    return FoldBufs{ws.xstart, ws.xgstart, ws.gmode, ws.segs};
+}
+
+template <class WSrc, class USrc>
+static int run_count(Workspace &ws, WSrc src, USrc us, long long M, int *rep)
+{
+    int rc = run_fold(ws, src, M, nullptr);
+    if (rc != kOk)
+        return rc;
+    hipLaunchKernelGGL((k_count<WSrc, USrc>), dim3(grid_tiles(M)),
+        dim3(kBlock), 0, ws.stream, src, us, M,
+        static_cast<const long long *>(nullptr), ws.prm, fold_bufs(ws), rep);
+    VSMCH_HIP(hipGetLastError());
+    return kOk;
+}
+
+static int check_err(Workspace &ws)
+{
+    int err = 0;
+    VSMCH_HIP(hipMemcpyAsync(&err, &ws.prm->err, sizeof(int),
+        hipMemcpyDeviceToHost, ws.stream));
+    VSMCH_HIP(hipStreamSynchronize(ws.stream));
+    if (err) {
+        set_error("weights must be non-negative and not NaN");
+        return kErrWeights;
+    }
+    return kOk;
+}
+
+static int setup_host(Workspace &ws, long long M, long long N,
+    unsigned int key, unsigned long long word0, int scheme)
+{
+    HostParams h{M, N, key, word0, scheme, 1};
+    hipLaunchKernelGGL(k_setup, dim3(1), dim3(1), 0, ws.stream, ws.prm, h);
+    VSMCH_HIP(hipGetLastError());
+    return kOk;
+}
+
+// Running maximum of u (n entries) into ws.v.
+static int run_maxscan(Workspace &ws, const double *u, long long n,
+    const long long *ndev)
+{
+    const unsigned int gt = grid_tiles(n);
+    hipLaunchKernelGGL(k_tile_max, dim3(gt), dim3(kBlock), 0, ws.stream, u, n,
+        ndev, ws.prm, ws.tsum);
+    hipLaunchKernelGGL(k_scan_max, dim3(1), dim3(kBlock), 0, ws.stream,
+        ws.tsum, n, ndev, ws.prm, ws.tpre);
+    hipLaunchKernelGGL(k_apply_max, dim3(gt), dim3(kBlock), 0, ws.stream, u,
+        n, ndev, ws.prm, ws.tpre, ws.v);
+    VSMCH_HIP(hipGetLastError());
+    return kOk;
+}
+
+int trans_u01_rep(Workspace &ws, long long M, long long N, const double *W,
+    const double *u01, int *rep)
+{
+    if (M <= 0)
+        return kOk;
+    if (!ws.ensure(M > N ? M : N))
+        return kErrAlloc;
+    int rc = setup_host(ws, M, N, 0, 0, kMultinomial);
+    if (rc != kOk)
+        return rc;
+    if (M == 1) {
+        hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, ws.stream, rep,
+            1LL, static_cast<int>(N), ws.prm);
+        VSMCH_HIP(hipGetLastError());
+        return kOk;
+    }
+    if (N == 0) {
+        hipLaunchKernelGGL(k_fill<int>, dim3((M + 255) / 256), dim3(256), 0,
+            ws.stream, rep, M, 0, ws.prm);
+        VSMCH_HIP(hipGetLastError());
+        return kOk;
+    }
+    rc = run_maxscan(ws, u01, N, nullptr);
+    if (rc != kOk)
+        return rc;
+    rc = run_count(ws, WArray{W}, UArr{ws.v, ws.prm}, M, rep);
+    if (rc != kOk)
+        return rc;
+    return check_err(ws);
+}
+
+int exact_cumsum(Workspace &ws, long long n, const double *w, double *out)
+{
+    if (n <= 0)
+        return kOk;
+    if (!ws.ensure(n))
+        return kErrAlloc;
+    int rc = setup_host(ws, n, n, 0, 0, kMultinomial);
+    if (rc != kOk)
+        return rc;
+    rc = run_fold(ws, WArray{w}, n, nullptr);
+    if (rc != kOk)
+        return rc;
+    hipLaunchKernelGGL((k_prefix<WArray, EpiRaw>), dim3(grid_tiles(n)),
+        dim3(kBlock), 0, ws.stream, WArray{w}, n,
+        static_cast<const long long *>(nullptr), ws.prm, fold_bufs(ws),
+        EpiRaw{out});
+    VSMCH_HIP(hipGetLastError());
+    return check_err(ws);
+}
+
+// Sorted uniforms of U01SequenceSorted for ndraw draws into out; ndraw may
+// live on the device (ndev, with nmax an upper bound).
+static int run_sorted(Workspace &ws, long long nmax, const long long *ndev,
+    double *out)
+{
+    WSortedTerms terms{ws.prm};
+    int rc = run_fold(ws, terms, nmax, ndev);
+    if (rc != kOk)
+        return rc;
+    hipLaunchKernelGGL((k_prefix<WSortedTerms, EpiSorted>),
+        dim3(grid_tiles(nmax)), dim3(kBlock), 0, ws.stream, terms, nmax, ndev,
+        ws.prm, fold_bufs(ws), EpiSorted{out});
+    VSMCH_HIP(hipGetLastError());
+    return kOk;
+}
+
+int sorted_uniforms(Workspace &ws, long long ndraw, unsigned int key,
+    unsigned long long word0, double *out)
+{
+    if (ndraw <= 0)
+        return kOk;
+    if (!ws.ensure(ndraw))
+        return kErrAlloc;
+    int rc = setup_host(ws, ndraw, ndraw, key, word0, kMultinomial);
+    if (rc != kOk)
+        return rc;
+    rc = run_sorted(ws, ndraw, nullptr, out);
+    if (rc != kOk)
+        return rc;
+    return check_err(ws);
+}
+
+template <class Base>
+static int scheme_counts(Workspace &ws, int scheme, long long M, long long N,
+    Base wb, int *rep)
+{
+    hipStream_t st = ws.stream;
+    const long long *ndraw_dev = &ws.prm->ndraw;
+    const bool resid = scheme == kResidual || scheme == kResidualStratified ||
+        scheme == kResidualSystematic;
+    int rc;
+    if (!resid) {
+        hipLaunchKernelGGL(k_draw_params, dim3(1), dim3(1), 0, st, ws.prm,
+            scheme);
+        if (scheme == kMultinomial) {
+            rc = run_sorted(ws, N, ndraw_dev, ws.aux);
+            if (rc != kOk)
+                return rc;
+            rc = run_maxscan(ws, ws.aux, N, ndraw_dev);
+            if (rc != kOk)
+                return rc;
+            return run_count(ws, wb, UArr{ws.v, ws.prm}, M, rep);
+        }
+        if (scheme == kStratified)
+            return run_count(ws, wb, UStrat{ws.prm}, M, rep);
+        return run_count(ws, wb, USys{ws.prm}, M, rep);
+    }
+    // residual schemes: exact total of the raw residuals, integer parts
+    const double coeff = static_cast<double>(N);
+    WResidRaw<Base> raw{wb, coeff};
+    rc = run_fold(ws, raw, M, nullptr);
+    if (rc != kOk)
+        return rc;
+    hipLaunchKernelGGL(k_integ_sum<WResidRaw<Base>>, dim3(grid_tiles(M)),
+        dim3(kBlock), 0, st, raw, M, ws.prm);
+    const long long ntiles = (M + kTile - 1) / kTile;
+    hipLaunchKernelGGL(k_resid_params, dim3(1), dim3(1), 0, st, ws.prm,
+        ws.xstart + ntiles);
+    hipLaunchKernelGGL(k_draw_params, dim3(1), dim3(1), 0, st, ws.prm, scheme);
+    VSMCH_HIP(hipGetLastError());
+    WResid<Base> res{wb, coeff, ws.prm};
+    if (scheme == kResidual) {
+        rc = run_sorted(ws, N, ndraw_dev, ws.aux);
+        if (rc != kOk)
+            return rc;
+        rc = run_maxscan(ws, ws.aux, N, ndraw_dev);
+        if (rc != kOk)
+            return rc;
+        return run_count(ws, res, UArr{ws.v, ws.prm}, M, rep);
+    }
+    if (scheme == kResidualStratified)
+        return run_count(ws, res, UStrat{ws.prm}, M, rep);
+    return run_count(ws, res, USys{ws.prm}, M, rep);
+}
+
+int resample_scheme(Workspace &ws, int scheme, long long M, long long N,
+    unsigned int key, unsigned long long word0, const double *weights,
+    int weight_kind, int *rep, bool device_active)
+{
+    if (M <= 0)
+        return kOk;
+    if (!ws.ensure(M > N ? M : N))
+        return kErrAlloc;
+    int rc;
+    if (!device_active) {
+        rc = setup_host(ws, M, N, key, word0, scheme);
+        if (rc != kOk)
+            return rc;
+    }
+    if (M == 1 || N == 0) {
+        // transform.hpp:49-59: nothing is drawn except the systematic
+        // sequence's constructor word.
+        if (M == 1)
+            hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(64), 0, ws.stream,
+                rep, 1LL, static_cast<int>(N), ws.prm);
+        else
+            hipLaunchKernelGGL(k_fill<int>, dim3((M + 255) / 256), dim3(256),
+                0, ws.stream, rep, M, 0, ws.prm);
+        const unsigned long long used =
+            (scheme == kSystematic || scheme == kResidualSystematic) ? 1 : 0;
+        hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(1), 0,
+            ws.stream, &ws.prm->words_used, 1LL, used, ws.prm);
+        VSMCH_HIP(hipGetLastError());
+        return kOk;
+    }
+    if (weight_kind == kWeightLog)
+        rc = scheme_counts(ws, scheme, M, N, WLog{weights, ws.prm}, rep);
+    else
+        rc = scheme_counts(ws, scheme, M, N, WArray{weights}, rep);
+    if (rc != kOk)
+        return rc;
+    const bool resid = scheme == kResidual || scheme == kResidualStratified ||
+        scheme == kResidualSystematic;
+    hipLaunchKernelGGL(k_words_used, dim3(1), dim3(1), 0, ws.stream, ws.prm,
+        rep, scheme, resid ? 1 : 0, weight_kind == kWeightArray ? weights : nullptr,
+        weight_kind, weight_kind == kWeightLog ? weights : nullptr);
+    VSMCH_HIP(hipGetLastError());
+    if (!device_active)
+        return check_err(ws);
+    return kOk;
+}
+
+int read_words_used(Workspace &ws, unsigned long long *out)
+{
+    VSMCH_HIP(hipMemcpyAsync(out, &ws.prm->words_used, sizeof(*out),
+        hipMemcpyDeviceToHost, ws.stream));
+    VSMCH_HIP(hipStreamSynchronize(ws.stream));
+    return kOk;
+}
+
+int trans_rep_index(Workspace &ws, long long M, long long N, const int *rep,
+    int *idx, void *state, int dim, long long ld, int elem_bytes,
+    bool device_active, bool force_expand)
+{
+    if (M <= 0 || N <= 0)
+        return kOk;
+    if (!ws.ensure(M > N ? M : N))
+        return kErrAlloc;
+    hipStream_t st = ws.stream;
+    const ResampleDev *prm = device_active ? ws.prm : nullptr;
+    const long long ntiles = (M + kTile - 1) / kTile;
+    const bool expand = M != N || force_expand;
+    if (expand) {
+        hipLaunchKernelGGL(k_ridx_agg<true>, dim3(ntiles), dim3(kBlock), 0,
+            st, rep, M, prm, ws.tagg);
+    } else {
+        hipLaunchKernelGGL(k_ridx_agg<false>, dim3(ntiles), dim3(kBlock), 0,
+            st, rep, M, prm, ws.tagg);
+    }
+    hipLaunchKernelGGL(k_ridx_scan, dim3(1), dim3(kBlock), 0, st, ws.tagg,
+        ntiles, prm, ws.toff);
+    if (expand) {
+        hipLaunchKernelGGL(k_ridx_scatter<true>, dim3(ntiles), dim3(kBlock), 0,
+            st, rep, M, prm, ws.toff, ws.zdst, ws.cE, ws.csrc, idx);
+    } else {
+        hipLaunchKernelGGL(k_ridx_scatter<false>, dim3(ntiles), dim3(kBlock),
+            0, st, rep, M, prm, ws.toff, ws.zdst, ws.cE, ws.csrc, idx);
+    }
+    const unsigned int ge = grid_tiles(N);
+    if (expand) {
+        hipLaunchKernelGGL((k_expand<true, float>), dim3(ge), dim3(kBlock), 0,
+            st, prm, ws.toff, ntiles, ws.zdst, ws.cE, ws.csrc, idx,
+            static_cast<float *>(nullptr), 0, 0LL);
+    } else if (state == nullptr || elem_bytes == 4) {
+        hipLaunchKernelGGL((k_expand<false, float>), dim3(ge), dim3(kBlock), 0,
+            st, prm, ws.toff, ntiles, ws.zdst, ws.cE, ws.csrc, idx,
+            static_cast<float *>(state), dim, ld);
+    } else {
+        hipLaunchKernelGGL((k_expand<false, double>), dim3(ge), dim3(kBlock),
+            0, st, prm, ws.toff, ntiles, ws.zdst, ws.cE, ws.csrc, idx,
+            static_cast<double *>(state), dim, ld);
+    }
+    VSMCH_HIP(hipGetLastError());
+    return kOk;
+}
+
+int trans_index_rep(Workspace &ws, long long M, long long N, const int *idx,
+    int *rep)
+{
+    if (M <= 0 || N <= 0)
+        return kOk;
+    hipStream_t st = ws.stream;
+    hipLaunchKernelGGL(k_fill<int>, dim3((M + 255) / 256), dim3(256), 0, st,
+        rep, M, 0, static_cast<const ResampleDev *>(nullptr));
+    hipLaunchKernelGGL(k_index_rep, dim3((N + 255) / 256), dim3(256), 0, st,
+        idx, N, rep);
+    VSMCH_HIP(hipGetLastError());
+    return kOk;
+}
+
+__global__ void k_resid_write(const double *W, long long M, double coeff,
+    double *resid, int *integ)
+{
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < M) {
+        double ip;
+        resid[i] = modf(coeff * W[i], &ip);
+        integ[i] = static_cast<int>(ip);
+    }
+}
+
+__global__ void k_resid_scale(double *resid, long long M, const ResampleDev *p)
+{
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < M)
+        resid[i] = p->inv_resid * resid[i];
+}
+
+int trans_residual(Workspace &ws, long long M, long long N, const double *W,
+    double *resid, int *integ, long long *R_out)
+{
+    if (M <= 0) {
+        *R_out = N;
+        return kOk;
+    }
+    if (!ws.ensure(M > N ? M : N))
+        return kErrAlloc;
+    int rc = setup_host(ws, M, N, 0, 0, kResidual);
+    if (rc != kOk)
+        return rc;
+    hipStream_t st = ws.stream;
+    const double coeff = static_cast<double>(N);
+    hipLaunchKernelGGL(k_resid_write, dim3((M + 255) / 256), dim3(256), 0, st,
+        W, M, coeff, resid, integ);
+    WResidRaw<WArray> raw{WArray{W}, coeff};
+    rc = run_fold(ws, raw, M, nullptr);
+    if (rc != kOk)
+        return rc;
+    hipLaunchKernelGGL(k_integ_sum<WResidRaw<WArray>>, dim3(grid_tiles(M)),
+        dim3(kBlock), 0, st, raw, M, ws.prm);
+    const long long ntiles = (M + kTile - 1) / kTile;
+    hipLaunchKernelGGL(k_resid_params, dim3(1), dim3(1), 0, st, ws.prm,
+        ws.xstart + ntiles);
+    hipLaunchKernelGGL(k_resid_scale, dim3((M + 255) / 256), dim3(256), 0, st,
+        resid, M, ws.prm);
+    VSMCH_HIP(hipGetLastError());
+    long long isum = 0;
+    VSMCH_HIP(hipMemcpyAsync(&isum, &ws.prm->integ_sum, sizeof(isum),
+        hipMemcpyDeviceToHost, st));
+    rc = check_err(ws);
+    if (rc != kOk)
+        return rc;
+    *R_out = N - isum;
+    return kOk;
+}
+
+} // namespace vsmch
