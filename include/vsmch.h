@@ -144,6 +144,10 @@ vsmch_status vsmch_particle_destroy(vsmch_particle *p);
  * log-weights. */
 vsmch_status vsmch_particle_state(vsmch_particle *p, void **state);
 vsmch_status vsmch_particle_log_weight(vsmch_particle *p, double **logw);
+/* Copies the whole state (dim * n values) to / from a device array laid
+ * out the same way (StateMatrix::read_state / state assignment). */
+vsmch_status vsmch_particle_read_state(vsmch_particle *p, void *dst);
+vsmch_status vsmch_particle_write_state(vsmch_particle *p, const void *src);
 /* Key of particle i's stream and of the resampling engine; the resampling
  * engine's next unread word. */
 vsmch_status vsmch_particle_keys(vsmch_particle *p, uint32_t *key_first_host,
@@ -171,6 +175,19 @@ vsmch_status vsmch_weight_read(vsmch_particle *p, double *W);
  * rep_out / idx_out (device, may be NULL) receive counts and ancestors. */
 vsmch_status vsmch_particle_resample(vsmch_particle *p, int scheme,
     double threshold, int *resampled_host, int *rep_out, int *idx_out);
+
+/* Per-particle engine positions (next unread word of particle i's stream),
+ * device array of n uint64, zero at creation; device functors that draw a
+ * data-dependent number of words keep them here (the reference keeps one
+ * CounterEngine per particle, include/vsmc/rng/rng_set.hpp:81-122). */
+vsmch_status vsmch_particle_rng_words(vsmch_particle *p, uint64_t **words);
+/* n doubles of device scratch owned by the particle set (a functor's
+ * per-particle log-weight increments between eval_sp and eval_post). */
+vsmch_status vsmch_particle_scratch(vsmch_particle *p, double **scratch);
+/* Monitor::eval's dgemv (include/vsmc/core/monitor.hpp:263-266):
+ * r_d = sum_i W_i buf[i * dim + d] for a device buffer of n x dim values. */
+vsmch_status vsmch_weighted_sum(vsmch_particle *p, int dim,
+    const double *buf, double *r_host);
 
 /* Number of resampling events so far (the length of the true entries of
  * Sampler::resampled_history, include/vsmc/core/sampler.hpp:702-707). */

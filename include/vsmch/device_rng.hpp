@@ -203,6 +203,13 @@ struct normal_pair {
     T z1;
 };
 
+// Round-to-nearest add and multiply that the compiler may not contract into
+// an FMA: the reference's host build evaluates a * b + c with two roundings.
+__device__ inline double rn_add(double a, double b) { return __dadd_rn(a, b); }
+__device__ inline float rn_add(float a, float b) { return __fadd_rn(a, b); }
+__device__ inline double rn_mul(double a, double b) { return __dmul_rn(a, b); }
+__device__ inline float rn_mul(float a, float b) { return __fmul_rn(a, b); }
+
 __device__ inline void sincos_t(double a, double *s, double *c)
 {
     sincos(a, s, c);

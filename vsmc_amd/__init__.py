@@ -56,6 +56,8 @@ SIGNATURES = {
     "vsmch_particle_create": (_c.c_int, [_c.POINTER(_P), _P]),
     "vsmch_particle_destroy": (_c.c_int, [_P]),
     "vsmch_particle_state": (_c.c_int, [_P, _c.POINTER(_P)]),
+    "vsmch_particle_read_state": (_c.c_int, [_P, _P]),
+    "vsmch_particle_write_state": (_c.c_int, [_P, _P]),
     "vsmch_particle_log_weight": (_c.c_int, [_P, _c.POINTER(_P)]),
     "vsmch_particle_keys": (_c.c_int, [_P, _c.POINTER(_c.c_uint32),
                                        _c.POINTER(_c.c_uint32), _c.POINTER(_c.c_uint64)]),
@@ -68,6 +70,9 @@ SIGNATURES = {
     "vsmch_particle_resample": (_c.c_int, [_P, _c.c_int, _c.c_double,
                                            _c.POINTER(_c.c_int), _P, _P]),
     "vsmch_particle_resample_count": (_c.c_int, [_P, _c.POINTER(_c.c_int64)]),
+    "vsmch_particle_rng_words": (_c.c_int, [_P, _c.POINTER(_P)]),
+    "vsmch_particle_scratch": (_c.c_int, [_P, _c.POINTER(_P)]),
+    "vsmch_weighted_sum": (_c.c_int, [_P, _c.c_int, _P, _c.POINTER(_c.c_double)]),
     "vsmch_monitor_state_moments": (_c.c_int, [_P, _c.c_int, _c.c_int,
                                                _c.POINTER(_c.c_double)]),
     "vsmch_cv_init": (_c.c_int, [_P, _P, _c.c_double, _c.c_double]),
@@ -102,14 +107,34 @@ class VsmchError(RuntimeError):
 _lib = None
 
 
+def hip_runtimes():
+    """Paths of the HIP runtime libraries mapped into this process."""
+    try:
+        with open("/proc/self/maps") as f:
+            return sorted({ln.split()[-1] for ln in f if "libamdhip64" in ln})
+    except OSError:
+        return []
+
+
 def lib():
-    """Loads libvsmch.so (built by vsmc_amd.build); raises if it is absent."""
+    """Loads libvsmch.so (built by vsmc_amd.build); raises if it is absent.
+
+    torch is imported first so that the library's libamdhip64.so.7 dependency
+    binds to the HIP runtime torch already carries (same soname); device
+    pointers then come from one runtime. Two runtimes in one process would
+    hand each other unknown allocations, so that case raises."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise VsmchError(
                 f"{LIB_PATH} is missing: build it with `python -m vsmc_amd.build`")
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
+        if len(hip_runtimes()) > 1:
+            raise VsmchError(f"two HIP runtimes are loaded: {hip_runtimes()}")
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)
             f.restype = res

@@ -18,7 +18,7 @@ FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # packed FP32 instructions are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+    "-Wall", "-Wshadow", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
 
@@ -53,6 +53,33 @@ def build(verbose=True):
     return LIB
 
 
+ROOT = os.path.dirname(HERE)
+CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", "pf_hip.cpp")]
+
+
+def build_cpp_tests(verbose=True):
+    """C++ programs written against include/vsmch/sampler.hpp (the drop-in
+    layer), linked to the in-tree libvsmch.so. -ffp-contract=off keeps the
+    device functors' a * b + c rounded twice, as the reference's host build."""
+    outs = []
+    for src in CPP_TESTS:
+        exe = os.path.splitext(src)[0]
+        cmd = [HIPCC, "-O3", "-std=c++17", f"--offload-arch={ARCH}",
+               "-ffp-contract=off",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+               "-I", os.path.join(ROOT, "include"), src,
+               "-L", HERE, "-lvsmch", "-Wl,-rpath,$ORIGIN/../../vsmc_amd",
+               "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")
+        outs.append(exe)
+        if verbose:
+            print(f"built {exe}")
+    return outs
+
+
 if __name__ == "__main__":
     build()
+    build_cpp_tests()
     sys.exit(0)

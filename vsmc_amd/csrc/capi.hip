@@ -266,6 +266,30 @@ vsmch_status vsmch_particle_log_weight(vsmch_particle *p, double **logw)
     return VSMCH_OK;
 }
 
+vsmch_status vsmch_particle_read_state(vsmch_particle *p, void *dst)
+{
+    if (!p || !dst)
+        return VSMCH_ERR_INVALID;
+    ParticleSet *ps = p->p;
+    VSMCH_HIP(hipMemcpyAsync(dst, ps->state,
+        static_cast<size_t>(ps->n) * ps->dim * ps->bytes, hipMemcpyDeviceToDevice,
+        ps->ws.stream));
+    VSMCH_HIP(hipStreamSynchronize(ps->ws.stream));
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_particle_write_state(vsmch_particle *p, const void *src)
+{
+    if (!p || !src)
+        return VSMCH_ERR_INVALID;
+    ParticleSet *ps = p->p;
+    VSMCH_HIP(hipMemcpyAsync(ps->state, src,
+        static_cast<size_t>(ps->n) * ps->dim * ps->bytes, hipMemcpyDeviceToDevice,
+        ps->ws.stream));
+    VSMCH_HIP(hipStreamSynchronize(ps->ws.stream));
+    return VSMCH_OK;
+}
+
 vsmch_status vsmch_particle_keys(vsmch_particle *p, uint32_t *key_first_host,
     uint32_t *resample_key_host, uint64_t *resample_word_host)
 {
@@ -347,6 +371,45 @@ vsmch_status vsmch_particle_resample(vsmch_particle *p, int scheme,
         rc = read_pdev(ps, &h);
         *resampled_host = h.resampled;
     }
+    return rc;
+}
+
+vsmch_status vsmch_particle_rng_words(vsmch_particle *p, uint64_t **words)
+{
+    if (!p || !words)
+        return VSMCH_ERR_INVALID;
+    ParticleSet *ps = p->p;
+    if (!ps->rng_words) {
+        VSMCH_HIP(hipMalloc(&ps->rng_words, sizeof(unsigned long long) * ps->n));
+        VSMCH_HIP(hipMemset(ps->rng_words, 0, sizeof(unsigned long long) * ps->n));
+    }
+    *words = reinterpret_cast<uint64_t *>(ps->rng_words);
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_particle_scratch(vsmch_particle *p, double **scratch)
+{
+    if (!p || !scratch)
+        return VSMCH_ERR_INVALID;
+    *scratch = p->p->scratch;
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_weighted_sum(vsmch_particle *p, int dim, const double *buf,
+    double *r_host)
+{
+    if (!p || !buf || !r_host || dim < 1)
+        return VSMCH_ERR_INVALID;
+    ParticleSet *ps = p->p;
+    double *out = nullptr;
+    VSMCH_HIP(hipMalloc(&out, sizeof(double) * dim));
+    int rc = weighted_sum(ps, dim, buf, out);
+    if (rc == VSMCH_OK) {
+        VSMCH_HIP(hipMemcpyAsync(r_host, out, sizeof(double) * dim,
+            hipMemcpyDeviceToHost, ps->ws.stream));
+        VSMCH_HIP(hipStreamSynchronize(ps->ws.stream));
+    }
+    (void)hipFree(out);
     return rc;
 }
 

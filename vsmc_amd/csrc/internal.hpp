@@ -58,6 +58,7 @@ struct Workspace {
 
     bool ensure(long long n);
     void release();
+    bool all_allocated() const;
 };
 
 Workspace &default_workspace(int device);

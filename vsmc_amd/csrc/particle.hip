@@ -385,7 +385,7 @@ void particle_destroy(ParticleSet *p)
     if (!p)
         return;
     void *ps[] = {p->state, p->logw, p->scratch, p->rep, p->part, p->mpart,
-        p->mout, p->pd};
+        p->mout, p->pd, p->rng_words};
     for (void *q : ps)
         if (q)
             (void)hipFree(q);
@@ -441,6 +441,50 @@ int resample_step(ParticleSet *p, int scheme, double threshold, long long t,
         return rc;
     hipLaunchKernelGGL(k_after_resample, dim3(1), dim3(1), 0, st, p->pd,
         p->ws.prm, p->n);
+    VSMCH_HIP(hipGetLastError());
+    return kOk;
+}
+
+// r_d = sum_i W_i buf[i * dim + d0 + d] for 8 components at a time.
+__global__ void __launch_bounds__(kPB) k_wsum(const double *buf,
+    const double *l, long long n, int dim, int d0, const PDev *pd,
+    double *part)
+{
+    const long long base = static_cast<long long>(blockIdx.x) * kPTile;
+    const int nd = (dim - d0) < 8 ? (dim - d0) : 8;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int j = 0; j < kPItems; ++j) {
+        const long long i = base + j * kPB + threadIdx.x;
+        if (i < n) {
+            const double w = pd->equal ? 1.0 / static_cast<double>(n) :
+                                         exp(l[i] - pd->c);
+            for (int d = 0; d < nd; ++d)
+                acc[d] += w * buf[i * dim + d0 + d];
+        }
+    }
+    __shared__ double lds[kWaves];
+    for (int d = 0; d < nd; ++d) {
+        const double v = block_reduce(acc[d], OpAddD(), lds);
+        if (threadIdx.x == 0)
+            part[blockIdx.x * 8 + d] = v;
+    }
+}
+
+int weighted_sum(ParticleSet *p, int dim, const double *buf, double *out_dev)
+{
+    if (dim < 1) {
+        set_error("weighted_sum: dim must be positive");
+        return kErrInvalid;
+    }
+    hipStream_t st = p->ws.stream;
+    const long long nb = pblocks(p->n);
+    for (int d0 = 0; d0 < dim; d0 += 8) {
+        const int nd = (dim - d0) < 8 ? (dim - d0) : 8;
+        hipLaunchKernelGGL(k_wsum, dim3(nb), dim3(kPB), 0, st, buf, p->logw,
+            p->n, dim, d0, p->pd, p->mpart);
+        hipLaunchKernelGGL(k_moments_finalize, dim3(1), dim3(kPB), 0, st,
+            p->mpart, nb, nd, out_dev + d0);
+    }
     VSMCH_HIP(hipGetLastError());
     return kOk;
 }

@@ -45,8 +45,11 @@ struct ParticleSet {
     double *mpart = nullptr;
     double *mout = nullptr;
     PDev *pd = nullptr;
+    unsigned long long *rng_words = nullptr;  // allocated on first request
     Workspace ws;
 };
+
+int weighted_sum(ParticleSet *p, int dim, const double *buf, double *out_dev);
 
 int particle_create(ParticleSet **out, long long n, int dim, int bytes,
     uint32_t seed, hipStream_t st);

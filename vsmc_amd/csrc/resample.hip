@@ -16,6 +16,7 @@
 #include "../../include/vsmch/device_rng.hpp"
 
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 namespace vsmch {
@@ -1018,36 +1019,48 @@ bool Workspace::ensure(long long n)
         return true;
     release();
     const long long nn = n < 1 ? 1 : n;
-    const long long groups = tiles / kGroup + 2;
+    const long long ngroups = tiles / kGroup + 2;
     bool ok = true;
-    auto A = [&](void **p, size_t bytes) {
-        if (hipMalloc(p, bytes ? bytes : 8) != hipSuccess)
+    auto A = [&](auto **p, size_t bytes) {
+        void *q = nullptr;
+        if (hipMalloc(&q, bytes ? bytes : 8) != hipSuccess || q == nullptr)
             ok = false;
+        *p = static_cast<std::remove_reference_t<decltype(**p)> *>(q);
     };
-    A((void **)&tsum, sizeof(double) * tiles);
-    A((void **)&tpre, sizeof(double) * tiles);
-    A((void **)&segs, sizeof(TileSeg) * tiles);
-    A((void **)&xstart, sizeof(double) * (tiles + 1));
-    A((void **)&xgstart, sizeof(double) * (groups + 1));
-    A((void **)&gmode, sizeof(int) * (groups + 1));
-    A((void **)&groups, sizeof(GroupSeg) * (groups + 1));
-    A((void **)&tagg, sizeof(long long) * 3 * tiles);
-    A((void **)&toff, sizeof(long long) * 3 * (tiles + 1));
-    A((void **)&zdst, sizeof(int) * nn);
-    A((void **)&cE, sizeof(int) * nn);
-    A((void **)&csrc, sizeof(int) * nn);
-    A((void **)&v, sizeof(double) * nn);
-    A((void **)&aux, sizeof(double) * nn);
-    A((void **)&rep, sizeof(int) * nn);
-    A((void **)&prm, sizeof(ResampleDev));
-    A((void **)&scal, sizeof(long long) * 16);
-    if (!ok) {
+    A(&tsum, sizeof(double) * tiles);
+    A(&tpre, sizeof(double) * tiles);
+    A(&segs, sizeof(TileSeg) * tiles);
+    A(&xstart, sizeof(double) * (tiles + 1));
+    A(&xgstart, sizeof(double) * (ngroups + 1));
+    A(&gmode, sizeof(int) * (ngroups + 1));
+    A(&groups, sizeof(GroupSeg) * (ngroups + 1));
+    A(&tagg, sizeof(long long) * 3 * tiles);
+    A(&toff, sizeof(long long) * 3 * (tiles + 1));
+    A(&zdst, sizeof(int) * nn);
+    A(&cE, sizeof(int) * nn);
+    A(&csrc, sizeof(int) * nn);
+    A(&v, sizeof(double) * nn);
+    A(&aux, sizeof(double) * nn);
+    A(&rep, sizeof(int) * nn);
+    A(&prm, sizeof(ResampleDev));
+    A(&scal, sizeof(long long) * 16);
+    if (!ok || !all_allocated()) {
         release();
         set_error("workspace allocation failed");
         return false;
     }
     cap_n = nn;
     cap_tiles = tiles;
+    return true;
+}
+
+bool Workspace::all_allocated() const
+{
+    const void *ps[] = {tsum, tpre, segs, xstart, xgstart, gmode, groups, tagg,
+        toff, zdst, cE, csrc, v, aux, rep, prm, scal};
+    for (const void *p : ps)
+        if (!p)
+            return false;
     return true;
 }
 

@@ -47,16 +47,16 @@ class Particle:
         """Copy of the state as a (dim, n) tensor (ColMajor StateMatrix)."""
         import torch
         out = torch.empty((self.dim, self.n), dtype=self.dtype, device="cuda")
-        elem = 4 if self.dtype == torch.float32 else 8
-        _memcpy_d2d(out.data_ptr(), self._state_ptr(), self.dim * self.n * elem)
+        torch.cuda.synchronize()
+        check(lib().vsmch_particle_read_state(self._h, ctypes.c_void_p(out.data_ptr())))
         return out
 
     def set_state(self, x):
         import torch
         x = x.to(device="cuda", dtype=self.dtype).contiguous()
         assert x.shape == (self.dim, self.n)
-        elem = 4 if self.dtype == torch.float32 else 8
-        _memcpy_d2d(self._state_ptr(), x.data_ptr(), self.dim * self.n * elem)
+        torch.cuda.synchronize()
+        check(lib().vsmch_particle_write_state(self._h, ctypes.c_void_p(x.data_ptr())))
 
     def keys(self):
         k0, kr = ctypes.c_uint32(), ctypes.c_uint32()
@@ -130,35 +130,6 @@ class Particle:
 def _f64():
     import torch
     return torch.float64
-
-
-def _memcpy_d2d(dst, src, nbytes):
-    import torch
-    # torch owns no copy of raw pointers; go through a tiny view-free copy
-    # with the HIP runtime that torch links.
-    hip = _hip()
-    rc = hip.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src),
-                       ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
-    if rc != 0:
-        raise VsmchError(f"hipMemcpy failed ({rc})")
-    torch.cuda.synchronize()
-
-
-_hiplib = None
-
-
-def _hip():
-    global _hiplib
-    if _hiplib is None:
-        import glob
-        import os
-        cands = glob.glob("/opt/rocm/lib/libamdhip64.so*")
-        _hiplib = ctypes.CDLL(sorted(cands)[0] if cands else "libamdhip64.so",
-                              mode=os.RTLD_GLOBAL if hasattr(os, "RTLD_GLOBAL") else 0)
-        _hiplib.hipMemcpy.restype = ctypes.c_int
-        _hiplib.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
-                                      ctypes.c_int]
-    return _hiplib
 
 
 class CvFilter(Particle):
