@@ -59,6 +59,14 @@ struct OpCompose {
     __device__ FMap operator()(FMap a, FMap b) const { return compose(a, b); }
 };
 
+// Saturating sum of map offsets (both <= kCap, so no overflow).
+struct OpAddSatL {
+    __device__ long long operator()(long long a, long long b) const
+    {
+        return sat(a + b);
+    }
+};
+
 // Binade exponent of x > 0 (finite); subnormals share the lowest normal ulp.
 __host__ __device__ inline int binade(double x)
 {
@@ -178,26 +186,58 @@ __device__ inline void tile_exact(const double (&w)[kItems], double x0,
         const int e = binade(cur);
         const long long a0 = mant(cur);
         const int p0 = static_cast<int>(a0 & 1);
-        FMap agg = ident();
-#pragma unroll
-        for (int j = 0; j < kItems; ++j)
-            if (base + j >= pos)
-                agg = compose(agg, elem_map(w[j], e));
-        FMap total;
-        FMap run = block_exclusive_scan(agg, OpCompose(), ident(), sh.scan,
-            &total);
-        long long lc = kTile;
+        // Without an exact half-ulp tie the maps do not depend on parity:
+        // the fold is a saturating integer prefix sum of the rounded steps.
+        long long agg = 0;
+        bool tie = false;
 #pragma unroll
         for (int j = 0; j < kItems; ++j) {
-            const int idx = base + j;
-            if (idx >= pos) {
-                run = compose(run, elem_map(w[j], e));
-                const long long a = a0 + (p0 ? run.d1 : run.d0);
-                if (a >= kTwo53) {
-                    if (lc == kTile)
-                        lc = idx;
-                } else {
-                    s[j] = from_mant(a, e);
+            if (base + j >= pos) {
+                const FMap m = elem_map(w[j], e);
+                agg = sat(agg + m.d0);
+                tie = tie || (m.d0 != m.d1);
+            }
+        }
+        long long lc = kTile;
+        if (!__syncthreads_or(tie)) {
+            long long ltot;
+            long long run = block_exclusive_scan(agg, OpAddSatL(), 0LL, sh.red,
+                &ltot);
+#pragma unroll
+            for (int j = 0; j < kItems; ++j) {
+                const int idx = base + j;
+                if (idx >= pos) {
+                    run = sat(run + elem_map(w[j], e).d0);
+                    const long long a = a0 + run;
+                    if (a >= kTwo53) {
+                        if (lc == kTile)
+                            lc = idx;
+                    } else {
+                        s[j] = from_mant(a, e);
+                    }
+                }
+            }
+        } else {
+            FMap agg = ident();
+#pragma unroll
+            for (int j = 0; j < kItems; ++j)
+                if (base + j >= pos)
+                    agg = compose(agg, elem_map(w[j], e));
+            FMap total;
+            FMap run = block_exclusive_scan(agg, OpCompose(), ident(), sh.scan,
+                &total);
+#pragma unroll
+            for (int j = 0; j < kItems; ++j) {
+                const int idx = base + j;
+                if (idx >= pos) {
+                    run = compose(run, elem_map(w[j], e));
+                    const long long a = a0 + (p0 ? run.d1 : run.d0);
+                    if (a >= kTwo53) {
+                        if (lc == kTile)
+                            lc = idx;
+                    } else {
+                        s[j] = from_mant(a, e);
+                    }
                 }
             }
         }
@@ -246,6 +286,7 @@ struct SegShared {
     FMap fscan[kWaves];
     int fflag[kWaves];
     SegPair pscan[kWaves];
+    double tlast[kBlock];
     unsigned char cross[kTile + 1];
     TileSeg seg;
 };
@@ -280,9 +321,24 @@ __device__ inline void tile_segments(const double (&w)[kItems], int nvalid,
         }
         cr[j] = c;
         eb[j] = (prev > 0.0 && prev < INFINITY) ? binade(prev) : kEmpty;
-        sh.cross[base + j] = c ? 1 : 0;
-        ncl += c ? 1 : 0;
         prev = t;
+    }
+    // The block scan and the previous thread's running sum approximate the
+    // same prefix with different roundings; where they disagree on the
+    // binade, this thread's first element is taken as a crossing so that
+    // no run mixes binades (each run's binade is verified at its start).
+    sh.tlast[threadIdx.x] = prev;
+    __syncthreads();
+    if (threadIdx.x > 0 && base < nvalid && !cr[0]) {
+        const double tl = sh.tlast[threadIdx.x - 1];
+        const int el = (tl > 0.0 && tl < INFINITY) ? binade(tl) : kEmpty;
+        if (el != eb[0])
+            cr[0] = true;
+    }
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        sh.cross[base + j] = cr[j] ? 1 : 0;
+        ncl += cr[j] ? 1 : 0;
     }
     if (threadIdx.x == 0)
         sh.cross[kTile] = 1;
@@ -299,6 +355,34 @@ __device__ inline void tile_segments(const double (&w)[kItems], int nvalid,
             sh.seg.cross[r] = 0.0;
     }
     __syncthreads();
+    if (ltot == 0) {
+        // one run in one binade (every prev > 0 and finite); without a
+        // half-ulp tie its map is the saturating sum of the rounded steps
+        long long dsum = 0;
+        bool tie = false;
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            if (base + j < nvalid) {
+                const FMap m = elem_map(w[j], eb[j]);
+                dsum = sat(dsum + m.d0);
+                tie = tie || (m.d0 != m.d1);
+            }
+        }
+        if (!__syncthreads_or(tie)) {
+            const long long d = block_reduce(dsum, OpAddSatL(), sh.lscan);
+            if (threadIdx.x == 0) {
+                sh.seg.e[0] = eb[0];
+                sh.seg.map[0] = FMap{d, d};
+            }
+            __syncthreads();
+            const int words = static_cast<int>(sizeof(TileSeg) / 4);
+            const int *src = reinterpret_cast<const int *>(&sh.seg);
+            int *dst = reinterpret_cast<int *>(out);
+            for (int i = threadIdx.x; i < words; i += kBlock)
+                dst[i] = src[i];
+            return;
+        }
+    }
     // segmented scan: a run starts at element 0, at a crossing, and right
     // after a crossing; crossing elements carry the identity
     SegPair agg{0, ident()};

@@ -122,10 +122,11 @@ int trans_u01_rep(Workspace &ws, long long M, long long N, const double *W,
 // applied in place on an SoA state of `dim` components of elem_bytes each,
 // component stride `ld` elements. idx may be null when only gathering.
 // force_expand: lay out the expanded list (src repeated rep[src] times)
-// even when M == N (resample_trans_u01_index).
+// even when M == N (resample_trans_u01_index). agg_ready: the per-tile
+// aggregates in ws.tagg were produced by the counting pass that wrote rep.
 int trans_rep_index(Workspace &ws, long long M, long long N, const int *rep,
     int *idx, void *state, int dim, long long ld, int elem_bytes,
-    bool device_active, bool force_expand = false);
+    bool device_active, bool force_expand = false, bool agg_ready = false);
 
 int trans_index_rep(Workspace &ws, long long M, long long N, const int *idx,
     int *rep);
@@ -139,9 +140,12 @@ int trans_residual(Workspace &ws, long long M, long long N, const double *W,
 // `weights` points at the log-weights. When device_active is true the
 // kernels honour prm->active (set on the device) and the host does not
 // synchronise; otherwise the host sets everything.
+// prepared: (device_active, systematic / stratified) the draw parameters
+// and ws.tpre were set by the weight side (k_prepare), so the tile-sum and
+// parameter kernels are skipped.
 int resample_scheme(Workspace &ws, int scheme, long long M, long long N,
     unsigned int key, unsigned long long word0, const double *weights,
-    int weight_kind, int *rep, bool device_active);
+    int weight_kind, int *rep, bool device_active, bool prepared = false);
 
 // Words the last resample_scheme call consumed (read back to the host).
 int read_words_used(Workspace &ws, unsigned long long *out);

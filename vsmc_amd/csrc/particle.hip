@@ -19,6 +19,9 @@ namespace vsmch {
 constexpr int kPB = 256;              // threads per particle block
 constexpr int kPItems = 16;           // particles per thread
 constexpr int kPTile = kPB * kPItems; // particles per block
+// k_prepare writes the resampling fold's tile prefix from these blocks'
+// weight partials: one particle block is one fold tile.
+static_assert(kPTile == kTile, "particle block must equal the fold tile");
 
 struct Part3 {
     double m, s, q;
@@ -148,11 +151,91 @@ __global__ void k_decide(PDev *pd, ResampleDev *prm, long long n,
     }
 }
 
-// After the event: equal weights, advance the resampling stream.
-__global__ void k_after_resample(PDev *pd, const ResampleDev *prm, long long n)
+// Decision and event parameters as k_decide, plus what the resampling
+// passes would otherwise recompute from the weights: delta = fl(1/N), the
+// systematic start u0 = U delta (u01_sequence.hpp:182-187, :231-236), and
+// the approximate exclusive prefix of the normalised weights over tiles of
+// kPTile particles, from the weight pass's block partials
+// (sum_b exp(m_b - lmax) / sum, one term per block).
+__global__ void __launch_bounds__(kPB) k_prepare(PDev *pd, ResampleDev *prm,
+    const Part3 *part, long long nb, long long n, double threshold,
+    long long t, double *ess_hist, int *rs_hist, int scheme, double *tpre)
+{
+    __shared__ int act;
+    __shared__ double lds[kWaves];
+    if (threadIdx.x == 0) {
+        const double ess = pd->ess;
+        const int active = ess < threshold * static_cast<double>(n) ? 1 : 0;
+        if (ess_hist)
+            ess_hist[t] = ess;
+        if (rs_hist)
+            rs_hist[t] = active;
+        pd->resampled = active;
+        prm->active = active;
+        prm->M = n;
+        prm->nsample = n;
+        prm->ndraw = n;
+        prm->key = pd->rkey;
+        prm->word0 = pd->rword;
+        prm->err = 0;
+        prm->integ_sum = 0;
+        prm->words_used = 0;
+        prm->equal = pd->equal;
+        prm->lmax = pd->equal ? pd->log_equal : pd->lmax;
+        prm->inv_sum = pd->equal ? 1.0 : 1 / pd->sum;
+        const double delta = 1 / static_cast<double>(n);
+        prm->delta = delta;
+        prm->u0 = (scheme == kSystematic) ?
+            u01_u32d(philox_word(pd->rkey, pd->rword)) * delta : 0.0;
+        act = active;
+    }
+    __syncthreads();
+    if (!act)
+        return;
+    const int equal = pd->equal;
+    const double lmax = pd->lmax;
+    const double inv = equal ? 1.0 : 1 / pd->sum;
+    const double inv_n = 1 / static_cast<double>(n);
+    double carry = 0.0;
+    for (long long c = 0; c < nb; c += kPB) {
+        const long long b = c + threadIdx.x;
+        double v = 0.0;
+        if (b < nb) {
+            if (equal) {
+                const long long rem = n - b * kPTile;
+                v = static_cast<double>(rem < kPTile ? rem : kPTile) * inv_n;
+            } else {
+                const Part3 q = part[b];
+                v = q.s > 0.0 ? q.s * exp(q.m - lmax) * inv : 0.0;
+            }
+        }
+        double tot;
+        const double ex = block_exclusive_scan(v, OpAddD(), 0.0, lds, &tot);
+        if (b < nb)
+            tpre[b] = carry + ex;
+        carry += tot;
+    }
+}
+
+// After the event: equal weights, advance the resampling stream. With
+// `words` the event's stream use is computed here (systematic: the one
+// constructor word; stratified: up to and including the first uniform above
+// s_{M-2}, transform.hpp:61-69).
+__global__ void k_after_resample(PDev *pd, ResampleDev *prm, long long n,
+    const int *rep, int scheme, int words)
 {
     if (!prm->active)
         return;
+    if (words) {
+        unsigned long long used = 1;
+        if (scheme == kStratified) {
+            const long long jend = n - rep[n - 1];
+            used = static_cast<unsigned long long>(jend + 1 < n ? jend + 1 : n);
+        }
+        if (n < 2)
+            used = scheme == kSystematic ? 1 : 0;
+        prm->words_used = used;
+    }
     pd->nresampled += 1;
     pd->rword += prm->words_used;
     pd->equal = 1;
@@ -258,10 +341,65 @@ __device__ inline Real cv_loglik(const CvModelDev &m, Real x, Real y, Real ox,
 // vy += sd_vel z3, then log-likelihood; weights by add_log.
 // Also accumulates the monitor moments of the state *before* the move
 // (the previous step's post-resampling "pos" record) when mon != null.
+template <typename Real>
+struct alignas(4 * sizeof(Real)) Vec4 {
+    Real v[4];
+};
+
+// seed_key(s, i + 1) from k = seed_key(s, i): keys run 1 .. 2^32 - 1.
+__device__ inline uint32_t next_key(uint32_t k)
+{
+    return k == 0xFFFFFFFFu ? 1u : k + 1u;
+}
+
+// One particle's move (or initialisation) from its Philox block `blk`.
 template <typename Real, bool kInit>
-__global__ void __launch_bounds__(kPB) k_cv_step(Real *state, double *l,
-    long long n, uint32_t seed, long long t, Real ox, Real oy, CvModelDev m,
-    const PDev *pd, Part3 *part, double *mon)
+__device__ inline void cv_advance(const CvModelDev &m, uint32_t key,
+    unsigned long long blk, Real &x, Real &y, Real &vx, Real &vy)
+{
+    const u32x4 wd = philox_block(key, blk);
+    const normal_pair<Real> p0 = box_muller<Real>(wd.v[0], wd.v[1]);
+    const normal_pair<Real> p1 = box_muller<Real>(wd.v[2], wd.v[3]);
+    if (kInit) {
+        x = rmul(static_cast<Real>(m.sd_pos0), p0.z0);
+        y = rmul(static_cast<Real>(m.sd_pos0), p0.z1);
+        vx = rmul(static_cast<Real>(m.sd_vel0), p1.z0);
+        vy = rmul(static_cast<Real>(m.sd_vel0), p1.z1);
+    } else {
+        const Real dt = static_cast<Real>(m.dt);
+        x = radd(x, radd(rmul(static_cast<Real>(m.sd_pos), p0.z0), rmul(dt, vx)));
+        y = radd(y, radd(rmul(static_cast<Real>(m.sd_pos), p0.z1), rmul(dt, vy)));
+        vx = radd(vx, rmul(static_cast<Real>(m.sd_vel), p1.z0));
+        vy = radd(vy, rmul(static_cast<Real>(m.sd_vel), p1.z1));
+    }
+}
+
+// Partials (max, sum exp, sum exp^2) of k log-weights held by one thread:
+// the maximum first, then one exp per value. -inf and NaN contribute
+// nothing, as in part_of / part_merge.
+template <int K>
+__device__ inline Part3 part_of_array(const double (&v)[K])
+{
+    double mx = -INFINITY;
+    for (int k = 0; k < K; ++k)
+        mx = v[k] > mx ? v[k] : mx;
+    Part3 p{mx, 0.0, 0.0};
+    if (mx > -INFINITY) {
+        for (int k = 0; k < K; ++k) {
+            if (v[k] > -INFINITY) {
+                const double r = exp(v[k] - mx);
+                p.s += r;
+                p.q += r * r;
+            }
+        }
+    }
+    return p;
+}
+
+template <typename Real, bool kInit, int kB>
+__global__ void __launch_bounds__(kPB) k_cv_step(Real *__restrict__ state,
+    double *__restrict__ l, long long n, uint32_t seed, long long t, Real ox,
+    Real oy, CvModelDev m, const PDev *pd, Part3 *part, double *mon)
 {
     const long long base = static_cast<long long>(blockIdx.x) * kPTile;
     Part3 acc{-INFINITY, 0.0, 0.0};
@@ -273,46 +411,107 @@ __global__ void __launch_bounds__(kPB) k_cv_step(Real *state, double *l,
     const double lw_equal = pd->log_equal;
     const int equal = pd->equal;
     const double c = pd->c;
-    for (int j = 0; j < kPItems; ++j) {
-        const long long i = base + j * kPB + threadIdx.x;
-        if (i >= n)
-            continue;
-        const uint32_t key = seed_key(seed, static_cast<unsigned long long>(i));
-        const u32x4 wd = philox_block(key, static_cast<unsigned long long>(t) + 1);
-        const normal_pair<Real> p0 = box_muller<Real>(wd.v[0], wd.v[1]);
-        const normal_pair<Real> p1 = box_muller<Real>(wd.v[2], wd.v[3]);
-        Real x, y, vx, vy;
-        double lold = 0.0;
-        if (kInit) {
-            x = rmul(static_cast<Real>(m.sd_pos0), p0.z0);
-            y = rmul(static_cast<Real>(m.sd_pos0), p0.z1);
-            vx = rmul(static_cast<Real>(m.sd_vel0), p1.z0);
-            vy = rmul(static_cast<Real>(m.sd_vel0), p1.z1);
-        } else {
-            x = X[i];
-            y = Y[i];
-            vx = VX[i];
-            vy = VY[i];
-            lold = equal ? lw_equal : l[i] - c;
-            if (mon) {
-                const double w = exp(lold);
-                mx += w * static_cast<double>(x);
-                my += w * static_cast<double>(y);
+    const unsigned long long blk = static_cast<unsigned long long>(t) + 1;
+    if ((n & 3) == 0 && base + kPTile <= n) {
+        // Full tile, every component row 16-byte aligned: each thread takes
+        // kR runs of 4 consecutive particles, kB runs at a time; a batch's
+        // loads are all issued before its arithmetic so that they overlap.
+        constexpr int kR = kPTile / (4 * kPB);
+        static_assert(kR % kB == 0, "batch must divide the runs");
+        double lv[kR * 4];
+#pragma unroll
+        for (int r0 = 0; r0 < kR; r0 += kB) {
+            Vec4<Real> xs[kB], ys[kB], vxs[kB], vys[kB];
+#pragma unroll
+            for (int r = 0; r < kB; ++r) {
+                const long long i0 = base + ((r0 + r) * kPB + threadIdx.x) * 4;
+                if (!kInit) {
+                    xs[r] = *reinterpret_cast<const Vec4<Real> *>(X + i0);
+                    ys[r] = *reinterpret_cast<const Vec4<Real> *>(Y + i0);
+                    vxs[r] = *reinterpret_cast<const Vec4<Real> *>(VX + i0);
+                    vys[r] = *reinterpret_cast<const Vec4<Real> *>(VY + i0);
+                    if (!equal) {
+                        const Vec4<double> lq = *reinterpret_cast<const Vec4<double> *>(l + i0);
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            lv[(r0 + r) * 4 + q] = lq.v[q] - c;
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            lv[(r0 + r) * 4 + q] = lw_equal;
+                    }
+                }
             }
-            const Real dt = static_cast<Real>(m.dt);
-            x = radd(x, radd(rmul(static_cast<Real>(m.sd_pos), p0.z0), rmul(dt, vx)));
-            y = radd(y, radd(rmul(static_cast<Real>(m.sd_pos), p0.z1), rmul(dt, vy)));
-            vx = radd(vx, rmul(static_cast<Real>(m.sd_vel), p1.z0));
-            vy = radd(vy, rmul(static_cast<Real>(m.sd_vel), p1.z1));
+#pragma unroll
+            for (int r = 0; r < kB; ++r) {
+                const long long i0 = base + ((r0 + r) * kPB + threadIdx.x) * 4;
+                uint32_t key = seed_key(seed, static_cast<unsigned long long>(i0));
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    double &lq = lv[(r0 + r) * 4 + q];
+                    Real x{}, y{}, vx{}, vy{};
+                    if (!kInit) {
+                        x = xs[r].v[q];
+                        y = ys[r].v[q];
+                        vx = vxs[r].v[q];
+                        vy = vys[r].v[q];
+                        if (mon) {
+                            const double w = exp(lq);
+                            mx += w * static_cast<double>(x);
+                            my += w * static_cast<double>(y);
+                        }
+                    }
+                    cv_advance<Real, kInit>(m, key, blk, x, y, vx, vy);
+                    key = next_key(key);
+                    xs[r].v[q] = x;
+                    ys[r].v[q] = y;
+                    vxs[r].v[q] = vx;
+                    vys[r].v[q] = vy;
+                    const double ll = static_cast<double>(cv_loglik<Real>(m, x, y, ox, oy));
+                    lq = kInit ? ll : lq + ll;
+                }
+                *reinterpret_cast<Vec4<Real> *>(X + i0) = xs[r];
+                *reinterpret_cast<Vec4<Real> *>(Y + i0) = ys[r];
+                *reinterpret_cast<Vec4<Real> *>(VX + i0) = vxs[r];
+                *reinterpret_cast<Vec4<Real> *>(VY + i0) = vys[r];
+                Vec4<double> lo;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    lo.v[q] = lv[(r0 + r) * 4 + q];
+                *reinterpret_cast<Vec4<double> *>(l + i0) = lo;
+            }
         }
-        X[i] = x;
-        Y[i] = y;
-        VX[i] = vx;
-        VY[i] = vy;
-        const double ll = static_cast<double>(cv_loglik<Real>(m, x, y, ox, oy));
-        const double lnew = kInit ? ll : lold + ll;
-        l[i] = lnew;
-        acc = part_merge(acc, part_of(lnew));
+        acc = part_of_array(lv);
+    } else {
+        for (int j = 0; j < kPItems; ++j) {
+            const long long i = base + j * kPB + threadIdx.x;
+            if (i >= n)
+                continue;
+            const uint32_t key = seed_key(seed, static_cast<unsigned long long>(i));
+            Real x{}, y{}, vx{}, vy{};
+            double lold = 0.0;
+            if (!kInit) {
+                x = X[i];
+                y = Y[i];
+                vx = VX[i];
+                vy = VY[i];
+                lold = equal ? lw_equal : l[i] - c;
+                if (mon) {
+                    const double w = exp(lold);
+                    mx += w * static_cast<double>(x);
+                    my += w * static_cast<double>(y);
+                }
+            }
+            cv_advance<Real, kInit>(m, key, blk, x, y, vx, vy);
+            X[i] = x;
+            Y[i] = y;
+            VX[i] = vx;
+            VY[i] = vy;
+            const double ll = static_cast<double>(cv_loglik<Real>(m, x, y, ox, oy));
+            const double lnew = kInit ? ll : lold + ll;
+            l[i] = lnew;
+            acc = part_merge(acc, part_of(lnew));
+        }
     }
     block_partials(acc, part);
     if (mon) {
@@ -427,20 +626,30 @@ int resample_step(ParticleSet *p, int scheme, double threshold, long long t,
     double *ess_hist, int *rs_hist, int *rep_out, int *idx_out)
 {
     hipStream_t st = p->ws.stream;
-    hipLaunchKernelGGL(k_decide, dim3(1), dim3(1), 0, st, p->pd, p->ws.prm,
-        p->n, threshold, t, ess_hist, rs_hist);
+    // systematic / stratified with N >= 2: one kernel prepares the decision,
+    // the draw parameters and the tile prefix from the weight partials
+    const bool prep = p->n >= 2 && (scheme == kSystematic || scheme == kStratified);
+    if (prep)
+        hipLaunchKernelGGL(k_prepare, dim3(1), dim3(kPB), 0, st, p->pd,
+            p->ws.prm, static_cast<const Part3 *>(p->part), pblocks(p->n),
+            p->n, threshold, t, ess_hist, rs_hist, scheme, p->ws.tpre);
+    else
+        hipLaunchKernelGGL(k_decide, dim3(1), dim3(1), 0, st, p->pd, p->ws.prm,
+            p->n, threshold, t, ess_hist, rs_hist);
     VSMCH_HIP(hipGetLastError());
     int *rep = rep_out ? rep_out : p->rep;
     int rc = resample_scheme(p->ws, scheme, p->n, p->n, 0, 0, p->logw,
-        kWeightLog, rep, true);
+        kWeightLog, rep, true, prep);
     if (rc != kOk)
         return rc;
+    // the counting pass left the rep -> index aggregates in ws.tagg (a
+    // single particle takes the fill path, which writes none)
     rc = trans_rep_index(p->ws, p->n, p->n, rep, idx_out, p->state, p->dim,
-        p->n, p->bytes, true, false);
+        p->n, p->bytes, true, false, p->n >= 2);
     if (rc != kOk)
         return rc;
     hipLaunchKernelGGL(k_after_resample, dim3(1), dim3(1), 0, st, p->pd,
-        p->ws.prm, p->n);
+        p->ws.prm, p->n, static_cast<const int *>(rep), scheme, prep ? 1 : 0);
     VSMCH_HIP(hipGetLastError());
     return kOk;
 }
@@ -511,6 +720,37 @@ int moments(ParticleSet *p, int c0, int dim, double *out_dev)
     return kOk;
 }
 
+// Runs of 4 particles whose loads a move thread issues together: 2 for
+// FP32 state, 1 for FP64 (register budget); VSMCH_MOVE_BATCH (1, 2 or 4)
+// overrides it for tuning.
+static int move_batch(int bytes)
+{
+    static const int env = [] {
+        const char *v = getenv("VSMCH_MOVE_BATCH");
+        const int b = v ? atoi(v) : 0;
+        return (b == 1 || b == 2 || b == 4) ? b : 0;
+    }();
+    if (env)
+        return env;
+    return bytes == 4 ? 2 : 1;
+}
+
+template <typename Real, bool kInit>
+static void launch_cv(int batch, long long nb, hipStream_t st, Real *s,
+    double *l, long long n, uint32_t seed, long long t, Real ox, Real oy,
+    const CvModelDev &m, const PDev *pd, Part3 *part, double *mon)
+{
+    if (batch == 4)
+        hipLaunchKernelGGL((k_cv_step<Real, kInit, 4>), dim3(nb), dim3(kPB),
+            0, st, s, l, n, seed, t, ox, oy, m, pd, part, mon);
+    else if (batch == 2)
+        hipLaunchKernelGGL((k_cv_step<Real, kInit, 2>), dim3(nb), dim3(kPB),
+            0, st, s, l, n, seed, t, ox, oy, m, pd, part, mon);
+    else
+        hipLaunchKernelGGL((k_cv_step<Real, kInit, 1>), dim3(nb), dim3(kPB),
+            0, st, s, l, n, seed, t, ox, oy, m, pd, part, mon);
+}
+
 int cv_step(ParticleSet *p, const CvModelDev &m, long long t, double ox,
     double oy, bool with_monitor)
 {
@@ -521,26 +761,25 @@ int cv_step(ParticleSet *p, const CvModelDev &m, long long t, double ox,
     hipStream_t st = p->ws.stream;
     const long long nb = pblocks(p->n);
     double *mon = with_monitor ? p->mpart : nullptr;
+    Part3 *part = static_cast<Part3 *>(p->part);
+    const int batch = move_batch(p->bytes);
     if (p->bytes == 4) {
+        float *s = static_cast<float *>(p->state);
+        const float fx = static_cast<float>(ox), fy = static_cast<float>(oy);
         if (t == 0)
-            hipLaunchKernelGGL((k_cv_step<float, true>), dim3(nb), dim3(kPB), 0,
-                st, static_cast<float *>(p->state), p->logw, p->n, p->seed, t,
-                static_cast<float>(ox), static_cast<float>(oy), m, p->pd,
-                static_cast<Part3 *>(p->part), static_cast<double *>(nullptr));
+            launch_cv<float, true>(batch, nb, st, s, p->logw, p->n, p->seed,
+                t, fx, fy, m, p->pd, part, nullptr);
         else
-            hipLaunchKernelGGL((k_cv_step<float, false>), dim3(nb), dim3(kPB),
-                0, st, static_cast<float *>(p->state), p->logw, p->n, p->seed,
-                t, static_cast<float>(ox), static_cast<float>(oy), m, p->pd,
-                static_cast<Part3 *>(p->part), mon);
+            launch_cv<float, false>(batch, nb, st, s, p->logw, p->n, p->seed,
+                t, fx, fy, m, p->pd, part, mon);
     } else {
+        double *s = static_cast<double *>(p->state);
         if (t == 0)
-            hipLaunchKernelGGL((k_cv_step<double, true>), dim3(nb), dim3(kPB),
-                0, st, static_cast<double *>(p->state), p->logw, p->n, p->seed,
-                t, ox, oy, m, p->pd, static_cast<Part3 *>(p->part), static_cast<double *>(nullptr));
+            launch_cv<double, true>(batch, nb, st, s, p->logw, p->n, p->seed,
+                t, ox, oy, m, p->pd, part, nullptr);
         else
-            hipLaunchKernelGGL((k_cv_step<double, false>), dim3(nb), dim3(kPB),
-                0, st, static_cast<double *>(p->state), p->logw, p->n, p->seed,
-                t, ox, oy, m, p->pd, static_cast<Part3 *>(p->part), mon);
+            launch_cv<double, false>(batch, nb, st, s, p->logw, p->n, p->seed,
+                t, ox, oy, m, p->pd, part, mon);
     }
     hipLaunchKernelGGL(k_weight_finalize, dim3(1), dim3(kPB), 0, st, static_cast<const Part3 *>(p->part),
         nb, p->n, t == 0 ? 0 : 1, p->pd);

@@ -311,6 +311,11 @@ __global__ void __launch_bounds__(kWave) k_groups(long long nh,
 struct ResolveShared {
     double tile[kTileLds];
     fold::ExactShared ex;
+    // single-run summaries of the group being walked, and one full summary
+    int gnc[kGroup];
+    int ge[kGroup];
+    FMap gmap[kGroup];
+    fold::TileSeg one;
     double bx;
     int bL;
     int bok;
@@ -406,15 +411,23 @@ __global__ void __launch_bounds__(kBlock) k_resolve(WSrc src, long long nh,
             xgstart[g] = x;
         }
         long long t = g * kGroup;
+        const long long t0 = t;
         const long long tend = (t + kGroup < ntiles) ? t + kGroup : ntiles;
+        // one round trip for the group's single-run summaries
+        if (threadIdx.x < tend - t0) {
+            const TileSeg &sg = segs[t0 + threadIdx.x];
+            sh.gnc[threadIdx.x] = sg.ncross;
+            sh.ge[threadIdx.x] = sg.e[0];
+            sh.gmap[threadIdx.x] = sg.map[0];
+        }
+        __syncthreads();
         while (t < tend) {
             if (threadIdx.x < kWave) {
                 double xn;
                 auto get = [&](long long tt, int &e, FMap &m) {
-                    const int nc = segs[tt].ncross;
-                    e = segs[tt].e[0];
-                    m = segs[tt].map[0];
-                    return nc == 0;
+                    e = sh.ge[tt - t0];
+                    m = sh.gmap[tt - t0];
+                    return sh.gnc[tt - t0] == 0;
                 };
                 const int L2 = wave_advance(x, t, tend - t, get, xstart,
                     static_cast<int *>(nullptr), &xn);
@@ -431,10 +444,19 @@ __global__ void __launch_bounds__(kBlock) k_resolve(WSrc src, long long nh,
                 t += L2;
                 continue;
             }
+            {
+                // stage the full summary in LDS, then apply it
+                const int words = static_cast<int>(sizeof(TileSeg) / 4);
+                const int *src_w = reinterpret_cast<const int *>(&segs[t]);
+                int *dst_w = reinterpret_cast<int *>(&sh.one);
+                for (int q = threadIdx.x; q < words; q += kBlock)
+                    dst_w[q] = src_w[q];
+            }
+            __syncthreads();
             if (threadIdx.x == 0) {
                 xstart[t] = x;
                 double y = x;
-                sh.bok = fold::apply_segments(segs[t], y) ? 1 : 0;
+                sh.bok = fold::apply_segments(sh.one, y) ? 1 : 0;
                 sh.bx = y;
             }
             __syncthreads();
@@ -467,37 +489,46 @@ __global__ void __launch_bounds__(kBlock) k_resolve(WSrc src, long long nh,
     }
 }
 
-// Exact start of tile t after the resolver.
-__device__ inline double tile_start(long long t, const double *xstart,
-    const double *xgstart, const int *gmode, const TileSeg *segs)
+// Exact tile starts inside the groups the resolver advanced whole (gmode):
+// tile t0 + k starts at the group start mapped by the first k tile maps,
+// all in the group's binade and below its top. One wave per group; the
+// resolver wrote xstart for the tiles of every other group.
+__global__ void __launch_bounds__(kWave) k_starts(long long nh,
+    const long long *ndev, const ResampleDev *prm, const TileSeg *segs,
+    const double *xgstart, const int *gmode, double *xstart)
 {
-    const long long g = t / kGroup;
-    if (gmode[g]) {
-        const double x = xgstart[g];
-        const long long t0 = g * kGroup;
-        if (t == t0)
-            return x;
-        FMap P = fold::ident();
-        for (long long tt = t0; tt < t; ++tt)
-            P = fold::compose(P, segs[tt].map[0]);
+    if (!prm->active)
+        return;
+    const long long n = eff_n(ndev, nh);
+    const long long ntiles = ceil_div(n, kTile);
+    const long long t0 = static_cast<long long>(blockIdx.x) * kGroup;
+    if (t0 >= ntiles || !gmode[blockIdx.x])
+        return;
+    const long long t = t0 + threadIdx.x;
+    FMap m = fold::ident();
+    if (t < ntiles)
+        m = segs[t].map[0];
+    const FMap P = wave_inclusive_scan(m, fold::OpCompose());
+    FMap E = shfl_up_any(P, 1);
+    if (threadIdx.x == 0)
+        E = fold::ident();
+    if (t < ntiles) {
+        const double x = xgstart[blockIdx.x];
         const long long a0 = fold::mant(x);
-        return fold::from_mant(a0 + ((a0 & 1) ? P.d1 : P.d0), fold::binade(x));
+        xstart[t] = fold::from_mant(a0 + ((a0 & 1) ? E.d1 : E.d0),
+            fold::binade(x));
     }
-    return xstart[t];
 }
 
 struct FoldBufs {
-    const double *xstart;
-    const double *xgstart;
-    const int *gmode;
-    const TileSeg *segs;
+    const double *xstart;   // exact start of every tile, total at [ntiles]
 };
 
 // Replication counts (and residual integer parts) for one tile.
 template <class WSrc, class USrc>
 __global__ void __launch_bounds__(kBlock) k_count(WSrc src, USrc us,
     long long nh, const long long *ndev, const ResampleDev *prm, FoldBufs fb,
-    int *rep)
+    int *rep, long long *tagg)
 {
     if (!prm->active)
         return;
@@ -507,39 +538,53 @@ __global__ void __launch_bounds__(kBlock) k_count(WSrc src, USrc us,
         return;
     __shared__ double tile[kTileLds];
     __shared__ fold::ExactShared ex;
-    __shared__ long long lastG[kBlock];
+    __shared__ double lasts[kBlock];
     __shared__ double x0s;
+    __shared__ long long red[kWaves];
     double w[kItems], s[kItems];
     load_tile_blocked(src, base, M, w, tile, 0.0);
     if (threadIdx.x == 0)
-        x0s = tile_start(blockIdx.x, fb.xstart, fb.xgstart, fb.gmode, fb.segs);
+        x0s = fb.xstart[blockIdx.x];
     __syncthreads();
     const double x0 = x0s;
     fold::tile_exact(w, x0, s, ex);
+    lasts[threadIdx.x] = s[kItems - 1];
+    __syncthreads();
     const long long ib = base + static_cast<long long>(threadIdx.x) * kItems;
     const long long nd = us.ndraw();
-    long long G[kItems];
-#pragma unroll
-    for (int j = 0; j < kItems; ++j) {
-        const long long i = ib + j;
-        G[j] = 0;
-        if (i < M)
-            G[j] = (i == M - 1) ? nd : us.count(s[j]);
+    // count before this thread's first element: the previous element's sum
+    // (0 before particle 0: the merge starts at j = 0)
+    long long gp = 0;
+    if (ib < M) {
+        if (threadIdx.x == 0)
+            gp = (blockIdx.x == 0) ? 0 : us.count(x0);
+        else
+            gp = us.count(lasts[threadIdx.x - 1]);
     }
-    lastG[threadIdx.x] = G[kItems - 1];
-    __syncthreads();
-    long long gp;
-    if (threadIdx.x == 0)
-        gp = (blockIdx.x == 0) ? 0 : us.count(x0);
-    else
-        gp = lastG[threadIdx.x - 1];
+    long long z = 0, e = 0, c = 0;
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
         const long long i = ib + j;
         if (i < M) {
-            const long long r = G[j] - gp + src.integ(i);
-            gp = G[j];
+            const long long g = (i == M - 1) ? nd : us.count(s[j]);
+            const long long r = g - gp + src.integ(i);
+            gp = g;
             rep[i] = static_cast<int>(r);
+            z += r == 0;
+            e += r > 1 ? r - 1 : 0;
+            c += r > 1;
+        }
+    }
+    if (tagg) {
+        // rep -> index aggregates of this tile (childless slots, extra
+        // copies, sources with extras)
+        z = block_reduce(z, OpAddL(), red);
+        e = block_reduce(e, OpAddL(), red);
+        c = block_reduce(c, OpAddL(), red);
+        if (threadIdx.x == 0) {
+            tagg[3 * blockIdx.x + 0] = z;
+            tagg[3 * blockIdx.x + 1] = e;
+            tagg[3 * blockIdx.x + 2] = c;
         }
     }
 }
@@ -574,7 +619,7 @@ __global__ void __launch_bounds__(kBlock) k_prefix(WSrc src, long long nh,
     double w[kItems], s[kItems];
     load_tile_blocked(src, base, n, w, tile, 0.0);
     if (threadIdx.x == 0)
-        x0s = tile_start(blockIdx.x, fb.xstart, fb.xgstart, fb.gmode, fb.segs);
+        x0s = fb.xstart[blockIdx.x];
     __syncthreads();
     fold::tile_exact(w, x0s, s, ex);
     const long long ib = base + static_cast<long long>(threadIdx.x) * kItems;
@@ -1104,16 +1149,20 @@ static inline unsigned int grid_tiles(long long n)
 // resolver) over n weights from src; n may live on the device (ndev).
 template <class WSrc>
 static int run_fold(Workspace &ws, WSrc src, long long nmax,
-    const long long *ndev)
+    const long long *ndev, bool tpre_ready = false)
 {
     hipStream_t st = ws.stream;
     const unsigned int gt = grid_tiles(nmax);
     const unsigned int gg = static_cast<unsigned int>((gt + kGroup - 1) / kGroup);
     int *err = reinterpret_cast<int *>(&ws.prm->err);
-    hipLaunchKernelGGL(k_tile_sum<WSrc>, dim3(gt), dim3(kBlock), 0, st, src,
-        nmax, ndev, ws.prm, ws.tsum, err);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, st, ws.tsum,
-        nmax, ndev, ws.prm, ws.tpre);
+    // tpre_ready: the approximate tile prefix was written from the weight
+    // pass's block partials (blocks and tiles both hold kTile particles)
+    if (!tpre_ready) {
+        hipLaunchKernelGGL(k_tile_sum<WSrc>, dim3(gt), dim3(kBlock), 0, st,
+            src, nmax, ndev, ws.prm, ws.tsum, err);
+        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, st,
+            ws.tsum, nmax, ndev, ws.prm, ws.tpre);
+    }
     hipLaunchKernelGGL(k_segments<WSrc>, dim3(gt), dim3(kBlock), 0, st, src,
         nmax, ndev, ws.prm, ws.tpre, ws.segs);
     hipLaunchKernelGGL(k_groups, dim3(gg), dim3(kWave), 0, st, nmax, ndev,
@@ -1121,24 +1170,28 @@ static int run_fold(Workspace &ws, WSrc src, long long nmax,
     hipLaunchKernelGGL(k_resolve<WSrc>, dim3(1), dim3(kBlock), 0, st, src,
         nmax, ndev, ws.prm, ws.segs, ws.groups, ws.xstart, ws.xgstart,
         ws.gmode);
+    hipLaunchKernelGGL(k_starts, dim3(gg), dim3(kWave), 0, st, nmax, ndev,
+        ws.prm, ws.segs, ws.xgstart, ws.gmode, ws.xstart);
     VSMCH_HIP(hipGetLastError());
     return kOk;
 }
 
 static FoldBufs fold_bufs(Workspace &ws)
 {
-    return FoldBufs{ws.xstart, ws.xgstart, ws.gmode, ws.segs};
+    return FoldBufs{ws.xstart};
 }
 
 template <class WSrc, class USrc>
-static int run_count(Workspace &ws, WSrc src, USrc us, long long M, int *rep)
+static int run_count(Workspace &ws, WSrc src, USrc us, long long M, int *rep,
+    bool tpre_ready = false)
 {
-    int rc = run_fold(ws, src, M, nullptr);
+    int rc = run_fold(ws, src, M, nullptr, tpre_ready);
     if (rc != kOk)
         return rc;
     hipLaunchKernelGGL((k_count<WSrc, USrc>), dim3(grid_tiles(M)),
         dim3(kBlock), 0, ws.stream, src, us, M,
-        static_cast<const long long *>(nullptr), ws.prm, fold_bufs(ws), rep);
+        static_cast<const long long *>(nullptr), ws.prm, fold_bufs(ws), rep,
+        ws.tagg);
     VSMCH_HIP(hipGetLastError());
     return kOk;
 }
@@ -1265,13 +1318,19 @@ int sorted_uniforms(Workspace &ws, long long ndraw, unsigned int key,
 
 template <class Base>
 static int scheme_counts(Workspace &ws, int scheme, long long M, long long N,
-    Base wb, int *rep)
+    Base wb, int *rep, bool prepared)
 {
     hipStream_t st = ws.stream;
     const long long *ndraw_dev = &ws.prm->ndraw;
     const bool resid = scheme == kResidual || scheme == kResidualStratified ||
         scheme == kResidualSystematic;
     int rc;
+    if (prepared) {
+        // draw parameters and the weights' tile prefix were set on the device
+        if (scheme == kStratified)
+            return run_count(ws, wb, UStrat{ws.prm}, M, rep, true);
+        return run_count(ws, wb, USys{ws.prm}, M, rep, true);
+    }
     if (!resid) {
         hipLaunchKernelGGL(k_draw_params, dim3(1), dim3(1), 0, st, ws.prm,
             scheme);
@@ -1318,7 +1377,7 @@ static int scheme_counts(Workspace &ws, int scheme, long long M, long long N,
 
 int resample_scheme(Workspace &ws, int scheme, long long M, long long N,
     unsigned int key, unsigned long long word0, const double *weights,
-    int weight_kind, int *rep, bool device_active)
+    int weight_kind, int *rep, bool device_active, bool prepared)
 {
     if (M <= 0)
         return kOk;
@@ -1346,17 +1405,22 @@ int resample_scheme(Workspace &ws, int scheme, long long M, long long N,
         VSMCH_HIP(hipGetLastError());
         return kOk;
     }
+    const bool prep = prepared && device_active &&
+        (scheme == kSystematic || scheme == kStratified);
     if (weight_kind == kWeightLog)
-        rc = scheme_counts(ws, scheme, M, N, WLog{weights, ws.prm}, rep);
+        rc = scheme_counts(ws, scheme, M, N, WLog{weights, ws.prm}, rep, prep);
     else
-        rc = scheme_counts(ws, scheme, M, N, WArray{weights}, rep);
+        rc = scheme_counts(ws, scheme, M, N, WArray{weights}, rep, prep);
     if (rc != kOk)
         return rc;
     const bool resid = scheme == kResidual || scheme == kResidualStratified ||
         scheme == kResidualSystematic;
-    hipLaunchKernelGGL(k_words_used, dim3(1), dim3(1), 0, ws.stream, ws.prm,
-        rep, scheme, resid ? 1 : 0, weight_kind == kWeightArray ? weights : nullptr,
-        weight_kind, weight_kind == kWeightLog ? weights : nullptr);
+    // prepared events count their stream words in the caller's closing kernel
+    if (!prep)
+        hipLaunchKernelGGL(k_words_used, dim3(1), dim3(1), 0, ws.stream, ws.prm,
+            rep, scheme, resid ? 1 : 0,
+            weight_kind == kWeightArray ? weights : nullptr, weight_kind,
+            weight_kind == kWeightLog ? weights : nullptr);
     VSMCH_HIP(hipGetLastError());
     if (!device_active)
         return check_err(ws);
@@ -1373,7 +1437,7 @@ int read_words_used(Workspace &ws, unsigned long long *out)
 
 int trans_rep_index(Workspace &ws, long long M, long long N, const int *rep,
     int *idx, void *state, int dim, long long ld, int elem_bytes,
-    bool device_active, bool force_expand)
+    bool device_active, bool force_expand, bool agg_ready)
 {
     if (M <= 0 || N <= 0)
         return kOk;
@@ -1386,7 +1450,7 @@ int trans_rep_index(Workspace &ws, long long M, long long N, const int *rep,
     if (expand) {
         hipLaunchKernelGGL(k_ridx_agg<true>, dim3(ntiles), dim3(kBlock), 0,
             st, rep, M, prm, ws.tagg);
-    } else {
+    } else if (!agg_ready) {
         hipLaunchKernelGGL(k_ridx_agg<false>, dim3(ntiles), dim3(kBlock), 0,
             st, rep, M, prm, ws.tagg);
     }
