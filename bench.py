@@ -86,6 +86,88 @@ def move_kernel_ms(va, torch, f, t, reps, ox, oy):
     return e0.elapsed_time(e1) / reps
 
 
+def copy_bandwidth(torch, nbytes=1 << 30, reps=5):
+    """Device-to-device copy of a known size (read + write bytes / time):
+    the box's attainable HBM bandwidth, and the reference kernel whose
+    FETCH_SIZE / WRITE_SIZE counters scale the traffic figures."""
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 2 * nbytes / (ms / 1e3) / 1e9
+
+
+def _events_ms(torch, fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def microbench(va, torch, reps=5):
+    """2^28 FP64 U01 draws from Philox4x32-10 and Threefry4x64-20 and 2^28
+    FP64 normals (8 bytes written per variate), and resampling of N = 2^24
+    log-weights ~ N(0, s^2): set_log + the scheme's counts, ancestors and
+    an in-place gather of one FP32 component, every call resampling."""
+    out = {}
+    n = 1 << 28
+    buf = torch.empty(n, dtype=torch.float64, device="cuda")
+    for name, fn in (
+            ("u01_philox", lambda: va.u01_fill(va.PHILOX4X32, 7, 0, n, out=buf)),
+            ("u01_threefry", lambda: va.u01_fill(va.THREEFRY4X64, 7, 0, n, out=buf)),
+            ("normal_philox", lambda: va.normal_fill(va.PHILOX4X32, 7, 0, n, out=buf)),
+            ("normal_threefry", lambda: va.normal_fill(va.THREEFRY4X64, 7, 0, n, out=buf))):
+        ms = _events_ms(torch, fn, reps)
+        out[name] = {"n": n, "ms": ms, "variates_per_s": n / (ms / 1e3),
+                     "gbs": 8 * n / (ms / 1e3) / 1e9,
+                     "hbm_frac": 8 * n / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+    del buf
+    m = 1 << 24
+    g = torch.Generator(device="cuda")
+    for s in (0.5, 2.0, 8.0):
+        g.manual_seed(int(10 * s))
+        lw = torch.randn(m, generator=g, dtype=torch.float64, device="cuda") * s
+        p = va.Particle(m, 1, "float32", seed=3)
+        for sname in ("multinomial", "stratified", "systematic", "residual"):
+            sch = va.SCHEMES[sname]
+
+            def one():
+                p.set_log(lw)
+                p.resample_async(sch, float("inf"))
+            ms = _events_ms(torch, one, reps)
+            out[f"resample_s{s:g}_{sname}"] = {
+                "n": m, "ms": ms, "particles_per_s": m / (ms / 1e3),
+                "hbm_frac": m * (8 + 8 + 8 + 4 + 4 + 4 + 4) / (ms / 1e3) / 1e9
+                / HBM_PEAK_GBS}
+        p.close()
+    return out
+
+
+def measured_traffic():
+    """HBM bytes per move launch from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy kernel), if any."""
+    p = os.path.join(ROOT, "profiles", "cv_move_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as fh:
+        return json.load(fh).get("bytes_per_launch")
+
+
 def cpu_baseline(budget_s):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
@@ -128,6 +210,7 @@ def main():
     mk_ms = move_kernel_ms(va, torch, f, t, reps, ox, oy)
     f.close()
     torch.cuda.synchronize()
+    copy_gbs = copy_bandwidth(torch)
 
     secs = ms / 1e3
     value = a.n * a.steps / secs
@@ -153,9 +236,11 @@ def main():
         "step_hbm_frac": bytes_alg / secs / 1e9 / HBM_PEAK_GBS,
         "resampled_steps": nres,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(),
                      "kernel": "k_cv_step<float> + weight merge",
                      "kernel_ms": mk_ms},
+        "copy_gbs": copy_gbs,
     }
     if not a.no_extra:
         extra = {}
@@ -170,6 +255,7 @@ def main():
                            "step_hbm_frac": a.n * (MOVE_BYTES * 20 + RESAMPLE_BYTES * nres2)
                            / (ms2 / 1e3) / 1e9 / HBM_PEAK_GBS}
         out["extra"] = extra
+        out["microbench"] = microbench(va, torch)
     out["cpu_baseline"] = None if a.no_cpu else cpu_baseline(a.cpu_seconds)
     print(json.dumps(out), flush=True)
 

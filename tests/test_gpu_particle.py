@@ -134,3 +134,53 @@ def test_pf_fp32_large(dev):
     h.close()
     assert np.isfinite(g["logz"]).all()
     assert abs(g["logz"][-1] - k["logz"][-1]) < 2.0
+
+
+def _cv_normals_f32(words):
+    """The FP32 path's Box-Muller pairs in FP64 from the same words: FP32
+    uniforms (u >> 8) 2^-24 (u01.hpp:126-147), radius sqrt(-2 log ua),
+    angle 2 pi ub."""
+    u = (words >> 8).astype(np.float64) * 2.0 ** -24
+    r0, a0 = np.sqrt(-2 * np.log(u[0])), 2 * np.pi * u[1]
+    r1, a1 = np.sqrt(-2 * np.log(u[2])), 2 * np.pi * u[3]
+    return r0 * np.cos(a0), r0 * np.sin(a0), r1 * np.cos(a1), r1 * np.sin(a1)
+
+
+def _student_ll(x, y, ox, oy, scale=0.1, nu=10.0):
+    rx, ry = (x - ox) / scale, (y - oy) / scale
+    return -0.5 * (nu + 1) * (np.log1p(rx * rx / nu) + np.log1p(ry * ry / nu))
+
+
+@pytest.mark.parametrize("n", [4096, 10002])
+def test_cv_fp32_step_math(dev, n):
+    """FP32 state: init and one move against the model equations
+    (pf_seq.cpp:70-81, :111-122) evaluated in FP64 from the same Philox
+    words: state within FP32 rounding of the arithmetic (2e-6), log-weights
+    within 5e-5. n = 10002 also runs the scalar tail path."""
+    import vsmc_amd as va
+    ox, oy = c0_obs()
+    f = va.CvFilter(n, "float32", seed=9)
+    k0 = f.keys()[0]
+    keys = (k0 - 1 + np.arange(n, dtype=np.int64)) % 0xFFFFFFFF + 1
+    f.init(float(ox[0]), float(oy[0]))
+    s0 = _np(f.state()).astype(np.float64)
+    W0 = _np(f.weights())
+    f.move(1, float(ox[1]), float(oy[1]))
+    s1 = _np(f.state()).astype(np.float64)
+    W1 = _np(f.weights())
+    f.close()
+    w = np.array([ol.philox_words(int(k), 0, 8) for k in keys], dtype=np.uint64).T
+    z = _cv_normals_f32(w[0:4])
+    e0 = np.stack([2.0 * z[0], 2.0 * z[1], 1.0 * z[2], 1.0 * z[3]])
+    np.testing.assert_allclose(s0, e0, rtol=2e-6, atol=2e-6)
+    z = _cv_normals_f32(w[4:8])
+    e1 = np.stack([s0[0] + (0.02 * z[0] + 0.1 * s0[2]), s0[1] + (0.02 * z[1] + 0.1 * s0[3]),
+                   s0[2] + 0.02 * z[2], s0[3] + 0.02 * z[3]])
+    np.testing.assert_allclose(s1, e1, rtol=2e-6, atol=2e-6)
+    l0 = _student_ll(s0[0], s0[1], ox[0], oy[0])
+    l1 = l0 + _student_ll(s1[0], s1[1], ox[1], oy[1])
+    for W, l in ((W0, l0), (W1, l1)):
+        lw = l - l.max()
+        lw -= np.log(np.exp(lw).sum())
+        keep = lw > -600
+        np.testing.assert_allclose(np.log(W[keep]), lw[keep], rtol=0, atol=5e-5)

@@ -34,6 +34,7 @@ CvModelDev cv_dev(const vsmch_cv_model *m)
     d.obs_scale = m->obs_scale;
     d.inv_scale = 1 / m->obs_scale;
     d.obs_df = m->obs_df;
+    d.inv_df = 1 / m->obs_df;
     d.obs_model = m->obs_model;
     const double two_pi = 6.283185307179586476925286766559005768394;
     if (m->obs_model == 1) {

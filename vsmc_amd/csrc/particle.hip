@@ -88,12 +88,31 @@ __global__ void __launch_bounds__(kPB) k_weight_pass(double *l,
 __global__ void __launch_bounds__(kPB) k_weight_finalize(const Part3 *part,
     long long nblocks, long long n, int mode, PDev *pd)
 {
-    __shared__ Part3 lds[kWaves];
+    __shared__ double dlds[kWaves];
     Part3 acc{-INFINITY, 0.0, 0.0};
     if (mode != 2) {
-        for (long long b = threadIdx.x; b < nblocks; b += kPB)
-            acc = part_merge(acc, part[b]);
-        acc = block_reduce(acc, OpPart(), lds);
+        // the maximum first, then independent scaled sums (no serial chain
+        // of exponentials through a running maximum)
+        double mx = -INFINITY;
+        for (long long b = threadIdx.x; b < nblocks; b += kPB) {
+            const double m = part[b].m;
+            mx = m > mx ? m : mx;
+        }
+        mx = block_reduce(mx, OpMaxD(), dlds);
+        double s = 0.0, q = 0.0;
+        if (mx > -INFINITY) {
+            for (long long b = threadIdx.x; b < nblocks; b += kPB) {
+                const Part3 p = part[b];
+                if (p.m > -INFINITY) {
+                    const double r = exp(p.m - mx);
+                    s += p.s * r;
+                    q += p.q * r * r;
+                }
+            }
+        }
+        acc = Part3{mx, s, q};
+        acc.s = block_reduce(acc.s, OpAddD(), dlds);
+        acc.q = block_reduce(acc.q, OpAddD(), dlds);
     }
     if (threadIdx.x != 0)
         return;
@@ -310,13 +329,27 @@ __device__ inline float rmul<float>(float a, float b) { return __fmul_rn(a, b); 
 // log-likelihood of example/pf/include/pf.hpp:78-87 (Student-t with the
 // reference's dropped constants), optionally with the constants, or
 // Gaussian. Evaluated in Real.
+//
+// FP64 state follows the reference's operations one for one. FP32 state is
+// a GPU-only configuration with no reference counterpart to match bit for
+// bit; there the two logarithms use the hardware log2 (v_log_f32, about
+// 1e-7 absolute error on log-weights whose scale is 1) and the divisions
+// become products with precomputed reciprocals.
+__device__ inline double ll_log(double v) { return log(v); }
+__device__ inline float ll_log(float v) { return __logf(v); }
+__device__ inline double ll_div(double v, double d, double) { return v / d; }
+__device__ inline float ll_div(float v, double, double inv)
+{
+    return v * static_cast<float>(inv);
+}
+
 template <typename Real>
 __device__ inline Real cv_loglik(const CvModelDev &m, Real x, Real y, Real ox,
     Real oy)
 {
     if (m.obs_model == 2) {
-        const Real rx = (x - ox) / static_cast<Real>(m.obs_scale);
-        const Real ry = (y - oy) / static_cast<Real>(m.obs_scale);
+        const Real rx = ll_div(static_cast<Real>(x - ox), m.obs_scale, m.inv_scale);
+        const Real ry = ll_div(static_cast<Real>(y - oy), m.obs_scale, m.inv_scale);
         return static_cast<Real>(-0.5) * (rx * rx + ry * ry) +
             static_cast<Real>(m.ll_const);
     }
@@ -324,8 +357,8 @@ __device__ inline Real cv_loglik(const CvModelDev &m, Real x, Real y, Real ox,
     const Real nu = static_cast<Real>(m.obs_df);
     Real lx = rmul(scale, static_cast<Real>(x - ox));
     Real ly = rmul(scale, static_cast<Real>(y - oy));
-    lx = log(radd(static_cast<Real>(1), rmul(lx, lx) / nu));
-    ly = log(radd(static_cast<Real>(1), rmul(ly, ly) / nu));
+    lx = ll_log(radd(static_cast<Real>(1), ll_div(rmul(lx, lx), m.obs_df, m.inv_df)));
+    ly = ll_log(radd(static_cast<Real>(1), ll_div(rmul(ly, ly), m.obs_df, m.inv_df)));
     const Real c = rmul(static_cast<Real>(-0.5), radd(nu, static_cast<Real>(1)));
     Real ll = rmul(c, radd(lx, ly));
     if (m.obs_model == 1)
@@ -352,14 +385,32 @@ __device__ inline uint32_t next_key(uint32_t k)
     return k == 0xFFFFFFFFu ? 1u : k + 1u;
 }
 
+// A Box-Muller pair of the model's noise. FP64: the reference's
+// NormalDistribution formula. FP32: the same radius; the angle's sine and
+// cosine come from sincospif(2u), i.e. sin(2 pi u) without first rounding
+// 2 pi u to float (no reference counterpart: FP32 state is GPU-only).
+template <typename Real>
+__device__ inline normal_pair<Real> cv_normals(uint32_t wr, uint32_t wa)
+{
+    return box_muller<Real>(wr, wa);
+}
+template <>
+__device__ inline normal_pair<float> cv_normals<float>(uint32_t wr, uint32_t wa)
+{
+    const float r = sqrtf(-2.0f * logf(u01_u32f(wr)));
+    float s, c;
+    sincospif(2.0f * u01_u32f(wa), &s, &c);
+    return normal_pair<float>{r * c, r * s};
+}
+
 // One particle's move (or initialisation) from its Philox block `blk`.
 template <typename Real, bool kInit>
 __device__ inline void cv_advance(const CvModelDev &m, uint32_t key,
     unsigned long long blk, Real &x, Real &y, Real &vx, Real &vy)
 {
     const u32x4 wd = philox_block(key, blk);
-    const normal_pair<Real> p0 = box_muller<Real>(wd.v[0], wd.v[1]);
-    const normal_pair<Real> p1 = box_muller<Real>(wd.v[2], wd.v[3]);
+    const normal_pair<Real> p0 = cv_normals<Real>(wd.v[0], wd.v[1]);
+    const normal_pair<Real> p1 = cv_normals<Real>(wd.v[2], wd.v[3]);
     if (kInit) {
         x = rmul(static_cast<Real>(m.sd_pos0), p0.z0);
         y = rmul(static_cast<Real>(m.sd_pos0), p0.z1);
@@ -720,9 +771,10 @@ int moments(ParticleSet *p, int c0, int dim, double *out_dev)
     return kOk;
 }
 
-// Runs of 4 particles whose loads a move thread issues together: 2 for
-// FP32 state, 1 for FP64 (register budget); VSMCH_MOVE_BATCH (1, 2 or 4)
-// overrides it for tuning.
+// Runs of 4 particles whose loads a move thread issues together (1: the
+// move is ALU-heavy and more resident waves hide the loads better than
+// longer batches; measured on MI355X at N = 2^24). VSMCH_MOVE_BATCH (1, 2
+// or 4) overrides it for tuning.
 static int move_batch(int bytes)
 {
     static const int env = [] {
@@ -730,9 +782,8 @@ static int move_batch(int bytes)
         const int b = v ? atoi(v) : 0;
         return (b == 1 || b == 2 || b == 4) ? b : 0;
     }();
-    if (env)
-        return env;
-    return bytes == 4 ? 2 : 1;
+    (void)bytes;
+    return env ? env : 1;
 }
 
 template <typename Real, bool kInit>

@@ -26,7 +26,7 @@ struct CvModelDev {
     double dt;
     double sd_pos, sd_vel;
     double sd_pos0, sd_vel0;
-    double obs_scale, inv_scale, obs_df;
+    double obs_scale, inv_scale, obs_df, inv_df;
     double ll_const;   // normalising constant added for obs_model 1, 2
     int obs_model;
     int pad;
