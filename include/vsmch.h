@@ -234,6 +234,64 @@ vsmch_status vsmch_cv_run(vsmch_particle *p, const vsmch_cv_model *m,
     int scheme, double threshold, double *ess_host, int *resampled_host,
     double *pos_host, double *logz_host);
 
+/* ------------------------------------------------ learned proposal ----
+ * No counterpart in the reference, which has only the bootstrap proposal
+ * (the transition, user_guide/cpp/pf_seq.cpp:111-122). A 6 -> 64 -> 64 -> 8
+ * tanh network (bfloat16 weights and activations on the matrix cores, FP32
+ * accumulation) maps the features
+ *   ox - (x + dt vx), oy - (y + dt vy), vx, vy, ox - x, oy - y
+ * (normalised as (f - shift) * scale) to a diagonal Gaussian proposal in
+ * units of the transition: mean offsets o[0..3], log-scale offsets
+ * o[4..7]. The filter stays exact in expectation: each weight is corrected
+ * by log g + log f - log q. Parameter vector (FP32, VSMCH_MLP_PARAMS):
+ *   W1[64][6] | b1[64] | W2[64][64] | b2[64] | W3[8][64] | b3[8]. */
+#define VSMCH_MLP_PARAMS 5128
+
+typedef struct vsmch_mlp vsmch_mlp;
+
+/* New network with all parameters zero (the bootstrap proposal), shift 0,
+ * scale 1, on the library stream. */
+vsmch_status vsmch_mlp_create(vsmch_mlp **m);
+vsmch_status vsmch_mlp_destroy(vsmch_mlp *m);
+/* Host arrays; params may be NULL (kept), shift / scale of 6 (NULL: kept).
+ * Restarts the optimiser state. */
+vsmch_status vsmch_mlp_set_params(vsmch_mlp *m, const float *params_host,
+    const float *shift_host, const float *scale_host);
+vsmch_status vsmch_mlp_get_params(vsmch_mlp *m, float *params_host);
+/* Device arrays: feat[n][8] (6 used) -> out[n][8]. */
+vsmch_status vsmch_mlp_forward(vsmch_mlp *m, int64_t n, const float *feat,
+    float *out);
+/* Mean Gaussian negative log-likelihood of target[n][4] (standardised
+ * transitions) and its gradient: grad (device, VSMCH_MLP_PARAMS + 1
+ * floats, the loss last); loss_host may be NULL. */
+vsmch_status vsmch_mlp_loss_grad(vsmch_mlp *m, int64_t n, const float *feat,
+    const float *target, float *grad, float *loss_host);
+/* `steps` Adam steps (lr; betas 0.9, 0.999) on batches of `batch` of the n
+ * device samples (batch k mod n / batch: samples k + j n / batch); the
+ * losses before each step go to loss_host[steps] (may be NULL). */
+vsmch_status vsmch_mlp_train(vsmch_mlp *m, int64_t n, const float *feat,
+    const float *target, int64_t batch, int64_t steps, float lr,
+    float *loss_host);
+/* Training data from the bootstrap filter itself: runs vsmch_cv_run's loop
+ * (without histories) over T observations and, before each move t >= 1,
+ * writes one transition per particle from the current cloud with an
+ * observation drawn from the model: feat[(t - 1) n + i][8] and
+ * target[(t - 1) n + i][4] (device, (T - 1) n rows). FP32 state; a
+ * Student-t model needs an integer df in [1, 28]. */
+vsmch_status vsmch_cv_transitions(vsmch_particle *p, const vsmch_cv_model *m,
+    int64_t T, const double *obs_x_host, const double *obs_y_host,
+    int scheme, double threshold, uint32_t train_seed, float *feat,
+    float *target);
+/* One move at step iter >= 1 with the learned proposal, then the weight
+ * update (the counterpart of vsmch_cv_move). */
+vsmch_status vsmch_cv_move_learned(vsmch_particle *p, const vsmch_cv_model *m,
+    const vsmch_mlp *net, int64_t iter, double obs_x, double obs_y);
+/* vsmch_cv_run with the learned proposal for every move t >= 1. */
+vsmch_status vsmch_cv_run_learned(vsmch_particle *p, const vsmch_cv_model *m,
+    const vsmch_mlp *net, int64_t T, const double *obs_x_host,
+    const double *obs_y_host, int scheme, double threshold, double *ess_host,
+    int *resampled_host, double *pos_host, double *logz_host);
+
 #ifdef __cplusplus
 }
 #endif

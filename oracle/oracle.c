@@ -943,6 +943,17 @@ static double pf_loglik(const vo_pf_config *c, double x, double y, double ox,
     return ll;
 }
 
+double vo_cv_loglik(double x, double y, double ox, double oy, int obs_model,
+    double obs_scale, double obs_df)
+{
+    vo_pf_config c;
+    memset(&c, 0, sizeof(c));
+    c.obs_model = obs_model;
+    c.obs_scale = obs_scale;
+    c.obs_df = obs_df;
+    return pf_loglik(&c, x, y, ox, oy);
+}
+
 static double now_seconds(void)
 {
     struct timespec ts;

@@ -216,6 +216,27 @@ double vo_kalman_cv(int64_t T, double dt, double sd_pos, double sd_vel,
     double sd_pos0, double sd_vel0, double sd_obs, const double *obs_x,
     const double *obs_y, double *logz_out);
 
+/* Observation log-density of the PF model at position (x, y) (obs_model
+ * as in vo_pf_config). */
+double vo_cv_loglik(double x, double y, double ox, double oy, int obs_model,
+    double obs_scale, double obs_df);
+
+/* ---------------------------------------- learned proposal (mlp.c) -- */
+
+int vo_mlp_nparams(void);
+double vo_bf16(double x);
+void vo_cv_features(double x, double y, double vx, double vy, double ox,
+    double oy, double dt, double f[6]);
+void vo_mlp_forward(const float *params, const double *shift,
+    const double *scale, int64_t n, const double *feat, int round, double *out);
+double vo_mlp_loss_grad(const float *params, const double *shift,
+    const double *scale, int64_t n, const double *feat, const double *target,
+    int round, double *grad);
+double vo_cv_learned_particle(const double s0[4], double ox, double oy,
+    const double o[8], const double z[4], double dt, double sd_pos,
+    double sd_vel, int obs_model, double obs_scale, double obs_df,
+    double s1[4]);
+
 #ifdef __cplusplus
 }
 #endif

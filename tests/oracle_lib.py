@@ -279,3 +279,81 @@ def cumsum(w):
     s = np.zeros(max(len(w), 1))
     L.vo_cumsum(len(w), w if len(w) else np.zeros(1), s)
     return s[:len(w)]
+
+
+# ------------------------------------------------------ learned proposal
+
+MLP_NPARAMS = 5128
+MLP_OFF = {"W1": (0, (64, 6)), "b1": (384, (64,)), "W2": (448, (64, 64)),
+           "b2": (4544, (64,)), "W3": (4608, (8, 64)), "b3": (5120, (8,))}
+
+
+def _mlp_sig():
+    L = oracle()
+    if getattr(L, "_mlp_ready", False):
+        return L
+    L.vo_mlp_nparams.restype = c.c_int
+    L.vo_bf16.restype = c.c_double
+    L.vo_bf16.argtypes = [c.c_double]
+    L.vo_cv_features.restype = None
+    L.vo_cv_features.argtypes = [c.c_double] * 7 + [P(np.float64)]
+    L.vo_mlp_forward.restype = None
+    L.vo_mlp_forward.argtypes = [P(np.float32), P(np.float64), P(np.float64), c.c_int64,
+                                 P(np.float64), c.c_int, P(np.float64)]
+    L.vo_mlp_loss_grad.restype = c.c_double
+    L.vo_mlp_loss_grad.argtypes = [P(np.float32), P(np.float64), P(np.float64), c.c_int64,
+                                   P(np.float64), P(np.float64), c.c_int, P(np.float64)]
+    L.vo_cv_learned_particle.restype = c.c_double
+    L.vo_cv_learned_particle.argtypes = [P(np.float64), c.c_double, c.c_double,
+                                         P(np.float64), P(np.float64), c.c_double,
+                                         c.c_double, c.c_double, c.c_int, c.c_double,
+                                         c.c_double, P(np.float64)]
+    L.vo_cv_loglik.restype = c.c_double
+    L.vo_cv_loglik.argtypes = [c.c_double] * 4 + [c.c_int, c.c_double, c.c_double]
+    L._mlp_ready = True
+    return L
+
+
+def bf16(x):
+    return _mlp_sig().vo_bf16(float(x))
+
+
+def cv_features(x, y, vx, vy, ox, oy, dt=0.1):
+    f = np.zeros(6)
+    _mlp_sig().vo_cv_features(x, y, vx, vy, ox, oy, dt, f)
+    return f
+
+
+def mlp_forward(params, shift, scale, feat, round_bf16=True):
+    feat = np.ascontiguousarray(feat, np.float64).reshape(-1, 8)
+    out = np.zeros((len(feat), 8))
+    _mlp_sig().vo_mlp_forward(np.ascontiguousarray(params, np.float32),
+                              np.ascontiguousarray(shift, np.float64),
+                              np.ascontiguousarray(scale, np.float64), len(feat), feat,
+                              int(round_bf16), out)
+    return out
+
+
+def mlp_loss_grad(params, shift, scale, feat, target, round_bf16=True):
+    feat = np.ascontiguousarray(feat, np.float64).reshape(-1, 8)
+    target = np.ascontiguousarray(target, np.float64).reshape(-1, 4)
+    g = np.zeros(MLP_NPARAMS)
+    loss = _mlp_sig().vo_mlp_loss_grad(np.ascontiguousarray(params, np.float32),
+                                       np.ascontiguousarray(shift, np.float64),
+                                       np.ascontiguousarray(scale, np.float64), len(feat),
+                                       feat, target, int(round_bf16), g)
+    return loss, g
+
+
+def cv_learned_particle(s0, ox, oy, o, z, dt=0.1, sd_pos=0.02, sd_vel=0.02, obs_model=0,
+                        obs_scale=0.1, obs_df=10.0):
+    s1 = np.zeros(4)
+    inc = _mlp_sig().vo_cv_learned_particle(np.ascontiguousarray(s0, np.float64), ox, oy,
+                                            np.ascontiguousarray(o, np.float64),
+                                            np.ascontiguousarray(z, np.float64), dt, sd_pos,
+                                            sd_vel, obs_model, obs_scale, obs_df, s1)
+    return s1, inc
+
+
+def cv_loglik(x, y, ox, oy, obs_model=0, obs_scale=0.1, obs_df=10.0):
+    return _mlp_sig().vo_cv_loglik(x, y, ox, oy, obs_model, obs_scale, obs_df)

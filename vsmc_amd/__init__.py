@@ -79,7 +79,22 @@ SIGNATURES = {
     "vsmch_cv_move": (_c.c_int, [_P, _P, _c.c_int64, _c.c_double, _c.c_double]),
     "vsmch_cv_run": (_c.c_int, [_P, _P, _c.c_int64, _P, _P, _c.c_int, _c.c_double,
                                 _P, _P, _P, _P]),
+    "vsmch_mlp_create": (_c.c_int, [_c.POINTER(_P)]),
+    "vsmch_mlp_destroy": (_c.c_int, [_P]),
+    "vsmch_mlp_set_params": (_c.c_int, [_P, _P, _P, _P]),
+    "vsmch_mlp_get_params": (_c.c_int, [_P, _P]),
+    "vsmch_mlp_forward": (_c.c_int, [_P, _c.c_int64, _P, _P]),
+    "vsmch_mlp_loss_grad": (_c.c_int, [_P, _c.c_int64, _P, _P, _P, _P]),
+    "vsmch_mlp_train": (_c.c_int, [_P, _c.c_int64, _P, _P, _c.c_int64, _c.c_int64,
+                                   _c.c_float, _P]),
+    "vsmch_cv_transitions": (_c.c_int, [_P, _P, _c.c_int64, _P, _P, _c.c_int, _c.c_double,
+                                        _c.c_uint32, _P, _P]),
+    "vsmch_cv_move_learned": (_c.c_int, [_P, _P, _P, _c.c_int64, _c.c_double, _c.c_double]),
+    "vsmch_cv_run_learned": (_c.c_int, [_P, _P, _P, _c.c_int64, _P, _P, _c.c_int,
+                                        _c.c_double, _P, _P, _P, _P]),
 }
+
+MLP_PARAMS = 5128
 
 
 class ParticleConfig(ctypes.Structure):
@@ -263,4 +278,4 @@ def sorted_uniforms(n, key, word_offset=0, device="cuda"):
     return out[:n]
 
 
-from .sampler import Particle, CvFilter  # noqa: E402,F401
+from .sampler import Particle, CvFilter, ProposalNet  # noqa: E402,F401

@@ -1,5 +1,6 @@
 // capi.hip -- the extern "C" entry points declared in include/vsmch.h.
 #include "particle.hpp"
+#include "mlp.hpp"
 #include "../../include/vsmch.h"
 #include "../../include/vsmch/device_rng.hpp"
 
@@ -54,6 +55,18 @@ CvModelDev cv_dev(const vsmch_cv_model *m)
 struct vsmch_particle {
     ParticleSet *p;
 };
+
+struct vsmch_mlp {
+    MlpSet *m;
+};
+
+namespace {
+// The filter loop behind vsmch_cv_run and vsmch_cv_run_learned.
+vsmch_status run_filter(vsmch_particle *p, const vsmch_cv_model *m,
+    const MlpSet *net, int64_t T, const double *obs_x_host,
+    const double *obs_y_host, int scheme, double threshold, double *ess_host,
+    int *resampled_host, double *pos_host, double *logz_host);
+} // namespace
 
 extern "C" {
 
@@ -460,6 +473,115 @@ vsmch_status vsmch_cv_run(vsmch_particle *p, const vsmch_cv_model *m,
     int scheme, double threshold, double *ess_host, int *resampled_host,
     double *pos_host, double *logz_host)
 {
+    return run_filter(p, m, nullptr, T, obs_x_host, obs_y_host, scheme,
+        threshold, ess_host, resampled_host, pos_host, logz_host);
+}
+
+vsmch_status vsmch_mlp_create(vsmch_mlp **m)
+{
+    if (!m)
+        return VSMCH_ERR_INVALID;
+    MlpSet *s = nullptr;
+    const int rc = mlp_create(&s, g_stream);
+    if (rc != VSMCH_OK)
+        return rc;
+    *m = new vsmch_mlp{s};
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_mlp_destroy(vsmch_mlp *m)
+{
+    if (!m)
+        return VSMCH_ERR_INVALID;
+    mlp_destroy(m->m);
+    delete m;
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_mlp_set_params(vsmch_mlp *m, const float *params_host,
+    const float *shift_host, const float *scale_host)
+{
+    if (!m)
+        return VSMCH_ERR_INVALID;
+    return mlp_set_params(m->m, params_host, shift_host, scale_host);
+}
+
+vsmch_status vsmch_mlp_get_params(vsmch_mlp *m, float *params_host)
+{
+    if (!m || !params_host)
+        return VSMCH_ERR_INVALID;
+    return mlp_get_params(m->m, params_host);
+}
+
+vsmch_status vsmch_mlp_forward(vsmch_mlp *m, int64_t n, const float *feat,
+    float *out)
+{
+    if (!m || n < 0 || (n > 0 && (!feat || !out)))
+        return VSMCH_ERR_INVALID;
+    return mlp_forward(m->m, n, feat, out);
+}
+
+vsmch_status vsmch_mlp_loss_grad(vsmch_mlp *m, int64_t n, const float *feat,
+    const float *target, float *grad, float *loss_host)
+{
+    if (!m || n < 1 || !feat || !target || !grad)
+        return VSMCH_ERR_INVALID;
+    return mlp_loss_grad(m->m, n, feat, target, grad, loss_host);
+}
+
+vsmch_status vsmch_mlp_train(vsmch_mlp *m, int64_t n, const float *feat,
+    const float *target, int64_t batch, int64_t steps, float lr,
+    float *loss_host)
+{
+    if (!m || !feat || !target)
+        return VSMCH_ERR_INVALID;
+    return mlp_train(m->m, n, feat, target, batch, steps, lr, loss_host);
+}
+
+vsmch_status vsmch_cv_transitions(vsmch_particle *p, const vsmch_cv_model *m,
+    int64_t T, const double *obs_x_host, const double *obs_y_host,
+    int scheme, double threshold, uint32_t train_seed, float *feat,
+    float *target)
+{
+    if (!p || !m || T < 2 || !obs_x_host || !obs_y_host || scheme < 0 ||
+        scheme > 5 || !feat || !target)
+        return VSMCH_ERR_INVALID;
+    const int rc = cv_transitions(p->p, cv_dev(m), T, obs_x_host, obs_y_host,
+        scheme, threshold, train_seed, feat, target);
+    if (rc != VSMCH_OK)
+        return rc;
+    VSMCH_HIP(hipStreamSynchronize(p->p->ws.stream));
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_cv_move_learned(vsmch_particle *p, const vsmch_cv_model *m,
+    const vsmch_mlp *net, int64_t iter, double obs_x, double obs_y)
+{
+    if (!p || !m || !net || iter < 1)
+        return VSMCH_ERR_INVALID;
+    return cv_step_learned(p->p, cv_dev(m), net->m, iter, obs_x, obs_y);
+}
+
+vsmch_status vsmch_cv_run_learned(vsmch_particle *p, const vsmch_cv_model *m,
+    const vsmch_mlp *net, int64_t T, const double *obs_x_host,
+    const double *obs_y_host, int scheme, double threshold, double *ess_host,
+    int *resampled_host, double *pos_host, double *logz_host)
+{
+    if (!net)
+        return VSMCH_ERR_INVALID;
+    return run_filter(p, m, net->m, T, obs_x_host, obs_y_host, scheme,
+        threshold, ess_host, resampled_host, pos_host, logz_host);
+}
+
+} // extern "C"
+
+namespace {
+
+vsmch_status run_filter(vsmch_particle *p, const vsmch_cv_model *m,
+    const MlpSet *net, int64_t T, const double *obs_x_host,
+    const double *obs_y_host, int scheme, double threshold, double *ess_host,
+    int *resampled_host, double *pos_host, double *logz_host)
+{
     if (!p || !m || T < 1 || !obs_x_host || !obs_y_host || scheme < 0 ||
         scheme > 5)
         return VSMCH_ERR_INVALID;
@@ -467,12 +589,19 @@ vsmch_status vsmch_cv_run(vsmch_particle *p, const vsmch_cv_model *m,
     hipStream_t st = ps->ws.stream;
     double *ess_d = nullptr, *pos_d = nullptr, *logz_d = nullptr;
     int *rs_d = nullptr;
-    VSMCH_HIP(hipMalloc(&ess_d, sizeof(double) * T));
-    VSMCH_HIP(hipMalloc(&pos_d, sizeof(double) * 2 * T));
-    VSMCH_HIP(hipMalloc(&logz_d, sizeof(double) * T));
-    VSMCH_HIP(hipMalloc(&rs_d, sizeof(int) * T));
+    if (hipMalloc(&ess_d, sizeof(double) * T) != hipSuccess ||
+        hipMalloc(&pos_d, sizeof(double) * 2 * T) != hipSuccess ||
+        hipMalloc(&logz_d, sizeof(double) * T) != hipSuccess ||
+        hipMalloc(&rs_d, sizeof(int) * T) != hipSuccess) {
+        (void)hipFree(ess_d);
+        (void)hipFree(pos_d);
+        (void)hipFree(logz_d);
+        (void)hipFree(rs_d);
+        set_error("filter run: device allocation failed");
+        return VSMCH_ERR_ALLOC;
+    }
     int rc = cv_run(ps, cv_dev(m), T, obs_x_host, obs_y_host, scheme,
-        threshold, ess_d, rs_d, pos_d, logz_d);
+        threshold, ess_d, rs_d, pos_d, logz_d, net);
     if (rc == VSMCH_OK) {
         if (ess_host)
             (void)hipMemcpyAsync(ess_host, ess_d, sizeof(double) * T,
@@ -498,4 +627,4 @@ vsmch_status vsmch_cv_run(vsmch_particle *p, const vsmch_cv_model *m,
     return rc;
 }
 
-} // extern "C"
+} // namespace

@@ -218,13 +218,13 @@ __device__ inline void tile_exact(const double (&w)[kItems], double x0,
                 }
             }
         } else {
-            FMap agg = ident();
+            FMap magg = ident();
 #pragma unroll
             for (int j = 0; j < kItems; ++j)
                 if (base + j >= pos)
-                    agg = compose(agg, elem_map(w[j], e));
+                    magg = compose(magg, elem_map(w[j], e));
             FMap total;
-            FMap run = block_exclusive_scan(agg, OpCompose(), ident(), sh.scan,
+            FMap run = block_exclusive_scan(magg, OpCompose(), ident(), sh.scan,
                 &total);
 #pragma unroll
             for (int j = 0; j < kItems; ++j) {

@@ -12,6 +12,7 @@
 // produces per-block (max, sum exp, sum exp^2) partials; a one-block kernel
 // merges them into lmax, the sum, the ESS and the log-evidence increment.
 #include "particle.hpp"
+#include "mlp.hpp"
 #include "../../include/vsmch/device_rng.hpp"
 
 namespace vsmch {
@@ -576,6 +577,183 @@ __global__ void __launch_bounds__(kPB) k_cv_step(Real *__restrict__ state,
     }
 }
 
+// ------------------------------------------------- learned proposal
+
+// Move with the network's Gaussian proposal (FP32 state). Per particle:
+// features of (x_{t-1}, y_t), network outputs o (mean offsets o[0..3],
+// log-scale offsets o[4..7], clamped to [-5, 3]), the same four normals z
+// as the bootstrap move (Philox block t + 1 of the particle's stream),
+// u = o_mu + exp(ls) z, x_t = m + s u with the transition mean m and scales
+// s, and the weight increment log g + log f - log q
+// = ll(x_t) + sum_d (ls_d + z_d^2 / 2 - u_d^2 / 2).
+// A block is one 4096-particle tile (its weight partial feeds the same
+// merge and resampling tile prefix as the bootstrap move); each wave runs
+// the network on 4 column tiles of 16 particles per 64-particle chunk.
+__global__ void __launch_bounds__(kPB) k_cv_learned(float *__restrict__ state,
+    double *__restrict__ l, long long n, uint32_t seed, long long t, float ox,
+    float oy, CvModelDev m, const mlp::Frags *__restrict__ F, const PDev *pd,
+    Part3 *part)
+{
+    __shared__ float fl[kWaves][64][8];
+    __shared__ float ol[kWaves][64][8];
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int fo = 4 * (lane >> 4);
+    const long long base = static_cast<long long>(blockIdx.x) * kPTile;
+    float *X = state;
+    float *Y = state + n;
+    float *VX = state + 2 * n;
+    float *VY = state + 3 * n;
+    const double lw_equal = pd->log_equal;
+    const int equal = pd->equal;
+    const double c = pd->c;
+    const float dt = static_cast<float>(m.dt);
+    const float sp = static_cast<float>(m.sd_pos);
+    const float sv = static_cast<float>(m.sd_vel);
+    const unsigned long long blk = static_cast<unsigned long long>(t) + 1;
+    Part3 acc{-INFINITY, 0.0, 0.0};
+    for (int ch = 0; ch < kPTile / kPB; ++ch) {
+        const long long i = base + ch * kPB + wave * 64 + lane;
+        const bool valid = i < n;
+        float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f;
+        double lold = 0.0;
+        if (valid) {
+            x = X[i];
+            y = Y[i];
+            vx = VX[i];
+            vy = VY[i];
+            lold = equal ? lw_equal : l[i] - c;
+        }
+        const float mx = x + dt * vx, my = y + dt * vy;
+        fl[wave][lane][0] = ox - mx;
+        fl[wave][lane][1] = oy - my;
+        fl[wave][lane][2] = vx;
+        fl[wave][lane][3] = vy;
+        fl[wave][lane][4] = ox - x;
+        fl[wave][lane][5] = oy - y;
+        __syncthreads();
+#pragma unroll 1
+        for (int ct = 0; ct < 4; ++ct) {
+            const int pp = ct * 16 + (lane & 15);
+            float g[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                g[k] = fl[wave][pp][k];
+            mlp::Fwd r;
+            mlp::forward_tile(F, mlp::input_frag(F, g), r);
+            if (fo < 8) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    ol[wave][pp][fo + v] = r.o[v];
+            }
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t key = seed_key(seed, static_cast<unsigned long long>(i));
+            const u32x4 wd = philox_block(key, blk);
+            const normal_pair<float> p0 = cv_normals<float>(wd.v[0], wd.v[1]);
+            const normal_pair<float> p1 = cv_normals<float>(wd.v[2], wd.v[3]);
+            const float z[4] = {p0.z0, p0.z1, p1.z0, p1.z1};
+            const float mp[4] = {mx, my, vx, vy};
+            const float sc[4] = {sp, sp, sv, sv};
+            float s1[4];
+            float inc = 0.0f;
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                float ls = ol[wave][lane][4 + d];
+                ls = ls < -5.0f ? -5.0f : (ls > 3.0f ? 3.0f : ls);
+                const float u = ol[wave][lane][d] + expf(ls) * z[d];
+                s1[d] = mp[d] + sc[d] * u;
+                inc += ls + 0.5f * z[d] * z[d] - 0.5f * u * u;
+            }
+            X[i] = s1[0];
+            Y[i] = s1[1];
+            VX[i] = s1[2];
+            VY[i] = s1[3];
+            const float ll = cv_loglik<float>(m, s1[0], s1[1], ox, oy);
+            const double lnew = lold + static_cast<double>(inc) +
+                static_cast<double>(ll);
+            l[i] = lnew;
+            acc = part_merge(acc, part_of(lnew));
+        }
+        __syncthreads();
+    }
+    block_partials(acc, part);
+}
+
+// Draws from the word stream (key, first word) in order: Box-Muller pairs
+// (FP32 uniforms) for the training transitions.
+struct WordStream {
+    uint32_t key;
+    unsigned long long word;
+    __device__ uint32_t next() { return philox_word(key, word++); }
+    __device__ float normal(float &cache, bool &have)
+    {
+        if (have) {
+            have = false;
+            return cache;
+        }
+        const uint32_t a = next();
+        const uint32_t b = next();
+        const normal_pair<float> p = cv_normals<float>(a, b);
+        cache = p.z1;
+        have = true;
+        return p.z0;
+    }
+};
+
+// Training transitions from the current cloud (x_{t-1}): x_t ~ f, an
+// observation y ~ g(. | x_t) of the model (Gaussian, or Student-t with df
+// integer <= 28 as z / sqrt(chi2_df / df)), features of (x_{t-1}, y) and
+// the standardised target z = (x_t - m) / s. Stream: key
+// seed_key(train_seed, i), 64 words per step from word 64 t.
+__global__ void __launch_bounds__(kPB) k_cv_transitions(const float *state,
+    long long n, CvModelDev m, uint32_t train_seed, long long t, float *feat,
+    float *target)
+{
+    const long long i = static_cast<long long>(blockIdx.x) * kPB + threadIdx.x;
+    if (i >= n)
+        return;
+    const float x = state[i], y = state[n + i];
+    const float vx = state[2 * n + i], vy = state[3 * n + i];
+    WordStream ws{seed_key(train_seed, static_cast<unsigned long long>(i)),
+        64ULL * static_cast<unsigned long long>(t)};
+    float cache = 0.0f;
+    bool have = false;
+    float z[4];
+    for (int d = 0; d < 4; ++d)
+        z[d] = ws.normal(cache, have);
+    const float dt = static_cast<float>(m.dt);
+    const float px = x + dt * vx + static_cast<float>(m.sd_pos) * z[0];
+    const float py = y + dt * vy + static_cast<float>(m.sd_pos) * z[1];
+    float e[2];
+    for (int k = 0; k < 2; ++k) {
+        float v = ws.normal(cache, have);
+        if (m.obs_model != 2) {
+            const int df = static_cast<int>(m.obs_df);
+            float chi = 0.0f;
+            for (int q = 0; q < df; ++q) {
+                const float g = ws.normal(cache, have);
+                chi += g * g;
+            }
+            v = v * sqrtf(static_cast<float>(df) / chi);
+        }
+        e[k] = static_cast<float>(m.obs_scale) * v;
+    }
+    const float ox = px + e[0], oy = py + e[1];
+    float *f = feat + i * 8;
+    f[0] = ox - (x + dt * vx);
+    f[1] = oy - (y + dt * vy);
+    f[2] = vx;
+    f[3] = vy;
+    f[4] = ox - x;
+    f[5] = oy - y;
+    f[6] = 0.0f;
+    f[7] = 0.0f;
+    for (int d = 0; d < 4; ++d)
+        target[i * 4 + d] = z[d];
+}
+
 // ------------------------------------------------------------ host side
 
 static long long pblocks(long long n)
@@ -843,19 +1021,95 @@ __global__ void k_store_scalar(const PDev *pd, double *logz_hist, long long t)
     logz_hist[t] = pd->logz;
 }
 
+static int check_learned(const ParticleSet *p, const CvModelDev &m)
+{
+    if (p->dim != 4 || p->bytes != 4) {
+        set_error("learned proposal needs dim == 4 and FP32 state");
+        return kErrInvalid;
+    }
+    if (m.obs_model != 2 && !(m.obs_df >= 1 && m.obs_df <= 28 &&
+            m.obs_df == static_cast<double>(static_cast<int>(m.obs_df)))) {
+        set_error("learned proposal: Student-t df must be an integer in [1, 28]");
+        return kErrInvalid;
+    }
+    return kOk;
+}
+
+int cv_step_learned(ParticleSet *p, const CvModelDev &m, const MlpSet *net,
+    long long t, double ox, double oy)
+{
+    int rc = check_learned(p, m);
+    if (rc != kOk)
+        return rc;
+    if (t < 1) {
+        set_error("learned proposal: moves start at t = 1");
+        return kErrInvalid;
+    }
+    hipStream_t st = p->ws.stream;
+    const long long nb = pblocks(p->n);
+    hipLaunchKernelGGL(k_cv_learned, dim3(nb), dim3(kPB), 0, st,
+        static_cast<float *>(p->state), p->logw, p->n, p->seed, t,
+        static_cast<float>(ox), static_cast<float>(oy), m,
+        static_cast<const mlp::Frags *>(net->frags), static_cast<const PDev *>(p->pd),
+        static_cast<Part3 *>(p->part));
+    hipLaunchKernelGGL(k_weight_finalize, dim3(1), dim3(kPB), 0, st,
+        static_cast<const Part3 *>(p->part), nb, p->n, 1, p->pd);
+    VSMCH_HIP(hipGetLastError());
+    return kOk;
+}
+
+int cv_transitions(ParticleSet *p, const CvModelDev &m, long long T,
+    const double *ox, const double *oy, int scheme, double threshold,
+    uint32_t train_seed, float *feat, float *target)
+{
+    int rc = check_learned(p, m);
+    if (rc != kOk)
+        return rc;
+    hipStream_t st = p->ws.stream;
+    const long long g = (p->n + kPB - 1) / kPB;
+    for (long long t = 0; t < T; ++t) {
+        if (t > 0) {
+            const long long row = (t - 1) * p->n;
+            hipLaunchKernelGGL(k_cv_transitions, dim3(g), dim3(kPB), 0, st,
+                static_cast<const float *>(p->state), p->n, m, train_seed, t,
+                feat + row * 8, target + row * 4);
+        }
+        rc = cv_step(p, m, t, ox[t], oy[t], false);
+        if (rc != kOk)
+            return rc;
+        rc = resample_step(p, scheme, threshold, t, nullptr, nullptr, nullptr,
+            nullptr);
+        if (rc != kOk)
+            return rc;
+    }
+    VSMCH_HIP(hipGetLastError());
+    return kOk;
+}
+
 int cv_run(ParticleSet *p, const CvModelDev &m, long long T,
     const double *ox, const double *oy, int scheme, double threshold,
-    double *ess_d, int *rs_d, double *pos_d, double *logz_d)
+    double *ess_d, int *rs_d, double *pos_d, double *logz_d,
+    const MlpSet *net)
 {
     hipStream_t st = p->ws.stream;
     const long long nb = pblocks(p->n);
     int rc;
-    for (long long t = 0; t < T; ++t) {
-        // the move of step t also yields the "pos" record of step t - 1
-        rc = cv_step(p, m, t, ox[t], oy[t], pos_d != nullptr && t > 0);
+    if (net) {
+        rc = check_learned(p, m);
         if (rc != kOk)
             return rc;
-        if (pos_d && t > 0)
+    }
+    for (long long t = 0; t < T; ++t) {
+        if (net && t > 0) {
+            rc = cv_step_learned(p, m, net, t, ox[t], oy[t]);
+        } else {
+            // the bootstrap move of step t also yields the "pos" record of
+            // step t - 1
+            rc = cv_step(p, m, t, ox[t], oy[t], !net && pos_d != nullptr && t > 0);
+        }
+        if (rc != kOk)
+            return rc;
+        if (!net && pos_d && t > 0)
             hipLaunchKernelGGL(k_moments_finalize, dim3(1), dim3(kPB), 0, st,
                 p->mpart, nb, 2, pos_d + 2 * (t - 1));
         if (logz_d)
@@ -865,6 +1119,11 @@ int cv_run(ParticleSet *p, const CvModelDev &m, long long T,
             nullptr);
         if (rc != kOk)
             return rc;
+        if (net && pos_d && t + 1 < T) {
+            rc = moments(p, 0, 2, pos_d + 2 * t);
+            if (rc != kOk)
+                return rc;
+        }
     }
     if (pos_d && T > 0) {
         rc = moments(p, 0, 2, pos_d + 2 * (T - 1));

@@ -62,9 +62,20 @@ int resample_step(ParticleSet *p, int scheme, double threshold, long long t,
 int moments(ParticleSet *p, int c0, int dim, double *out_dev);
 int cv_step(ParticleSet *p, const CvModelDev &m, long long t, double ox,
     double oy, bool with_monitor);
+struct MlpSet;
+// net: the learned proposal for the moves t >= 1 (null: bootstrap)
 int cv_run(ParticleSet *p, const CvModelDev &m, long long T,
     const double *ox, const double *oy, int scheme, double threshold,
-    double *ess_d, int *rs_d, double *pos_d, double *logz_d);
+    double *ess_d, int *rs_d, double *pos_d, double *logz_d,
+    const MlpSet *net = nullptr);
+int cv_step_learned(ParticleSet *p, const CvModelDev &m, const MlpSet *net,
+    long long t, double ox, double oy);
+// Runs the bootstrap filter over the observations and, before each move
+// t >= 1, writes n training transitions from the current cloud:
+// feat[(t - 1) n + i][8], target[(t - 1) n + i][4] (device).
+int cv_transitions(ParticleSet *p, const CvModelDev &m, long long T,
+    const double *ox, const double *oy, int scheme, double threshold,
+    uint32_t train_seed, float *feat, float *target);
 
 // rng.hip
 int stream_fill(int kind, unsigned long long key, unsigned long long offset,

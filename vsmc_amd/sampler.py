@@ -160,3 +160,132 @@ class CvFilter(Particle):
         check(lib().vsmch_cv_run(self._h, ctypes.byref(self.model), T, P(ox), P(oy),
                                  scheme, threshold, P(ess), P(rs), P(pos), P(logz)))
         return {"ess": ess, "resampled": rs, "pos": pos.reshape(T, 2), "logz": logz}
+
+    def move_learned(self, net, t, ox, oy):
+        check(lib().vsmch_cv_move_learned(self._h, ctypes.byref(self.model), net._h, t,
+                                          ox, oy))
+
+    def run_learned(self, net, obs_x, obs_y, scheme, threshold):
+        """run() with the learned proposal for every move t >= 1."""
+        import numpy as np
+        T = len(obs_x)
+        ox = np.ascontiguousarray(obs_x, dtype=np.float64)
+        oy = np.ascontiguousarray(obs_y, dtype=np.float64)
+        ess = np.zeros(T)
+        rs = np.zeros(T, dtype=np.int32)
+        pos = np.zeros(2 * T)
+        logz = np.zeros(T)
+        P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        check(lib().vsmch_cv_run_learned(self._h, ctypes.byref(self.model), net._h, T,
+                                         P(ox), P(oy), scheme, threshold, P(ess), P(rs),
+                                         P(pos), P(logz)))
+        return {"ess": ess, "resampled": rs, "pos": pos.reshape(T, 2), "logz": logz}
+
+    def transitions(self, obs_x, obs_y, scheme, threshold, train_seed):
+        """Training transitions from this bootstrap filter's own clouds over
+        the observations: (feat[(T-1) n, 8], target[(T-1) n, 4]) on the
+        device."""
+        import numpy as np
+        import torch
+        T = len(obs_x)
+        ox = np.ascontiguousarray(obs_x, dtype=np.float64)
+        oy = np.ascontiguousarray(obs_y, dtype=np.float64)
+        rows = (T - 1) * self.n
+        feat = torch.empty((rows, 8), dtype=torch.float32, device="cuda")
+        target = torch.empty((rows, 4), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        check(lib().vsmch_cv_transitions(self._h, ctypes.byref(self.model), T, P(ox), P(oy),
+                                         scheme, threshold, train_seed,
+                                         ctypes.c_void_p(feat.data_ptr()),
+                                         ctypes.c_void_p(target.data_ptr())))
+        return feat, target
+
+
+class ProposalNet:
+    """The learned proposal's network (6 -> 64 -> 64 -> 8, bfloat16 matrix
+    cores): parameters, forward, loss gradient and Adam training, all on the
+    device through the C ABI."""
+
+    NPARAMS = 5128
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        check(lib().vsmch_mlp_create(ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            check(lib().vsmch_mlp_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params=None, shift=None, scale=None):
+        import numpy as np
+        P = lambda a: None if a is None else np.ascontiguousarray(a, np.float32).ctypes.data_as(ctypes.c_void_p)  # noqa: E731,E501
+        keep = [None if a is None else np.ascontiguousarray(a, np.float32)
+                for a in (params, shift, scale)]
+        check(lib().vsmch_mlp_set_params(self._h, *[P(a) for a in keep]))
+
+    def params(self):
+        import numpy as np
+        out = np.zeros(self.NPARAMS, dtype=np.float32)
+        check(lib().vsmch_mlp_get_params(self._h, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def forward(self, feat):
+        import torch
+        feat = feat.to(device="cuda", dtype=torch.float32).contiguous()
+        out = torch.empty((feat.shape[0], 8), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        check(lib().vsmch_mlp_forward(self._h, feat.shape[0], ctypes.c_void_p(feat.data_ptr()),
+                                      ctypes.c_void_p(out.data_ptr())))
+        torch.cuda.synchronize()
+        return out
+
+    def loss_grad(self, feat, target):
+        import torch
+        feat = feat.to(device="cuda", dtype=torch.float32).contiguous()
+        target = target.to(device="cuda", dtype=torch.float32).contiguous()
+        g = torch.empty(self.NPARAMS + 1, dtype=torch.float32, device="cuda")
+        loss = ctypes.c_float()
+        torch.cuda.synchronize()
+        check(lib().vsmch_mlp_loss_grad(self._h, feat.shape[0],
+                                        ctypes.c_void_p(feat.data_ptr()),
+                                        ctypes.c_void_p(target.data_ptr()),
+                                        ctypes.c_void_p(g.data_ptr()), ctypes.byref(loss)))
+        return loss.value, g[:self.NPARAMS].cpu().numpy()
+
+    def init_random(self, seed=0, gain=1.0):
+        """Hidden layers N(0, gain^2 / fan_in), biases and the last layer 0:
+        the proposal starts as the transition (bootstrap) and, unlike an
+        all-zero network, the hidden layers receive gradient."""
+        import numpy as np
+        rs = np.random.default_rng(seed)
+        p = np.zeros(self.NPARAMS, dtype=np.float32)
+        p[0:384] = rs.normal(0, gain / np.sqrt(6), 384)
+        p[448:4544] = rs.normal(0, gain / np.sqrt(64), 4096)
+        self.set_params(p)
+
+    def fit_normalisation(self, feat):
+        """Feature shift / scale from the training set (mean, 1 / std)."""
+        f = feat[:, :6].double()
+        shift = f.mean(dim=0).cpu().numpy()
+        sd = f.std(dim=0).cpu().numpy()
+        sd[sd <= 0] = 1.0
+        self.set_params(None, shift, 1.0 / sd)
+
+    def train(self, feat, target, batch=4096, steps=1000, lr=1e-3):
+        import numpy as np
+        import torch
+        loss = np.zeros(max(steps, 1), dtype=np.float32)
+        torch.cuda.synchronize()
+        check(lib().vsmch_mlp_train(self._h, feat.shape[0], ctypes.c_void_p(feat.data_ptr()),
+                                    ctypes.c_void_p(target.data_ptr()), batch, steps, lr,
+                                    loss.ctypes.data_as(ctypes.c_void_p)))
+        return loss[:steps]
