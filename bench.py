@@ -158,6 +158,120 @@ def microbench(va, torch, reps=5):
     return out
 
 
+LG_MODEL = dict(dt=0.1, sd_pos=0.02, sd_vel=0.02, sd_pos0=0.02, sd_vel0=0.02, obs_model=2,
+                obs_scale=0.01, obs_df=10.0)
+ST_MODEL = dict(dt=0.1, sd_pos=0.02, sd_vel=0.02, sd_pos0=0.02, sd_vel0=0.02, obs_model=1,
+                obs_scale=0.1, obs_df=10.0)
+
+
+def _timed(torch, fn):
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    r = fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return r, e0.elapsed_time(e1) / 1e3
+
+
+def _train_net(va, torch, model, track, n=1 << 14, steps=600):
+    ox, oy = track
+    f = va.CvFilter(n, "float32", seed=5, **model)
+    (feat, target), s_data = _timed(torch, lambda: f.transitions(ox, oy, va.SYSTEMATIC, 0.5,
+                                                                 train_seed=77))
+    f.close()
+    net = va.ProposalNet()
+    net.fit_normalisation(feat)
+    net.init_random(1)
+    loss, s_train = _timed(torch, lambda: net.train(feat, target, batch=4096, steps=steps,
+                                                    lr=2e-3))
+    return net, {"samples": int(feat.shape[0]), "steps": steps, "batch": 4096,
+                 "data_gpu_s": s_data, "train_gpu_s": s_train,
+                 "loss_first": float(loss[0]), "loss_last20": float(loss[-20:].mean())}
+
+
+def learned_vs_plain(va, torch, tracks=16, T=50, sizes=(1 << 12, 1 << 14, 1 << 16, 1 << 18,
+                                                           1 << 20)):
+    """log-evidence RMSE against the FP64 Kalman filter vs GPU-seconds per
+    filter run, learned proposal vs bootstrap, on the linear-Gaussian CV
+    model (obs sd 0.01), T = 50, systematic resampling at ESS < N/2; plus
+    the Student-t model with 5% outliers at 10x scale against a 2^22-particle
+    bootstrap reference."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import oracle_lib as ol
+    from vsmc_amd.models import cv_simulate
+
+    def track(seed, kind):
+        if kind == "lg":
+            _, ox, oy = cv_simulate(T, seed, sd_pos0=0.02, sd_vel0=0.02, obs="gauss",
+                                    obs_scale=0.01)
+        else:
+            _, ox, oy = cv_simulate(T, seed, sd_pos0=0.02, sd_vel0=0.02, obs="student",
+                                    obs_scale=0.1, obs_df=10.0, outlier_frac=0.05,
+                                    outlier_mult=10.0)
+        return ox, oy
+
+    def run(model, n, seed, net, tr):
+        f = va.CvFilter(n, "float32", seed=seed, **model)
+        ox, oy = tr
+        if net is None:
+            r, s = _timed(torch, lambda: f.run(ox, oy, va.SYSTEMATIC, 0.5))
+        else:
+            r, s = _timed(torch, lambda: f.run_learned(net, ox, oy, va.SYSTEMATIC, 0.5))
+        f.close()
+        return float(r["logz"][-1]), s
+
+    out = {}
+    for kind, model in (("linear_gaussian", LG_MODEL), ("student_t_outliers", ST_MODEL)):
+        k = "lg" if kind == "linear_gaussian" else "st"
+        net, train = _train_net(va, torch, model, track(11, k))
+        trs = [track(1000 + s, k) for s in range(tracks)]
+        if k == "lg":
+            ref = [ol.kalman_cv(ox, oy, 0.1, 0.02, 0.02, 0.02, 0.02, 0.01)[0] for ox, oy in trs]
+            ref_desc = "FP64 Kalman filter"
+            szs = sizes
+        else:
+            ref = [run(model, 1 << 22, 9000 + s, None, trs[s])[0] for s in range(tracks)]
+            ref_desc = "bootstrap filter at N = 2^22"
+            szs = sizes[:4]
+        rows = []
+        for n in szs:
+            for name, nn in (("bootstrap", None), ("learned", net)):
+                errs, secs = [], []
+                for s in range(tracks):
+                    z, sec = run(model, n, s, nn, trs[s])
+                    errs.append(z - ref[s])
+                    secs.append(sec)
+                rows.append({"method": name, "n": n,
+                             "rmse": float(np.sqrt(np.mean(np.square(errs)))),
+                             "bias": float(np.mean(errs)),
+                             "gpu_s_per_run": float(np.mean(secs))})
+        # GPU-seconds the bootstrap filter needs for each learned RMSE,
+        # interpolated log-log along its (gpu_s, rmse) curve
+        bs = sorted([r for r in rows if r["method"] == "bootstrap"], key=lambda r: r["n"])
+        lx = np.log([r["gpu_s_per_run"] for r in bs])
+        ly = np.log([max(r["rmse"], 1e-12) for r in bs])
+        ratios = []
+        for r in rows:
+            if r["method"] != "learned":
+                continue
+            target = np.log(max(r["rmse"], 1e-12))
+            if target >= ly.max() or target <= ly.min():
+                ratios.append({"n": r["n"], "rmse": r["rmse"], "gpu_s_ratio": None})
+                continue
+            order = np.argsort(ly)
+            need = float(np.exp(np.interp(target, ly[order], lx[order])))
+            ratios.append({"n": r["n"], "rmse": r["rmse"],
+                           "bootstrap_gpu_s_same_rmse": need,
+                           "gpu_s_ratio": need / r["gpu_s_per_run"]})
+        out[kind] = {"reference": ref_desc, "tracks": tracks, "T": T, "training": train,
+                     "rows": rows, "equal_rmse": ratios}
+        net.close()
+    return out
+
+
 def measured_traffic():
     """HBM bytes per move launch from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy kernel), if any."""
@@ -256,6 +370,7 @@ def main():
                            / (ms2 / 1e3) / 1e9 / HBM_PEAK_GBS}
         out["extra"] = extra
         out["microbench"] = microbench(va, torch)
+        out["learned_proposal"] = learned_vs_plain(va, torch)
     out["cpu_baseline"] = None if a.no_cpu else cpu_baseline(a.cpu_seconds)
     print(json.dumps(out), flush=True)
 

@@ -199,3 +199,25 @@ def test_learned_pf_beats_bootstrap_lg(dev):
     assert rl < 0.5 * rb
     assert rl < 1.0
     net.close()
+
+
+def test_learned_pf_large_n(dev):
+    """N = 2^19 over 50 steps draws ~2^25 FP32 uniforms: some are exactly 0
+    (infinite Box-Muller radius). Such a particle gets weight 0 instead of
+    a NaN log-weight, and the log-evidence stays close to the Kalman
+    value; the monitor stays finite."""
+    import vsmc_amd as va
+    net, _, _, _ = _trained_lg_net(steps=300)
+    ox, oy = _lg_track(50, 1003)
+    kz, _ = ol.kalman_cv(ox, oy, 0.1, 0.02, 0.02, 0.02, 0.02, 0.01)
+    for learned in (False, True):
+        f = va.CvFilter(1 << 19, "float32", seed=8, **LG)
+        r = f.run_learned(net, ox, oy, va.SYSTEMATIC, 0.5) if learned else \
+            f.run(ox, oy, va.SYSTEMATIC, 0.5)
+        W = f.weights().cpu().numpy()
+        f.close()
+        print("learned" if learned else "bootstrap", r["logz"][-1] - kz, flush=True)
+        assert np.isfinite(W).all()
+        assert np.isfinite(r["pos"]).all()
+        assert abs(r["logz"][-1] - kz) < 0.5
+    net.close()

@@ -60,6 +60,16 @@ __device__ inline void block_partials(Part3 p, Part3 *out)
         out[blockIdx.x] = p;
 }
 
+// A log-weight that came out NaN (a non-finite state: a float uniform of
+// exactly 0 gives an infinite Box-Muller radius, probability 2^-24 per
+// draw) becomes -inf: the particle carries no weight and the next
+// resampling replaces it. Zero-weight particles also stay out of the
+// weighted sums (0 * inf would be NaN).
+__device__ inline double dead_if_nan(double l)
+{
+    return l == l ? l : -INFINITY;
+}
+
 // log W_i of the current weights.
 __device__ inline double logw_of(const double *l, long long i, const PDev *pd)
 {
@@ -277,8 +287,9 @@ __global__ void __launch_bounds__(kPB) k_moments(const Real *state,
         const long long i = base + j * kPB + threadIdx.x;
         if (i < n) {
             const double w = exp(logw_of(l, i, pd));
-            for (int d = 0; d < dim; ++d)
-                acc[d] += w * static_cast<double>(state[(c0 + d) * n + i]);
+            if (w > 0.0)
+                for (int d = 0; d < dim; ++d)
+                    acc[d] += w * static_cast<double>(state[(c0 + d) * n + i]);
         }
     }
     __shared__ double lds[kWaves];
@@ -509,8 +520,10 @@ __global__ void __launch_bounds__(kPB) k_cv_step(Real *__restrict__ state,
                         vy = vys[r].v[q];
                         if (mon) {
                             const double w = exp(lq);
-                            mx += w * static_cast<double>(x);
-                            my += w * static_cast<double>(y);
+                            if (w > 0.0) {
+                                mx += w * static_cast<double>(x);
+                                my += w * static_cast<double>(y);
+                            }
                         }
                     }
                     cv_advance<Real, kInit>(m, key, blk, x, y, vx, vy);
@@ -520,7 +533,7 @@ __global__ void __launch_bounds__(kPB) k_cv_step(Real *__restrict__ state,
                     vxs[r].v[q] = vx;
                     vys[r].v[q] = vy;
                     const double ll = static_cast<double>(cv_loglik<Real>(m, x, y, ox, oy));
-                    lq = kInit ? ll : lq + ll;
+                    lq = dead_if_nan(kInit ? ll : lq + ll);
                 }
                 *reinterpret_cast<Vec4<Real> *>(X + i0) = xs[r];
                 *reinterpret_cast<Vec4<Real> *>(Y + i0) = ys[r];
@@ -550,8 +563,10 @@ __global__ void __launch_bounds__(kPB) k_cv_step(Real *__restrict__ state,
                 lold = equal ? lw_equal : l[i] - c;
                 if (mon) {
                     const double w = exp(lold);
-                    mx += w * static_cast<double>(x);
-                    my += w * static_cast<double>(y);
+                    if (w > 0.0) {
+                        mx += w * static_cast<double>(x);
+                        my += w * static_cast<double>(y);
+                    }
                 }
             }
             cv_advance<Real, kInit>(m, key, blk, x, y, vx, vy);
@@ -560,7 +575,7 @@ __global__ void __launch_bounds__(kPB) k_cv_step(Real *__restrict__ state,
             VX[i] = vx;
             VY[i] = vy;
             const double ll = static_cast<double>(cv_loglik<Real>(m, x, y, ox, oy));
-            const double lnew = kInit ? ll : lold + ll;
+            const double lnew = dead_if_nan(kInit ? ll : lold + ll);
             l[i] = lnew;
             acc = part_merge(acc, part_of(lnew));
         }
@@ -671,8 +686,9 @@ __global__ void __launch_bounds__(kPB) k_cv_learned(float *__restrict__ state,
             VX[i] = s1[2];
             VY[i] = s1[3];
             const float ll = cv_loglik<float>(m, s1[0], s1[1], ox, oy);
-            const double lnew = lold + static_cast<double>(inc) +
-                static_cast<double>(ll);
+            // an infinite normal makes inc = inf - inf: the particle dies
+            const double lnew = dead_if_nan(lold + static_cast<double>(inc) +
+                static_cast<double>(ll));
             l[i] = lnew;
             acc = part_merge(acc, part_of(lnew));
         }
@@ -896,8 +912,9 @@ __global__ void __launch_bounds__(kPB) k_wsum(const double *buf,
         if (i < n) {
             const double w = pd->equal ? 1.0 / static_cast<double>(n) :
                                          exp(l[i] - pd->c);
-            for (int d = 0; d < nd; ++d)
-                acc[d] += w * buf[i * dim + d0 + d];
+            if (w > 0.0)
+                for (int d = 0; d < nd; ++d)
+                    acc[d] += w * buf[i * dim + d0 + d];
         }
     }
     __shared__ double lds[kWaves];
