@@ -286,6 +286,49 @@ vsmch_status vsmch_cv_transitions(vsmch_particle *p, const vsmch_cv_model *m,
  * update (the counterpart of vsmch_cv_move). */
 vsmch_status vsmch_cv_move_learned(vsmch_particle *p, const vsmch_cv_model *m,
     const vsmch_mlp *net, int64_t iter, double obs_x, double obs_y);
+/* ---------------------------------------------------- batched filters --
+ * No counterpart in the reference, which runs one Sampler per filter
+ * (include/vsmc/core/sampler.hpp:443-476, :735-785). Many independent
+ * filters of the model above, each with its own particle count, horizon
+ * and seed (particle keys and resampling engine as a vsmch_particle with
+ * that seed), advanced together: each step is one persistent move over all
+ * active filters, one merge per filter, one resampling workgroup per
+ * resampling filter (systematic or stratified, counts bit-exact with the
+ * single-filter path) and a per-filter update; a run replays one captured
+ * HIP graph per step after timing the move's grid on its first steps.
+ * Every filter's results equal those of vsmch_cv_run (FP32 state) with
+ * the same seed and observations. Filters advance in their own stream. */
+typedef struct vsmch_batch vsmch_batch;
+
+typedef struct {
+    int64_t n;        /* particles, >= 2 */
+    int64_t T;        /* steps incl. the initial one, >= 1 */
+    uint32_t seed;    /* Seed::instance().set(seed) before its Sampler */
+} vsmch_batch_filter;
+
+vsmch_status vsmch_batch_create(vsmch_batch **b, int nfilters,
+    const vsmch_batch_filter *filters_host, const vsmch_cv_model *m,
+    int scheme, double threshold);
+vsmch_status vsmch_batch_destroy(vsmch_batch *b);
+/* obs_host[f][t][2] (x, y) for t < Tmax = max_f T_f. */
+vsmch_status vsmch_batch_set_obs(vsmch_batch *b, const double *obs_host);
+/* Runs every filter from t = 0 to its horizon; net: learned proposal for
+ * t >= 1 (NULL: bootstrap); use_graph: replay a captured step. */
+vsmch_status vsmch_batch_run(vsmch_batch *b, const vsmch_mlp *net,
+    int use_graph);
+/* [nfilters][Tmax] host arrays (each may be NULL): cumulative log-evidence,
+ * ESS before resampling, resampled flags; entries t >= T_f are 0. */
+vsmch_status vsmch_batch_histories(vsmch_batch *b, double *logz_host,
+    double *ess_host, int *resampled_host);
+/* Filter f's final state (device, 4 x n, component stride n) and
+ * normalised weights (device, n); either may be NULL. */
+vsmch_status vsmch_batch_read_filter(vsmch_batch *b, int f, float *state,
+    double *weight);
+/* Tmax, padded particle count, the move grid the autotuner chose and the
+ * GPU time of the last run (ms); any may be NULL. */
+vsmch_status vsmch_batch_info(vsmch_batch *b, int64_t *tmax,
+    int64_t *particles, int *move_blocks, double *run_ms);
+
 /* vsmch_cv_run with the learned proposal for every move t >= 1. */
 vsmch_status vsmch_cv_run_learned(vsmch_particle *p, const vsmch_cv_model *m,
     const vsmch_mlp *net, int64_t T, const double *obs_x_host,

@@ -90,6 +90,15 @@ SIGNATURES = {
     "vsmch_cv_transitions": (_c.c_int, [_P, _P, _c.c_int64, _P, _P, _c.c_int, _c.c_double,
                                         _c.c_uint32, _P, _P]),
     "vsmch_cv_move_learned": (_c.c_int, [_P, _P, _P, _c.c_int64, _c.c_double, _c.c_double]),
+    "vsmch_batch_create": (_c.c_int, [_c.POINTER(_P), _c.c_int, _P, _P, _c.c_int,
+                                      _c.c_double]),
+    "vsmch_batch_destroy": (_c.c_int, [_P]),
+    "vsmch_batch_set_obs": (_c.c_int, [_P, _P]),
+    "vsmch_batch_run": (_c.c_int, [_P, _P, _c.c_int]),
+    "vsmch_batch_histories": (_c.c_int, [_P, _P, _P, _P]),
+    "vsmch_batch_read_filter": (_c.c_int, [_P, _c.c_int, _P, _P]),
+    "vsmch_batch_info": (_c.c_int, [_P, _c.POINTER(_c.c_int64), _c.POINTER(_c.c_int64),
+                                    _c.POINTER(_c.c_int), _c.POINTER(_c.c_double)]),
     "vsmch_cv_run_learned": (_c.c_int, [_P, _P, _P, _c.c_int64, _P, _P, _c.c_int,
                                         _c.c_double, _P, _P, _P, _P]),
 }
@@ -108,6 +117,11 @@ class CvModel(ctypes.Structure):
                 ("sd_vel", ctypes.c_double), ("sd_pos0", ctypes.c_double),
                 ("sd_vel0", ctypes.c_double), ("obs_model", ctypes.c_int),
                 ("obs_scale", ctypes.c_double), ("obs_df", ctypes.c_double)]
+
+
+class BatchFilter(ctypes.Structure):
+    """vsmch_batch_filter."""
+    _fields_ = [("n", ctypes.c_int64), ("T", ctypes.c_int64), ("seed", ctypes.c_uint32)]
 
 
 def cv_model(dt=0.1, sd_pos=0.02, sd_vel=0.02, sd_pos0=2.0, sd_vel0=1.0,
@@ -278,4 +292,4 @@ def sorted_uniforms(n, key, word_offset=0, device="cuda"):
     return out[:n]
 
 
-from .sampler import Particle, CvFilter, ProposalNet  # noqa: E402,F401
+from .sampler import Particle, CvFilter, ProposalNet, CvBatch  # noqa: E402,F401

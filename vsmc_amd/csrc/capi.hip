@@ -60,6 +60,10 @@ struct vsmch_mlp {
     MlpSet *m;
 };
 
+struct vsmch_batch {
+    BatchSet *b;
+};
+
 namespace {
 // The filter loop behind vsmch_cv_run and vsmch_cv_run_learned.
 vsmch_status run_filter(vsmch_particle *p, const vsmch_cv_model *m,
@@ -560,6 +564,84 @@ vsmch_status vsmch_cv_move_learned(vsmch_particle *p, const vsmch_cv_model *m,
     if (!p || !m || !net || iter < 1)
         return VSMCH_ERR_INVALID;
     return cv_step_learned(p->p, cv_dev(m), net->m, iter, obs_x, obs_y);
+}
+
+vsmch_status vsmch_batch_create(vsmch_batch **b, int nfilters,
+    const vsmch_batch_filter *filters_host, const vsmch_cv_model *m,
+    int scheme, double threshold)
+{
+    if (!b || !filters_host || !m || nfilters < 1) {
+        set_error("vsmch_batch_create: invalid argument");
+        return VSMCH_ERR_INVALID;
+    }
+    std::vector<long long> n(nfilters), T(nfilters);
+    std::vector<uint32_t> seed(nfilters);
+    for (int f = 0; f < nfilters; ++f) {
+        n[f] = filters_host[f].n;
+        T[f] = filters_host[f].T;
+        seed[f] = filters_host[f].seed;
+    }
+    BatchSet *s = nullptr;
+    const int rc = batch_create(&s, nfilters, n.data(), T.data(), seed.data(),
+        cv_dev(m), scheme, threshold);
+    if (rc != VSMCH_OK)
+        return rc;
+    *b = new vsmch_batch{s};
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_batch_destroy(vsmch_batch *b)
+{
+    if (!b)
+        return VSMCH_ERR_INVALID;
+    batch_destroy(b->b);
+    delete b;
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_batch_set_obs(vsmch_batch *b, const double *obs_host)
+{
+    if (!b || !obs_host)
+        return VSMCH_ERR_INVALID;
+    return batch_set_obs(b->b, obs_host);
+}
+
+vsmch_status vsmch_batch_run(vsmch_batch *b, const vsmch_mlp *net,
+    int use_graph)
+{
+    if (!b)
+        return VSMCH_ERR_INVALID;
+    return batch_run(b->b, net ? net->m : nullptr, use_graph != 0);
+}
+
+vsmch_status vsmch_batch_histories(vsmch_batch *b, double *logz_host,
+    double *ess_host, int *resampled_host)
+{
+    if (!b)
+        return VSMCH_ERR_INVALID;
+    return batch_histories(b->b, logz_host, ess_host, resampled_host);
+}
+
+vsmch_status vsmch_batch_read_filter(vsmch_batch *b, int f, float *state,
+    double *weight)
+{
+    if (!b)
+        return VSMCH_ERR_INVALID;
+    return batch_read_filter(b->b, f, state, weight);
+}
+
+vsmch_status vsmch_batch_info(vsmch_batch *b, int64_t *tmax,
+    int64_t *particles, int *move_blocks, double *run_ms)
+{
+    if (!b)
+        return VSMCH_ERR_INVALID;
+    long long tm = 0, np = 0;
+    batch_info(b->b, &tm, &np, move_blocks, run_ms);
+    if (tmax)
+        *tmax = tm;
+    if (particles)
+        *particles = np;
+    return VSMCH_OK;
 }
 
 vsmch_status vsmch_cv_run_learned(vsmch_particle *p, const vsmch_cv_model *m,

@@ -31,6 +31,20 @@ struct ResampleDev {
     int equal;            // weights are 1/N (log-weights ignored)
 };
 
+// One filter of a batch: its particles occupy [off, off + n) of the batch's
+// arrays (off a multiple of the 4096-particle tile, so no tile holds two
+// filters), tiles [tile0, tile0 + ntiles); horizon T steps; particle i has
+// the stream key seed_key(seed, i), the resampling engine seed_key(seed, n).
+struct BatchFilter {
+    long long n;
+    long long off;
+    long long tile0;
+    long long ntiles;
+    long long T;
+    unsigned int seed;
+    int pad;
+};
+
 // Grow-only device workspace shared by the resampling passes.
 struct Workspace {
     hipStream_t stream = nullptr;
@@ -157,5 +171,15 @@ int sorted_uniforms(Workspace &ws, long long ndraw, unsigned int key,
 
 // Exact sequential prefix sums (for tests): out[i] = fl(...fl(w0 + w1)...)
 int exact_cumsum(Workspace &ws, long long n, const double *w, double *out);
+
+// One resampling event for every filter of a batch whose prm[f].active is
+// set (one workgroup per filter); weights from the log-weights l with
+// prm[f].lmax / inv_sum / equal; FP32 SoA state with component stride ld;
+// scratch rep / zdst / cE / csrc of the batch's particle count, tagg of 3
+// per tile, toff of 3 per (tile + filter).
+int batch_resample(int scheme, int nfilters, const BatchFilter *bf,
+    const ResampleDev *prm, const double *l, float *state, long long ld,
+    int *rep, long long *tagg, long long *toff, int *zdst, int *cE, int *csrc,
+    hipStream_t st);
 
 } // namespace vsmch

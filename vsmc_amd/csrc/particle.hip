@@ -52,12 +52,17 @@ __device__ inline Part3 part_of(double l)
     return Part3{l, (l > -INFINITY) ? 1.0 : 0.0, (l > -INFINITY) ? 1.0 : 0.0};
 }
 
-__device__ inline void block_partials(Part3 p, Part3 *out)
+__device__ inline void block_partials_at(Part3 p, Part3 *dst)
 {
     __shared__ Part3 lds[kWaves];
     p = block_reduce(p, OpPart(), lds);
     if (threadIdx.x == 0)
-        out[blockIdx.x] = p;
+        *dst = p;
+}
+
+__device__ inline void block_partials(Part3 p, Part3 *out)
+{
+    block_partials_at(p, out + blockIdx.x);
 }
 
 // A log-weight that came out NaN (a non-finite state: a float uniform of
@@ -96,8 +101,18 @@ __global__ void __launch_bounds__(kPB) k_weight_pass(double *l,
 // Merges block partials; mode 0: set_log, 1: add_log, 2: set_equal.
 // Also prepares the resampling parameters of this step (weights from l)
 // and records the history entry `t` when hist is non-null.
+__device__ inline void weight_merge(const Part3 *part, long long nblocks,
+    long long n, int mode, PDev *pd);
+
 __global__ void __launch_bounds__(kPB) k_weight_finalize(const Part3 *part,
     long long nblocks, long long n, int mode, PDev *pd)
+{
+    weight_merge(part, nblocks, n, mode, pd);
+}
+
+// The body of k_weight_finalize (one block).
+__device__ inline void weight_merge(const Part3 *part, long long nblocks,
+    long long n, int mode, PDev *pd)
 {
     __shared__ double dlds[kWaves];
     Part3 acc{-INFINITY, 0.0, 0.0};
@@ -187,6 +202,33 @@ __global__ void k_decide(PDev *pd, ResampleDev *prm, long long n,
 // the approximate exclusive prefix of the normalised weights over tiles of
 // kPTile particles, from the weight pass's block partials
 // (sum_b exp(m_b - lmax) / sum, one term per block).
+// The decision and draw parameters of a systematic / stratified event
+// (one thread): returns whether the step resamples.
+__device__ inline int decide_event(PDev *pd, ResampleDev *prm, long long n,
+    double threshold, int scheme)
+{
+    const double ess = pd->ess;
+    const int active = ess < threshold * static_cast<double>(n) ? 1 : 0;
+    pd->resampled = active;
+    prm->active = active;
+    prm->M = n;
+    prm->nsample = n;
+    prm->ndraw = n;
+    prm->key = pd->rkey;
+    prm->word0 = pd->rword;
+    prm->err = 0;
+    prm->integ_sum = 0;
+    prm->words_used = 0;
+    prm->equal = pd->equal;
+    prm->lmax = pd->equal ? pd->log_equal : pd->lmax;
+    prm->inv_sum = pd->equal ? 1.0 : 1 / pd->sum;
+    const double delta = 1 / static_cast<double>(n);
+    prm->delta = delta;
+    prm->u0 = (scheme == kSystematic) ?
+        u01_u32d(philox_word(pd->rkey, pd->rword)) * delta : 0.0;
+    return active;
+}
+
 __global__ void __launch_bounds__(kPB) k_prepare(PDev *pd, ResampleDev *prm,
     const Part3 *part, long long nb, long long n, double threshold,
     long long t, double *ess_hist, int *rs_hist, int scheme, double *tpre)
@@ -194,29 +236,11 @@ __global__ void __launch_bounds__(kPB) k_prepare(PDev *pd, ResampleDev *prm,
     __shared__ int act;
     __shared__ double lds[kWaves];
     if (threadIdx.x == 0) {
-        const double ess = pd->ess;
-        const int active = ess < threshold * static_cast<double>(n) ? 1 : 0;
         if (ess_hist)
-            ess_hist[t] = ess;
+            ess_hist[t] = pd->ess;
+        const int active = decide_event(pd, prm, n, threshold, scheme);
         if (rs_hist)
             rs_hist[t] = active;
-        pd->resampled = active;
-        prm->active = active;
-        prm->M = n;
-        prm->nsample = n;
-        prm->ndraw = n;
-        prm->key = pd->rkey;
-        prm->word0 = pd->rword;
-        prm->err = 0;
-        prm->integ_sum = 0;
-        prm->words_used = 0;
-        prm->equal = pd->equal;
-        prm->lmax = pd->equal ? pd->log_equal : pd->lmax;
-        prm->inv_sum = pd->equal ? 1.0 : 1 / pd->sum;
-        const double delta = 1 / static_cast<double>(n);
-        prm->delta = delta;
-        prm->u0 = (scheme == kSystematic) ?
-            u01_u32d(philox_word(pd->rkey, pd->rword)) * delta : 0.0;
         act = active;
     }
     __syncthreads();
@@ -251,7 +275,16 @@ __global__ void __launch_bounds__(kPB) k_prepare(PDev *pd, ResampleDev *prm,
 // `words` the event's stream use is computed here (systematic: the one
 // constructor word; stratified: up to and including the first uniform above
 // s_{M-2}, transform.hpp:61-69).
+__device__ inline void after_event(PDev *pd, ResampleDev *prm, long long n,
+    const int *rep, int scheme, int words);
+
 __global__ void k_after_resample(PDev *pd, ResampleDev *prm, long long n,
+    const int *rep, int scheme, int words)
+{
+    after_event(pd, prm, n, rep, scheme, words);
+}
+
+__device__ inline void after_event(PDev *pd, ResampleDev *prm, long long n,
     const int *rep, int scheme, int words)
 {
     if (!prm->active)
@@ -459,18 +492,19 @@ __device__ inline Part3 part_of_array(const double (&v)[K])
     return p;
 }
 
+// Tile `tile` (kPTile particles) of a filter of n particles whose state
+// components are X, Y, VX, VY; writes the tile's weight partial to
+// *part_out and, with mon_out, the monitor sums of the pre-move state.
 template <typename Real, bool kInit, int kB>
-__global__ void __launch_bounds__(kPB) k_cv_step(Real *__restrict__ state,
-    double *__restrict__ l, long long n, uint32_t seed, long long t, Real ox,
-    Real oy, CvModelDev m, const PDev *pd, Part3 *part, double *mon)
+__device__ inline void cv_tile(Real *__restrict__ X, Real *__restrict__ Y,
+    Real *__restrict__ VX, Real *__restrict__ VY, double *__restrict__ l,
+    long long n, long long tile, uint32_t seed, long long t, Real ox, Real oy,
+    const CvModelDev &m, const PDev *pd, Part3 *part_out, double *mon_out)
 {
-    const long long base = static_cast<long long>(blockIdx.x) * kPTile;
+    const bool mon = mon_out != nullptr;
+    const long long base = tile * kPTile;
     Part3 acc{-INFINITY, 0.0, 0.0};
     double mx = 0.0, my = 0.0;
-    Real *X = state;
-    Real *Y = state + n;
-    Real *VX = state + 2 * n;
-    Real *VY = state + 3 * n;
     const double lw_equal = pd->log_equal;
     const int equal = pd->equal;
     const double c = pd->c;
@@ -580,16 +614,26 @@ __global__ void __launch_bounds__(kPB) k_cv_step(Real *__restrict__ state,
             acc = part_merge(acc, part_of(lnew));
         }
     }
-    block_partials(acc, part);
+    block_partials_at(acc, part_out);
     if (mon) {
         __shared__ double lds[kWaves];
         mx = block_reduce(mx, OpAddD(), lds);
         my = block_reduce(my, OpAddD(), lds);
         if (threadIdx.x == 0) {
-            mon[blockIdx.x * 8 + 0] = mx;
-            mon[blockIdx.x * 8 + 1] = my;
+            mon_out[0] = mx;
+            mon_out[1] = my;
         }
     }
+}
+
+template <typename Real, bool kInit, int kB>
+__global__ void __launch_bounds__(kPB) k_cv_step(Real *__restrict__ state,
+    double *__restrict__ l, long long n, uint32_t seed, long long t, Real ox,
+    Real oy, CvModelDev m, const PDev *pd, Part3 *part, double *mon)
+{
+    cv_tile<Real, kInit, kB>(state, state + n, state + 2 * n, state + 3 * n, l,
+        n, blockIdx.x, seed, t, ox, oy, m, pd, part + blockIdx.x,
+        mon ? mon + 8 * static_cast<long long>(blockIdx.x) : nullptr);
 }
 
 // ------------------------------------------------- learned proposal
@@ -604,21 +648,18 @@ __global__ void __launch_bounds__(kPB) k_cv_step(Real *__restrict__ state,
 // A block is one 4096-particle tile (its weight partial feeds the same
 // merge and resampling tile prefix as the bootstrap move); each wave runs
 // the network on 4 column tiles of 16 particles per 64-particle chunk.
-__global__ void __launch_bounds__(kPB) k_cv_learned(float *__restrict__ state,
-    double *__restrict__ l, long long n, uint32_t seed, long long t, float ox,
-    float oy, CvModelDev m, const mlp::Frags *__restrict__ F, const PDev *pd,
-    Part3 *part)
+__device__ inline void cv_learned_tile(float *__restrict__ X,
+    float *__restrict__ Y, float *__restrict__ VX, float *__restrict__ VY,
+    double *__restrict__ l, long long n, long long tile, uint32_t seed,
+    long long t, float ox, float oy, const CvModelDev &m,
+    const mlp::Frags *__restrict__ F, const PDev *pd, Part3 *part_out)
 {
     __shared__ float fl[kWaves][64][8];
     __shared__ float ol[kWaves][64][8];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int fo = 4 * (lane >> 4);
-    const long long base = static_cast<long long>(blockIdx.x) * kPTile;
-    float *X = state;
-    float *Y = state + n;
-    float *VX = state + 2 * n;
-    float *VY = state + 3 * n;
+    const long long base = tile * kPTile;
     const double lw_equal = pd->log_equal;
     const int equal = pd->equal;
     const double c = pd->c;
@@ -694,7 +735,16 @@ __global__ void __launch_bounds__(kPB) k_cv_learned(float *__restrict__ state,
         }
         __syncthreads();
     }
-    block_partials(acc, part);
+    block_partials_at(acc, part_out);
+}
+
+__global__ void __launch_bounds__(kPB) k_cv_learned(float *__restrict__ state,
+    double *__restrict__ l, long long n, uint32_t seed, long long t, float ox,
+    float oy, CvModelDev m, const mlp::Frags *__restrict__ F, const PDev *pd,
+    Part3 *part)
+{
+    cv_learned_tile(state, state + n, state + 2 * n, state + 3 * n, l, n,
+        blockIdx.x, seed, t, ox, oy, m, F, pd, part + blockIdx.x);
 }
 
 // Draws from the word stream (key, first word) in order: Box-Muller pairs
@@ -768,6 +818,119 @@ __global__ void __launch_bounds__(kPB) k_cv_transitions(const float *state,
     f[7] = 0.0f;
     for (int d = 0; d < 4; ++d)
         target[i * 4 + d] = z[d];
+}
+
+// ------------------------------------------------- batched filters
+//
+// Many independent filters of the same model in one set of arrays (see
+// BatchFilter). A step is four launches on one stream: the persistent
+// move over every active filter's tiles, one merge + decision block per
+// filter, one resampling workgroup per resampling filter, and one thread
+// per filter for the post-event update (which also advances the device
+// step counter, so that one captured step replays for every step). Each
+// filter runs the same per-tile code as a single filter, so its results
+// are those of a single-filter run with the same seed and observations.
+
+__global__ void __launch_bounds__(kPB) k_batch_move(float *state, double *l,
+    long long ld, const BatchFilter *bf, const int *tile_filter,
+    long long ntiles, const long long *tstep, const double *obs,
+    long long tmax, CvModelDev m, const mlp::Frags *net, const PDev *pd,
+    Part3 *part)
+{
+    const long long t = *tstep;
+    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int f = tile_filter[tile];
+        const BatchFilter F = bf[f];
+        if (t >= F.T)
+            continue;
+        const long long o = 2 * (static_cast<long long>(f) * tmax + t);
+        const float ox = static_cast<float>(obs[o]);
+        const float oy = static_cast<float>(obs[o + 1]);
+        float *X = state + F.off;
+        double *lf = l + F.off;
+        const long long lt = tile - F.tile0;
+        if (t == 0)
+            cv_tile<float, true, 1>(X, X + ld, X + 2 * ld, X + 3 * ld, lf, F.n,
+                lt, F.seed, t, ox, oy, m, pd + f, part + tile, nullptr);
+        else if (net)
+            cv_learned_tile(X, X + ld, X + 2 * ld, X + 3 * ld, lf, F.n, lt,
+                F.seed, t, ox, oy, m, net, pd + f, part + tile);
+        else
+            cv_tile<float, false, 1>(X, X + ld, X + 2 * ld, X + 3 * ld, lf, F.n,
+                lt, F.seed, t, ox, oy, m, pd + f, part + tile, nullptr);
+    }
+}
+
+__global__ void __launch_bounds__(kPB) k_batch_finalize(const BatchFilter *bf,
+    const long long *tstep, const Part3 *part, PDev *pd, ResampleDev *prm,
+    double threshold, int scheme, long long tmax, double *logz_hist,
+    double *ess_hist, int *rs_hist)
+{
+    const int f = blockIdx.x;
+    const long long t = *tstep;
+    const BatchFilter F = bf[f];
+    if (t >= F.T) {
+        if (threadIdx.x == 0)
+            prm[f].active = 0;
+        return;
+    }
+    weight_merge(part + F.tile0, F.ntiles, F.n, t == 0 ? 0 : 1, pd + f);
+    if (threadIdx.x == 0) {
+        PDev *p = pd + f;
+        const long long h = static_cast<long long>(f) * tmax + t;
+        logz_hist[h] = p->logz;
+        ess_hist[h] = p->ess;
+        rs_hist[h] = decide_event(p, prm + f, F.n, threshold, scheme);
+    }
+}
+
+__global__ void k_batch_after(const BatchFilter *bf, PDev *pd,
+    ResampleDev *prm, const int *rep, int scheme, int nfilters,
+    long long *tstep)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < nfilters) {
+        const BatchFilter F = bf[f];
+        after_event(pd + f, prm + f, F.n, rep + F.off, scheme, 1);
+    }
+    // every other kernel of this step precedes this one on the stream
+    if (f == 0)
+        *tstep += 1;
+}
+
+int batch_step(const BatchDev &b, int move_blocks, const MlpSet *net,
+    hipStream_t st)
+{
+    hipLaunchKernelGGL(k_batch_move, dim3(move_blocks), dim3(kPB), 0, st,
+        b.state, b.l, b.ld, b.bf, b.tile_filter, b.ntiles,
+        static_cast<const long long *>(b.tstep), b.obs, b.tmax, b.m,
+        net ? static_cast<const mlp::Frags *>(net->frags) : nullptr,
+        static_cast<const PDev *>(b.pd), static_cast<Part3 *>(b.part));
+    hipLaunchKernelGGL(k_batch_finalize, dim3(b.nfilters), dim3(kPB), 0, st,
+        b.bf, static_cast<const long long *>(b.tstep),
+        static_cast<const Part3 *>(b.part), b.pd, b.prm, b.threshold,
+        b.scheme, b.tmax, b.logz_hist, b.ess_hist, b.rs_hist);
+    VSMCH_HIP(hipGetLastError());
+    int rc = batch_resample(b.scheme, b.nfilters, b.bf, b.prm, b.l, b.state,
+        b.ld, b.rep, b.tagg, b.toff, b.zdst, b.cE, b.csrc, st);
+    if (rc != kOk)
+        return rc;
+    hipLaunchKernelGGL(k_batch_after, dim3((b.nfilters + 255) / 256), dim3(256),
+        0, st, b.bf, b.pd, b.prm, static_cast<const int *>(b.rep), b.scheme,
+        b.nfilters, b.tstep);
+    VSMCH_HIP(hipGetLastError());
+    return kOk;
+}
+
+// Normalised weights of one filter of a batch.
+int batch_weights(const BatchDev &b, int f, const BatchFilter &F, double *W,
+    hipStream_t st)
+{
+    hipLaunchKernelGGL(k_weight_read, dim3((F.n + 255) / 256), dim3(256), 0,
+        st, static_cast<const double *>(b.l + F.off), F.n,
+        static_cast<const PDev *>(b.pd + f), W);
+    VSMCH_HIP(hipGetLastError());
+    return kOk;
 }
 
 // ------------------------------------------------------------ host side

@@ -77,6 +77,54 @@ int cv_transitions(ParticleSet *p, const CvModelDev &m, long long T,
     const double *ox, const double *oy, int scheme, double threshold,
     uint32_t train_seed, float *feat, float *target);
 
+// Device arrays of a batch of filters (batch.hip owns them).
+struct BatchDev {
+    int nfilters = 0;
+    long long ld = 0;         // padded particle count: component stride
+    long long ntiles = 0;
+    long long tmax = 0;
+    BatchFilter *bf = nullptr;
+    int *tile_filter = nullptr;
+    float *state = nullptr;   // 4 x ld FP32 (x, y, vx, vy)
+    double *l = nullptr;      // ld log-weights
+    void *part = nullptr;     // per-tile weight partials
+    PDev *pd = nullptr;       // per filter
+    ResampleDev *prm = nullptr;
+    int *rep = nullptr;
+    int *zdst = nullptr;
+    int *cE = nullptr;
+    int *csrc = nullptr;
+    long long *tagg = nullptr;   // 3 per tile
+    long long *toff = nullptr;   // 3 per (tile + filter)
+    double *obs = nullptr;       // [nfilters][tmax][2]
+    long long *tstep = nullptr;  // device step counter
+    double *logz_hist = nullptr; // [nfilters][tmax]
+    double *ess_hist = nullptr;
+    int *rs_hist = nullptr;
+    CvModelDev m{};
+    int scheme = 0;
+    double threshold = 0.5;
+};
+
+// Enqueues one step of every filter (four launches; see particle.hip).
+int batch_step(const BatchDev &b, int move_blocks, const MlpSet *net,
+    hipStream_t st);
+int batch_weights(const BatchDev &b, int f, const BatchFilter &F, double *W,
+    hipStream_t st);
+
+// batch.hip
+struct BatchSet;
+int batch_create(BatchSet **out, int nfilters, const long long *n,
+    const long long *T, const uint32_t *seed, const CvModelDev &m, int scheme,
+    double threshold);
+void batch_destroy(BatchSet *b);
+int batch_set_obs(BatchSet *b, const double *obs_host);
+int batch_run(BatchSet *b, const MlpSet *net, bool use_graph);
+int batch_histories(BatchSet *b, double *logz, double *ess, int *rs);
+int batch_read_filter(BatchSet *b, int f, float *state, double *W);
+void batch_info(const BatchSet *b, long long *tmax, long long *particles,
+    int *move_blocks, double *run_ms);
+
 // rng.hip
 int stream_fill(int kind, unsigned long long key, unsigned long long offset,
     long long n, void *out, int mode, hipStream_t st);

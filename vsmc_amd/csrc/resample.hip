@@ -524,6 +524,10 @@ struct FoldBufs {
     const double *xstart;   // exact start of every tile, total at [ntiles]
 };
 
+template <class WSrc, class USrc>
+__device__ inline double count_tile(WSrc src, USrc us, long long M,
+    long long tidx, double x0, int *rep, long long *tagg);
+
 // Replication counts (and residual integer parts) for one tile.
 template <class WSrc, class USrc>
 __global__ void __launch_bounds__(kBlock) k_count(WSrc src, USrc us,
@@ -533,20 +537,27 @@ __global__ void __launch_bounds__(kBlock) k_count(WSrc src, USrc us,
     if (!prm->active)
         return;
     const long long M = eff_n(ndev, nh);
-    const long long base = static_cast<long long>(blockIdx.x) * kTile;
-    if (base >= M)
+    if (static_cast<long long>(blockIdx.x) * kTile >= M)
         return;
+    count_tile(src, us, M, blockIdx.x, fb.xstart[blockIdx.x], rep, tagg);
+}
+
+// Replication counts (and residual integer parts) of tile `tidx` of M
+// weights from the exact start x0 of the tile; per-tile rep -> index
+// aggregates to tagg[3 tidx ..] when tagg is non-null. Returns the exact
+// sum after the tile's last element (the next tile's start).
+template <class WSrc, class USrc>
+__device__ inline double count_tile(WSrc src, USrc us, long long M,
+    long long tidx, double x0, int *rep, long long *tagg)
+{
+    const long long base = tidx * kTile;
     __shared__ double tile[kTileLds];
     __shared__ fold::ExactShared ex;
     __shared__ double lasts[kBlock];
-    __shared__ double x0s;
+    __shared__ double xend;
     __shared__ long long red[kWaves];
     double w[kItems], s[kItems];
     load_tile_blocked(src, base, M, w, tile, 0.0);
-    if (threadIdx.x == 0)
-        x0s = fb.xstart[blockIdx.x];
-    __syncthreads();
-    const double x0 = x0s;
     fold::tile_exact(w, x0, s, ex);
     lasts[threadIdx.x] = s[kItems - 1];
     __syncthreads();
@@ -557,10 +568,11 @@ __global__ void __launch_bounds__(kBlock) k_count(WSrc src, USrc us,
     long long gp = 0;
     if (ib < M) {
         if (threadIdx.x == 0)
-            gp = (blockIdx.x == 0) ? 0 : us.count(x0);
+            gp = (tidx == 0) ? 0 : us.count(x0);
         else
             gp = us.count(lasts[threadIdx.x - 1]);
     }
+    const long long last = (M - base < kTile ? M : base + kTile) - 1;
     long long z = 0, e = 0, c = 0;
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
@@ -573,6 +585,8 @@ __global__ void __launch_bounds__(kBlock) k_count(WSrc src, USrc us,
             z += r == 0;
             e += r > 1 ? r - 1 : 0;
             c += r > 1;
+            if (i == last)
+                xend = s[j];
         }
     }
     if (tagg) {
@@ -582,11 +596,15 @@ __global__ void __launch_bounds__(kBlock) k_count(WSrc src, USrc us,
         e = block_reduce(e, OpAddL(), red);
         c = block_reduce(c, OpAddL(), red);
         if (threadIdx.x == 0) {
-            tagg[3 * blockIdx.x + 0] = z;
-            tagg[3 * blockIdx.x + 1] = e;
-            tagg[3 * blockIdx.x + 2] = c;
+            tagg[3 * tidx + 0] = z;
+            tagg[3 * tidx + 1] = e;
+            tagg[3 * tidx + 2] = c;
         }
     }
+    __syncthreads();
+    const double xe = xend;
+    __syncthreads();
+    return xe;
 }
 
 // Writes f(s_i) for the exact prefix s (tests; sorted uniforms).
@@ -754,11 +772,22 @@ struct OpAddL3 {
     }
 };
 
+__device__ inline void scan_tiles_block(const long long *tagg, long long ntiles,
+    long long *toff);
+
 __global__ void __launch_bounds__(kBlock) k_ridx_scan(const long long *tagg,
     long long ntiles, const ResampleDev *prm, long long *toff)
 {
     if (prm && !prm->active)
         return;
+    scan_tiles_block(tagg, ntiles, toff);
+}
+
+// Exclusive offsets of the per-tile (childless, extras, sources) triples,
+// totals at toff[3 ntiles ..]. One block.
+__device__ inline void scan_tiles_block(const long long *tagg, long long ntiles,
+    long long *toff)
+{
     __shared__ L3 lds[kWaves];
     L3 carry{0, 0, 0};
     for (long long c0 = 0; c0 < ntiles; c0 += kBlock) {
@@ -783,13 +812,27 @@ __global__ void __launch_bounds__(kBlock) k_ridx_scan(const long long *tagg,
 }
 
 template <bool kExpand>
+__device__ inline void scatter_tile(const int *rep, long long M, long long tidx,
+    const long long *toff, int *zdst, int *cE, int *csrc, int *idx);
+
+template <bool kExpand>
 __global__ void __launch_bounds__(kBlock) k_ridx_scatter(const int *rep,
     long long M, const ResampleDev *prm, const long long *toff, int *zdst,
     int *cE, int *csrc, int *idx)
 {
     if (prm && !prm->active)
         return;
-    const long long base = static_cast<long long>(blockIdx.x) * kTile;
+    scatter_tile<kExpand>(rep, M, blockIdx.x, toff, zdst, cE, csrc, idx);
+}
+
+// The childless slots (zdst, in slot order) and the sources with extra
+// copies (csrc, with cE the exclusive prefix of their extra copies) of tile
+// tidx of rep, at the offsets toff[3 tidx ..] of the tile.
+template <bool kExpand>
+__device__ inline void scatter_tile(const int *rep, long long M, long long tidx,
+    const long long *toff, int *zdst, int *cE, int *csrc, int *idx)
+{
+    const long long base = tidx * kTile;
     __shared__ L3 lds[kWaves];
     __shared__ int tile[kTile];
 #pragma unroll
@@ -817,9 +860,9 @@ __global__ void __launch_bounds__(kBlock) k_ridx_scatter(const int *rep,
     }
     L3 tot;
     L3 ex = block_exclusive_scan(loc, OpAddL3(), L3{0, 0, 0}, lds, &tot);
-    long long zr = toff[3 * blockIdx.x + 0] + ex.a;
-    long long er = toff[3 * blockIdx.x + 1] + ex.b;
-    long long cr = toff[3 * blockIdx.x + 2] + ex.c;
+    long long zr = toff[3 * tidx + 0] + ex.a;
+    long long er = toff[3 * tidx + 1] + ex.b;
+    long long cr = toff[3 * tidx + 2] + ex.c;
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
         const long long i = ib + j;
@@ -868,6 +911,11 @@ __device__ inline long long upper_bound_i(const int *a, long long n, long long x
 // expand mode): the source whose copies cover k. Optionally gathers an SoA
 // state in place: x[d][dst] = x[d][src] (sources are never destinations).
 template <bool kExpand, typename E>
+__device__ inline void expand_chunk(long long chunk, long long K, long long C,
+    const int *zdst, const int *cE, const int *csrc, int *idx, E *state,
+    int dim, long long ld);
+
+template <bool kExpand, typename E>
 __global__ void __launch_bounds__(kBlock) k_expand(const ResampleDev *prm,
     const long long *toff, long long ntiles, const int *zdst, const int *cE,
     const int *csrc, int *idx, E *state, int dim, long long ld)
@@ -876,11 +924,20 @@ __global__ void __launch_bounds__(kBlock) k_expand(const ResampleDev *prm,
         return;
     const long long K = kExpand ? toff[3 * ntiles + 1] : toff[3 * ntiles + 0];
     const long long C = toff[3 * ntiles + 2];
-    const long long k0 = static_cast<long long>(blockIdx.x) * kTile;
-    if (k0 >= K)
+    if (static_cast<long long>(blockIdx.x) * kTile >= K || C <= 0)
         return;
-    if (C <= 0)
-        return;
+    expand_chunk<kExpand, E>(blockIdx.x, K, C, zdst, cE, csrc, idx, state, dim,
+        ld);
+}
+
+// Ranks [chunk kTile, (chunk + 1) kTile) of the K ranks: the source of each
+// rank is the one of the C sources whose copies cover it.
+template <bool kExpand, typename E>
+__device__ inline void expand_chunk(long long chunk, long long K, long long C,
+    const int *zdst, const int *cE, const int *csrc, int *idx, E *state,
+    int dim, long long ld)
+{
+    const long long k0 = chunk * kTile;
     const long long kl = (k0 + kTile < K ? k0 + kTile : K) - 1;
     __shared__ int win[kTile + 1];
     __shared__ long long clo_s, chi_s;
@@ -933,6 +990,76 @@ __global__ void k_fill(T *p, long long n, T v, const ResampleDev *prm)
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n)
         p[i] = v;
+}
+
+// ------------------------------------------------- batched filters
+
+// The whole resampling event of one filter of a batch in one workgroup:
+// the exact sequential fold tile by tile from 0 (each tile's exact start is
+// the previous tile's exact end, so no tile summaries or resolver are
+// needed), counts of the systematic or stratified uniforms, the rep ->
+// index aggregates, their scan, the scatter of childless slots and extra
+// copies, and the in-place gather of the FP32 SoA state (component stride
+// ld). Same counts, ancestors and state as the multi-kernel path. Filters
+// whose prm->active is 0 return at once.
+template <class USrc>
+__global__ void __launch_bounds__(kBlock) k_batch_resample(
+    const BatchFilter *bf, const ResampleDev *prm, const double *l,
+    float *state, long long ld, int *rep, long long *tagg, long long *toff,
+    int *zdst, int *cE, int *csrc)
+{
+    const int f = blockIdx.x;
+    const ResampleDev *p = prm + f;
+    if (!p->active)
+        return;
+    const BatchFilter F = bf[f];
+    const long long n = F.n;
+    int *repf = rep + F.off;
+    int *zf = zdst + F.off;
+    int *cf = cE + F.off;
+    int *sf = csrc + F.off;
+    long long *tg = tagg + 3 * F.tile0;
+    long long *to = toff + 3 * (F.tile0 + f);
+    const WLog src{l + F.off, p};
+    const USrc us{p};
+    double x = 0.0;
+    for (long long k = 0; k < F.ntiles; ++k)
+        x = count_tile(src, us, n, k, x, repf, tg);
+    __threadfence();
+    __syncthreads();
+    scan_tiles_block(tg, F.ntiles, to);
+    __threadfence();
+    __syncthreads();
+    for (long long k = 0; k < F.ntiles; ++k)
+        scatter_tile<false>(repf, n, k, to, zf, cf, sf, nullptr);
+    __threadfence();
+    __syncthreads();
+    const long long K = to[3 * F.ntiles + 0];
+    const long long C = to[3 * F.ntiles + 2];
+    if (C > 0)
+        for (long long chunk = 0; chunk * kTile < K; ++chunk)
+            expand_chunk<false, float>(chunk, K, C, zf, cf, sf, nullptr,
+                state + F.off, 4, ld);
+}
+
+int batch_resample(int scheme, int nfilters, const BatchFilter *bf,
+    const ResampleDev *prm, const double *l, float *state, long long ld,
+    int *rep, long long *tagg, long long *toff, int *zdst, int *cE, int *csrc,
+    hipStream_t st)
+{
+    if (scheme == kSystematic)
+        hipLaunchKernelGGL(k_batch_resample<USys>, dim3(nfilters), dim3(kBlock),
+            0, st, bf, prm, l, state, ld, rep, tagg, toff, zdst, cE, csrc);
+    else if (scheme == kStratified)
+        hipLaunchKernelGGL(k_batch_resample<UStrat>, dim3(nfilters),
+            dim3(kBlock), 0, st, bf, prm, l, state, ld, rep, tagg, toff, zdst,
+            cE, csrc);
+    else {
+        set_error("batched filters resample with the systematic or stratified scheme");
+        return kErrUnsupported;
+    }
+    VSMCH_HIP(hipGetLastError());
+    return kOk;
 }
 
 // ------------------------------------------------------- event setup

@@ -289,3 +289,75 @@ class ProposalNet:
                                     ctypes.c_void_p(target.data_ptr()), batch, steps, lr,
                                     loss.ctypes.data_as(ctypes.c_void_p)))
         return loss[:steps]
+
+
+class CvBatch:
+    """Many independent constant-velocity filters (FP32 state) advanced
+    together on the device: filter f has ns[f] particles, horizon Ts[f] and
+    seed seeds[f], and gives the results of CvFilter(ns[f], "float32",
+    seed=seeds[f]).run(...) on its own observations."""
+
+    def __init__(self, ns, Ts, seeds, scheme, threshold=0.5, **model):
+        from . import BatchFilter
+        self.ns = [int(n) for n in ns]
+        self.Ts = [int(t) for t in Ts]
+        self.nf = len(self.ns)
+        arr = (BatchFilter * self.nf)(*[BatchFilter(n, t, s) for n, t, s
+                                        in zip(self.ns, self.Ts, seeds)])
+        self.model = cv_model(**model)
+        h = ctypes.c_void_p()
+        check(lib().vsmch_batch_create(ctypes.byref(h), self.nf, ctypes.cast(arr, ctypes.c_void_p),
+                                       ctypes.byref(self.model), scheme, threshold))
+        self._h = h
+        self.tmax = max(self.Ts)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            check(lib().vsmch_batch_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_obs(self, tracks):
+        """tracks[f] = (obs_x, obs_y), each of length >= Ts[f]."""
+        import numpy as np
+        obs = np.zeros((self.nf, self.tmax, 2))
+        for f, (ox, oy) in enumerate(tracks):
+            T = self.Ts[f]
+            obs[f, :T, 0] = np.asarray(ox, dtype=np.float64)[:T]
+            obs[f, :T, 1] = np.asarray(oy, dtype=np.float64)[:T]
+        check(lib().vsmch_batch_set_obs(self._h, obs.ctypes.data_as(ctypes.c_void_p)))
+
+    def run(self, net=None, graph=True):
+        check(lib().vsmch_batch_run(self._h, net._h if net is not None else None, int(graph)))
+
+    def histories(self):
+        import numpy as np
+        logz = np.zeros((self.nf, self.tmax))
+        ess = np.zeros((self.nf, self.tmax))
+        rs = np.zeros((self.nf, self.tmax), dtype=np.int32)
+        P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        check(lib().vsmch_batch_histories(self._h, P(logz), P(ess), P(rs)))
+        return {"logz": logz, "ess": ess, "resampled": rs}
+
+    def filter_state(self, f):
+        import torch
+        n = self.ns[f]
+        st = torch.empty((4, n), dtype=torch.float32, device="cuda")
+        W = torch.empty(n, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        check(lib().vsmch_batch_read_filter(self._h, f, ctypes.c_void_p(st.data_ptr()),
+                                            ctypes.c_void_p(W.data_ptr())))
+        return st, W
+
+    def info(self):
+        tm, npart = ctypes.c_int64(), ctypes.c_int64()
+        mb, ms = ctypes.c_int(), ctypes.c_double()
+        check(lib().vsmch_batch_info(self._h, ctypes.byref(tm), ctypes.byref(npart),
+                                     ctypes.byref(mb), ctypes.byref(ms)))
+        return {"tmax": tm.value, "particles": npart.value, "move_blocks": mb.value,
+                "run_ms": ms.value}
