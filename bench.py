@@ -191,63 +191,62 @@ def _train_net(va, torch, model, track, n=1 << 14, steps=600):
                  "loss_first": float(loss[0]), "loss_last20": float(loss[-20:].mean())}
 
 
-def learned_vs_plain(va, torch, tracks=16, T=50, sizes=(1 << 12, 1 << 14, 1 << 16, 1 << 18,
-                                                           1 << 20)):
+def learned_vs_plain(va, torch, tracks=64, T=50, sizes=(1 << 10, 1 << 12, 1 << 14, 1 << 16,
+                                                           1 << 18)):
     """log-evidence RMSE against the FP64 Kalman filter vs GPU-seconds per
     filter run, learned proposal vs bootstrap, on the linear-Gaussian CV
     model (obs sd 0.01), T = 50, systematic resampling at ESS < N/2; plus
     the Student-t model with 5% outliers at 10x scale against a 2^22-particle
-    bootstrap reference."""
+    bootstrap reference. Each (method, N) runs the tracks as one batch of
+    filters (one graph-replayed step for all of them): GPU-seconds per filter
+    = the batch's GPU time / tracks."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
     import oracle_lib as ol
-    from vsmc_amd.models import cv_simulate
+    from vsmc_amd.models import cv_simulate, cv_simulate_many
 
-    def track(seed, kind):
+    def sim(kind, F, seed):
         if kind == "lg":
-            _, ox, oy = cv_simulate(T, seed, sd_pos0=0.02, sd_vel0=0.02, obs="gauss",
+            return cv_simulate_many(F, T, seed, sd_pos0=0.02, sd_vel0=0.02, obs="gauss",
                                     obs_scale=0.01)
-        else:
-            _, ox, oy = cv_simulate(T, seed, sd_pos0=0.02, sd_vel0=0.02, obs="student",
-                                    obs_scale=0.1, obs_df=10.0, outlier_frac=0.05,
-                                    outlier_mult=10.0)
-        return ox, oy
+        return cv_simulate_many(F, T, seed, sd_pos0=0.02, sd_vel0=0.02, obs="student",
+                                obs_scale=0.1, obs_df=10.0, outlier_frac=0.05,
+                                outlier_mult=10.0)
 
-    def run(model, n, seed, net, tr):
-        f = va.CvFilter(n, "float32", seed=seed, **model)
-        ox, oy = tr
-        if net is None:
-            r, s = _timed(torch, lambda: f.run(ox, oy, va.SYSTEMATIC, 0.5))
-        else:
-            r, s = _timed(torch, lambda: f.run_learned(net, ox, oy, va.SYSTEMATIC, 0.5))
-        f.close()
-        return float(r["logz"][-1]), s
+    def run_batch(model, n, net, ox, oy):
+        b = va.CvBatch([n] * tracks, [T] * tracks, list(range(1, tracks + 1)), va.SYSTEMATIC,
+                       0.5, **model)
+        b.set_obs(list(zip(ox, oy)))
+        b.run(net=net, graph=True)
+        z = b.histories()["logz"][:, T - 1].copy()
+        ms = b.info()["run_ms"]
+        b.close()
+        return z, ms / 1e3 / tracks
 
     out = {}
     for kind, model in (("linear_gaussian", LG_MODEL), ("student_t_outliers", ST_MODEL)):
         k = "lg" if kind == "linear_gaussian" else "st"
-        net, train = _train_net(va, torch, model, track(11, k))
-        trs = [track(1000 + s, k) for s in range(tracks)]
+        tx, ty = sim(k, 1, 11)
+        net, train = _train_net(va, torch, model, (tx[0], ty[0]))
+        ox, oy = sim(k, tracks, 1000)
         if k == "lg":
-            ref = [ol.kalman_cv(ox, oy, 0.1, 0.02, 0.02, 0.02, 0.02, 0.01)[0] for ox, oy in trs]
+            ref = np.array([ol.kalman_cv(ox[s], oy[s], 0.1, 0.02, 0.02, 0.02, 0.02, 0.01)[0]
+                            for s in range(tracks)])
             ref_desc = "FP64 Kalman filter"
-            szs = sizes
         else:
-            ref = [run(model, 1 << 22, 9000 + s, None, trs[s])[0] for s in range(tracks)]
+            ref, _ = run_batch(model, 1 << 22, None, ox, oy)
             ref_desc = "bootstrap filter at N = 2^22"
-            szs = sizes[:4]
         rows = []
-        for n in szs:
+        for n in sizes + (sizes[-1] * 4,):
             for name, nn in (("bootstrap", None), ("learned", net)):
-                errs, secs = [], []
-                for s in range(tracks):
-                    z, sec = run(model, n, s, nn, trs[s])
-                    errs.append(z - ref[s])
-                    secs.append(sec)
+                if nn is not None and n > sizes[-1]:
+                    continue   # the bootstrap curve extends past the learned sizes
+                z, sec = run_batch(model, n, nn, ox, oy)
+                errs = z - ref
                 rows.append({"method": name, "n": n,
                              "rmse": float(np.sqrt(np.mean(np.square(errs)))),
                              "bias": float(np.mean(errs)),
-                             "gpu_s_per_run": float(np.mean(secs))})
+                             "gpu_s_per_run": float(sec)})
         # GPU-seconds the bootstrap filter needs for each learned RMSE,
         # interpolated log-log along its (gpu_s, rmse) curve
         bs = sorted([r for r in rows if r["method"] == "bootstrap"], key=lambda r: r["n"])
@@ -270,6 +269,48 @@ def learned_vs_plain(va, torch, tracks=16, T=50, sizes=(1 << 12, 1 << 14, 1 << 1
                      "rows": rows, "equal_rmse": ratios}
         net.close()
     return out
+
+
+def batched_filters(va, torch, F=1024, seed=2024, plain_sample=24):
+    """1,024 independent filters of the CPU configuration's model (Student-t
+    observations, stratified resampling at ESS < N/2), N_f = floor(2^U(10,18)),
+    T_f ~ U{50..500}, per-filter seeds: one run of the batch (persistent move
+    + graph-replayed steps, autotuned move grid) against the plain form (one
+    launch sequence per filter, CvFilter.run) on a sample of the filters."""
+    import numpy as np
+    from vsmc_amd.models import cv_simulate_many
+    rs = np.random.default_rng(seed)
+    ns = np.floor(2.0 ** rs.uniform(10, 18, F)).astype(np.int64)
+    Ts = rs.integers(50, 501, F)
+    seeds = 100 + np.arange(F)
+    ox, oy = cv_simulate_many(F, int(Ts.max()), seed)
+    b = va.CvBatch(ns, Ts, seeds, va.STRATIFIED, 0.5)
+    b.set_obs(list(zip(ox, oy)))
+    b.run(graph=True)   # first run: allocation, code load
+    b.run(graph=True)
+    info = b.info()
+    h = b.histories()
+    b.close()
+    updates = float((ns * Ts).sum())
+    events = float(sum(int(h["resampled"][f, :Ts[f]].sum()) * ns[f] for f in range(F)))
+    secs = info["run_ms"] / 1e3
+    idx = rs.choice(F, plain_sample, replace=False)
+    p_updates, p_secs = 0.0, 0.0
+    for f in idx:
+        g = va.CvFilter(int(ns[f]), "float32", seed=int(seeds[f]))
+        _, s = _timed(torch, lambda: g.run(ox[f, :Ts[f]], oy[f, :Ts[f]], va.STRATIFIED, 0.5))
+        g.close()
+        p_updates += float(ns[f] * Ts[f])
+        p_secs += s
+    return {"filters": F, "particles": int(ns.sum()), "particle_steps": updates,
+            "t_max": int(Ts.max()), "batched_s": secs,
+            "batched_updates_per_s": updates / secs,
+            "batched_hbm_frac": (updates * MOVE_BYTES + events * RESAMPLE_BYTES) / secs / 1e9
+            / HBM_PEAK_GBS,
+            "move_blocks_autotuned": info["move_blocks"],
+            "plain_sample_filters": plain_sample,
+            "plain_updates_per_s": p_updates / p_secs,
+            "speedup_vs_plain": (updates / secs) / (p_updates / p_secs)}
 
 
 def measured_traffic():
@@ -371,6 +412,7 @@ def main():
         out["extra"] = extra
         out["microbench"] = microbench(va, torch)
         out["learned_proposal"] = learned_vs_plain(va, torch)
+        out["batched_filters"] = batched_filters(va, torch)
     out["cpu_baseline"] = None if a.no_cpu else cpu_baseline(a.cpu_seconds)
     print(json.dumps(out), flush=True)
 

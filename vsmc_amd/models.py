@@ -35,3 +35,30 @@ def cv_simulate(T, seed, dt=0.1, sd_pos=0.02, sd_vel=0.02, sd_pos0=2.0, sd_vel0=
         ox[t] = x[0] + e[0]
         oy[t] = x[1] + e[1]
     return states, ox, oy
+
+
+def cv_simulate_many(F, T, seed, dt=0.1, sd_pos=0.02, sd_vel=0.02, sd_pos0=2.0, sd_vel0=1.0,
+                     obs="student", obs_scale=0.1, obs_df=10.0, outlier_frac=0.0,
+                     outlier_mult=10.0):
+    """F independent tracks of T steps at once (vectorised over tracks):
+    returns (obs_x[F, T], obs_y[F, T])."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x = np.stack([rng.normal(0, sd_pos0, F), rng.normal(0, sd_pos0, F),
+                  rng.normal(0, sd_vel0, F), rng.normal(0, sd_vel0, F)])
+    ox = np.zeros((F, T))
+    oy = np.zeros((F, T))
+    for t in range(T):
+        if t > 0:
+            x = np.stack([x[0] + rng.normal(0, sd_pos, F) + dt * x[2],
+                          x[1] + rng.normal(0, sd_pos, F) + dt * x[3],
+                          x[2] + rng.normal(0, sd_vel, F),
+                          x[3] + rng.normal(0, sd_vel, F)])
+        if obs == "student":
+            e = rng.standard_t(obs_df, size=(2, F)) * obs_scale
+        else:
+            e = rng.normal(0, obs_scale, size=(2, F))
+        if outlier_frac > 0:
+            e = e * np.where(rng.random(F) < outlier_frac, outlier_mult, 1.0)
+        ox[:, t] = x[0] + e[0]
+        oy[:, t] = x[1] + e[1]
+    return ox, oy
