@@ -329,6 +329,51 @@ vsmch_status vsmch_batch_read_filter(vsmch_batch *b, int f, float *state,
 vsmch_status vsmch_batch_info(vsmch_batch *b, int64_t *tmax,
     int64_t *particles, int *move_blocks, double *run_ms);
 
+/* ------------------------------------ tempered SMC, Gaussian mixture --
+ * The sampler of example/gmm/include/gmm.hpp (gmm_state, gmm_init,
+ * gmm_move_smc, gmm_move_mu / _lambda / _weight, gmm_path_integrand) on the
+ * device, with adaptive tempering: each step takes the largest increment of
+ * alpha whose conditional ESS reaches `cess` (the reference anneals on a
+ * fixed schedule, gmm.hpp:567-604), reweights by add_log(increment x log
+ * likelihood), resamples when ESS < threshold N, then runs `moves`
+ * random-walk Metropolis-Hastings moves cycling mu, lambda, weight.
+ * Particles: 3c + 2 FP32 components (mu[c], lambda[c], weight[c], log
+ * prior, log likelihood) of an internal particle set. */
+typedef struct vsmch_gmm vsmch_gmm;
+
+typedef struct {
+    int64_t n;            /* particles */
+    int ncomp;            /* mixture components c, 2..8 */
+    uint32_t seed;        /* Seed::instance().set(seed) */
+    double cess;          /* conditional ESS target in (0, 1), e.g. 0.9 */
+    int moves;            /* MH moves per step */
+    int search_rounds;    /* rounds of the 16-candidate increment search */
+    int scheme;           /* stratified or systematic */
+    double threshold;     /* resample when ESS < threshold N */
+} vsmch_gmm_config;
+
+/* data_host: ndata observations (2..8192). */
+vsmch_status vsmch_gmm_create(vsmch_gmm **g, const vsmch_gmm_config *cfg,
+    const double *data_host, int64_t ndata);
+vsmch_status vsmch_gmm_destroy(vsmch_gmm *g);
+/* Initialisation at alpha = 0, then steps until alpha = 1, at most
+ * max_steps history entries (the initial one included). Host histories of
+ * max_steps entries (any may be NULL): alpha, ESS before resampling,
+ * resampled flags, cumulative log-evidence, the path-sampling integrand
+ * sum_i W_i loglik_i after the moves, and acceptance rates (3 per step:
+ * mu, lambda, weight; -1 where none was proposed). *steps_host: entries. */
+vsmch_status vsmch_gmm_run(vsmch_gmm *g, int64_t max_steps,
+    int64_t *steps_host, double *alpha_host, double *ess_host,
+    int *resampled_host, double *logz_host, double *integrand_host,
+    double *accept_host);
+/* The sampler's particle set (borrowed: valid until vsmch_gmm_destroy; not
+ * to be destroyed by the caller). */
+vsmch_status vsmch_gmm_particle(vsmch_gmm *g, vsmch_particle **p);
+/* One MH move of kind 0 (mu), 1 (lambda) or 2 (weight) over all particles
+ * at temperature alpha, with the draws of step iter, move index m. */
+vsmch_status vsmch_gmm_move(vsmch_gmm *g, int kind, double alpha,
+    int64_t iter, int m);
+
 /* vsmch_cv_run with the learned proposal for every move t >= 1. */
 vsmch_status vsmch_cv_run_learned(vsmch_particle *p, const vsmch_cv_model *m,
     const vsmch_mlp *net, int64_t T, const double *obs_x_host,

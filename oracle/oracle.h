@@ -221,6 +221,20 @@ double vo_kalman_cv(int64_t T, double dt, double sd_pos, double sd_vel,
 double vo_cv_loglik(double x, double y, double ox, double oy, int obs_model,
     double obs_scale, double obs_df);
 
+/* ----------------------------------- tempered mixture SMC (gmm.c) -- */
+
+void vo_gmm_hyper(const double *x, int64_t n, double h[4]);
+double vo_gmm_log_prior(int c, const double *mu, const double *lambda,
+    const double h[4]);
+double vo_gmm_log_lik(int c, const double *mu, const double *lambda,
+    const double *weight, const double *x, int64_t n);
+void vo_gmm_scales(double alpha, double sd[3]);
+int vo_gmm_move(int kind, int c, double *p, const double *z, double log_u,
+    double alpha, const double h[4], const double *x, int64_t n);
+double vo_cess(int64_t n, const double *W, const double *ll, double d);
+double vo_cess_solve(int64_t n, const double *W, const double *ll,
+    double dmax, double target, double tol);
+
 /* ---------------------------------------- learned proposal (mlp.c) -- */
 
 int vo_mlp_nparams(void);

@@ -357,3 +357,70 @@ def cv_learned_particle(s0, ox, oy, o, z, dt=0.1, sd_pos=0.02, sd_vel=0.02, obs_
 
 def cv_loglik(x, y, ox, oy, obs_model=0, obs_scale=0.1, obs_df=10.0):
     return _mlp_sig().vo_cv_loglik(x, y, ox, oy, obs_model, obs_scale, obs_df)
+
+
+# ------------------------------------------------- tempered mixture SMC
+
+def _gmm_sig():
+    L = oracle()
+    if getattr(L, "_gmm_ready", False):
+        return L
+    d = P(np.float64)
+    L.vo_gmm_hyper.restype = None
+    L.vo_gmm_hyper.argtypes = [d, c.c_int64, d]
+    L.vo_gmm_log_prior.restype = c.c_double
+    L.vo_gmm_log_prior.argtypes = [c.c_int, d, d, d]
+    L.vo_gmm_log_lik.restype = c.c_double
+    L.vo_gmm_log_lik.argtypes = [c.c_int, d, d, d, d, c.c_int64]
+    L.vo_gmm_scales.restype = None
+    L.vo_gmm_scales.argtypes = [c.c_double, d]
+    L.vo_gmm_move.restype = c.c_int
+    L.vo_gmm_move.argtypes = [c.c_int, c.c_int, d, d, c.c_double, c.c_double, d, d,
+                              c.c_int64]
+    L.vo_cess.restype = c.c_double
+    L.vo_cess.argtypes = [c.c_int64, d, d, c.c_double]
+    L.vo_cess_solve.restype = c.c_double
+    L.vo_cess_solve.argtypes = [c.c_int64, d, d, c.c_double, c.c_double, c.c_double]
+    L._gmm_ready = True
+    return L
+
+
+def _a(x):
+    return np.ascontiguousarray(x, np.float64)
+
+
+def gmm_hyper(x):
+    h = np.zeros(4)
+    _gmm_sig().vo_gmm_hyper(_a(x), len(x), h)
+    return h
+
+
+def gmm_log_prior(mu, lam, h):
+    return _gmm_sig().vo_gmm_log_prior(len(mu), _a(mu), _a(lam), _a(h))
+
+
+def gmm_log_lik(mu, lam, w, x):
+    return _gmm_sig().vo_gmm_log_lik(len(mu), _a(mu), _a(lam), _a(w), _a(x), len(x))
+
+
+def gmm_scales(alpha):
+    sd = np.zeros(3)
+    _gmm_sig().vo_gmm_scales(alpha, sd)
+    return sd
+
+
+def gmm_move(kind, p, z, log_u, alpha, h, x):
+    """p = (mu, lambda, weight, log prior, log likelihood); returns
+    (accepted, new p)."""
+    q = _a(p).copy()
+    c_ = (len(q) - 2) // 3
+    acc = _gmm_sig().vo_gmm_move(kind, c_, q, _a(z), log_u, alpha, _a(h), _a(x), len(x))
+    return acc, q
+
+
+def cess(W, ll, d):
+    return _gmm_sig().vo_cess(len(W), _a(W), _a(ll), d)
+
+
+def cess_solve(W, ll, dmax, target, tol=1e-12):
+    return _gmm_sig().vo_cess_solve(len(W), _a(W), _a(ll), dmax, target, tol)

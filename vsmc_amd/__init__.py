@@ -101,7 +101,21 @@ SIGNATURES = {
                                     _c.POINTER(_c.c_int), _c.POINTER(_c.c_double)]),
     "vsmch_cv_run_learned": (_c.c_int, [_P, _P, _P, _c.c_int64, _P, _P, _c.c_int,
                                         _c.c_double, _P, _P, _P, _P]),
+    "vsmch_gmm_create": (_c.c_int, [_c.POINTER(_P), _P, _P, _c.c_int64]),
+    "vsmch_gmm_destroy": (_c.c_int, [_P]),
+    "vsmch_gmm_run": (_c.c_int, [_P, _c.c_int64, _c.POINTER(_c.c_int64), _P, _P, _P, _P,
+                                 _P, _P]),
+    "vsmch_gmm_particle": (_c.c_int, [_P, _c.POINTER(_P)]),
+    "vsmch_gmm_move": (_c.c_int, [_P, _c.c_int, _c.c_double, _c.c_int64, _c.c_int]),
 }
+
+
+class GmmConfig(ctypes.Structure):
+    """vsmch_gmm_config."""
+    _fields_ = [("n", ctypes.c_int64), ("ncomp", ctypes.c_int), ("seed", ctypes.c_uint32),
+                ("cess", ctypes.c_double), ("moves", ctypes.c_int),
+                ("search_rounds", ctypes.c_int), ("scheme", ctypes.c_int),
+                ("threshold", ctypes.c_double)]
 
 MLP_PARAMS = 5128
 
@@ -292,4 +306,4 @@ def sorted_uniforms(n, key, word_offset=0, device="cuda"):
     return out[:n]
 
 
-from .sampler import Particle, CvFilter, ProposalNet, CvBatch  # noqa: E402,F401
+from .sampler import Particle, CvFilter, ProposalNet, CvBatch, GmmSampler  # noqa: E402,F401

@@ -54,6 +54,7 @@ CvModelDev cv_dev(const vsmch_cv_model *m)
 
 struct vsmch_particle {
     ParticleSet *p;
+    bool borrowed = false;   // owned by another object (vsmch_gmm)
 };
 
 struct vsmch_mlp {
@@ -62,6 +63,11 @@ struct vsmch_mlp {
 
 struct vsmch_batch {
     BatchSet *b;
+};
+
+struct vsmch_gmm {
+    GmmSet *s;
+    vsmch_particle view;
 };
 
 namespace {
@@ -262,6 +268,10 @@ vsmch_status vsmch_particle_destroy(vsmch_particle *p)
 {
     if (!p)
         return VSMCH_OK;
+    if (p->borrowed) {
+        set_error("vsmch_particle_destroy: a borrowed particle set belongs to its sampler");
+        return VSMCH_ERR_INVALID;
+    }
     (void)hipStreamSynchronize(p->p->ws.stream);
     particle_destroy(p->p);
     delete p;
@@ -642,6 +652,102 @@ vsmch_status vsmch_batch_info(vsmch_batch *b, int64_t *tmax,
     if (particles)
         *particles = np;
     return VSMCH_OK;
+}
+
+vsmch_status vsmch_gmm_create(vsmch_gmm **g, const vsmch_gmm_config *cfg,
+    const double *data_host, int64_t ndata)
+{
+    if (!g || !cfg || !data_host) {
+        set_error("vsmch_gmm_create: invalid argument");
+        return VSMCH_ERR_INVALID;
+    }
+    GmmSet *s = nullptr;
+    const int rc = gmm_create(&s, cfg->n, cfg->ncomp, cfg->seed, cfg->cess,
+        cfg->moves, cfg->search_rounds, cfg->scheme, cfg->threshold, data_host,
+        ndata, g_stream);
+    if (rc != VSMCH_OK)
+        return rc;
+    *g = new vsmch_gmm{s, vsmch_particle{gmm_particles(s), true}};
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_gmm_destroy(vsmch_gmm *g)
+{
+    if (!g)
+        return VSMCH_ERR_INVALID;
+    gmm_destroy(g->s);
+    delete g;
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_gmm_run(vsmch_gmm *g, int64_t max_steps,
+    int64_t *steps_host, double *alpha_host, double *ess_host,
+    int *resampled_host, double *logz_host, double *integrand_host,
+    double *accept_host)
+{
+    if (!g || max_steps < 1) {
+        set_error("vsmch_gmm_run: invalid argument");
+        return VSMCH_ERR_INVALID;
+    }
+    const size_t T = static_cast<size_t>(max_steps);
+    double *a = nullptr, *e = nullptr, *z = nullptr, *in = nullptr, *acc = nullptr;
+    int *rs = nullptr;
+    if (hipMalloc(&a, sizeof(double) * T) != hipSuccess ||
+        hipMalloc(&e, sizeof(double) * T) != hipSuccess ||
+        hipMalloc(&z, sizeof(double) * T) != hipSuccess ||
+        hipMalloc(&in, sizeof(double) * T) != hipSuccess ||
+        hipMalloc(&acc, sizeof(double) * 3 * T) != hipSuccess ||
+        hipMalloc(&rs, sizeof(int) * T) != hipSuccess) {
+        void *ps[] = {a, e, z, in, acc, rs};
+        for (void *p : ps)
+            (void)hipFree(p);
+        set_error("vsmch_gmm_run: device allocation failed");
+        return VSMCH_ERR_ALLOC;
+    }
+    long long steps = 0;
+    int rc = gmm_run(g->s, max_steps, a, e, rs, z, in, acc, &steps);
+    if (rc == VSMCH_OK) {
+        const size_t k = static_cast<size_t>(steps);
+        struct {
+            void *dst;
+            const void *src;
+            size_t bytes;
+        } cp[] = {{alpha_host, a, sizeof(double) * k}, {ess_host, e, sizeof(double) * k},
+            {resampled_host, rs, sizeof(int) * k}, {logz_host, z, sizeof(double) * k},
+            {integrand_host, in, sizeof(double) * k},
+            {accept_host, acc, sizeof(double) * 3 * k}};
+        for (auto &c : cp)
+            if (c.dst && c.bytes &&
+                hipMemcpy(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+                set_error("vsmch_gmm_run: copy failed");
+                rc = VSMCH_ERR_HIP;
+            }
+        if (steps_host)
+            *steps_host = steps;
+    }
+    void *ps[] = {a, e, z, in, acc, rs};
+    for (void *p : ps)
+        (void)hipFree(p);
+    return rc;
+}
+
+vsmch_status vsmch_gmm_particle(vsmch_gmm *g, vsmch_particle **p)
+{
+    if (!g || !p)
+        return VSMCH_ERR_INVALID;
+    *p = &g->view;
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_gmm_move(vsmch_gmm *g, int kind, double alpha,
+    int64_t iter, int m)
+{
+    if (!g || kind < 0 || kind > 2 || iter < 0 || m < 0 || m >= 32)
+        return VSMCH_ERR_INVALID;
+    int rc = gmm_set_alpha(g->s, alpha);
+    if (rc != VSMCH_OK)
+        return rc;
+    return gmm_move(g->s, kind, iter, m);
 }
 
 vsmch_status vsmch_cv_run_learned(vsmch_particle *p, const vsmch_cv_model *m,

@@ -125,6 +125,19 @@ int batch_read_filter(BatchSet *b, int f, float *state, double *W);
 void batch_info(const BatchSet *b, long long *tmax, long long *particles,
     int *move_blocks, double *run_ms);
 
+// gmm.hip -- tempered SMC for a Gaussian mixture
+struct GmmSet;
+int gmm_create(GmmSet **out, long long n, int c, uint32_t seed, double cess,
+    int moves, int rounds, int scheme, double threshold, const double *data,
+    long long ndata, hipStream_t st);
+void gmm_destroy(GmmSet *s);
+ParticleSet *gmm_particles(GmmSet *s);
+int gmm_move(GmmSet *s, int kind, long long t, int m);
+int gmm_set_alpha(GmmSet *s, double alpha);
+int gmm_run(GmmSet *s, long long max_steps, double *alpha_d, double *ess_d,
+    int *rs_d, double *logz_d, double *integ_d, double *acc_d,
+    long long *steps_out);
+
 // rng.hip
 int stream_fill(int kind, unsigned long long key, unsigned long long offset,
     long long n, void *out, int mode, hipStream_t st);

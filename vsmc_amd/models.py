@@ -62,3 +62,13 @@ def cv_simulate_many(F, T, seed, dt=0.1, sd_pos=0.02, sd_vel=0.02, sd_pos0=2.0, 
         ox[:, t] = x[0] + e[0]
         oy[:, t] = x[1] + e[1]
     return ox, oy
+
+
+def gmm_simulate(n=1000, seed=2016, means=(-3.0, 0.0, 2.0, 5.0), sds=(0.5, 1.5, 0.75, 1.0),
+                 weights=(0.1, 0.2, 0.3, 0.4)):
+    """n points from a seeded 1-D Gaussian mixture (the tempered SMC
+    workload: 4 components, means -3, 0, 2, 5, sd 0.5-1.5, weights
+    0.1-0.4)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    comp = rng.choice(len(means), size=n, p=np.asarray(weights))
+    return rng.normal(np.asarray(means)[comp], np.asarray(sds)[comp])

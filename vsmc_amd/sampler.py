@@ -22,9 +22,10 @@ class Particle:
         self._h = h
 
     def close(self):
-        if getattr(self, "_h", None):
+        # a borrowed view (GmmSampler.particle) belongs to its owner
+        if getattr(self, "_h", None) and not getattr(self, "_borrowed", False):
             lib().vsmch_particle_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         try:
@@ -289,6 +290,67 @@ class ProposalNet:
                                     ctypes.c_void_p(target.data_ptr()), batch, steps, lr,
                                     loss.ctypes.data_as(ctypes.c_void_p)))
         return loss[:steps]
+
+
+class GmmSampler:
+    """Tempered SMC for a c-component Gaussian mixture on the device (the
+    sampler of example/gmm/include/gmm.hpp with adaptive tempering to a
+    conditional ESS target); particles are 3c + 2 FP32 components."""
+
+    def __init__(self, n, data, ncomp=4, seed=101, cess=0.9, moves=5, search_rounds=8,
+                 scheme=None, threshold=0.5):
+        import numpy as np
+        from . import GmmConfig, STRATIFIED
+        self.n, self.c = int(n), int(ncomp)
+        cfg = GmmConfig(self.n, self.c, seed, cess, moves, search_rounds,
+                        STRATIFIED if scheme is None else scheme, threshold)
+        self.data = np.ascontiguousarray(data, dtype=np.float64)
+        h = ctypes.c_void_p()
+        check(lib().vsmch_gmm_create(ctypes.byref(h), ctypes.byref(cfg),
+                                     self.data.ctypes.data_as(ctypes.c_void_p), len(self.data)))
+        self._h = h
+        ph = ctypes.c_void_p()
+        check(lib().vsmch_gmm_particle(self._h, ctypes.byref(ph)))
+        self.particle = Particle.__new__(Particle)
+        self.particle._h, self.particle.n, self.particle.dim = ph, self.n, 3 * self.c + 2
+        import torch
+        self.particle.dtype = torch.float32
+        self.particle._borrowed = True
+
+    def close(self):
+        if getattr(self, "_h", None):
+            check(lib().vsmch_gmm_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, max_steps=200):
+        import numpy as np
+        T = int(max_steps)
+        steps = ctypes.c_int64()
+        a, e, z, g = (np.zeros(T) for _ in range(4))
+        rs = np.zeros(T, dtype=np.int32)
+        acc = np.zeros((T, 3))
+        P = lambda x: x.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        check(lib().vsmch_gmm_run(self._h, T, ctypes.byref(steps), P(a), P(e), P(rs), P(z),
+                                  P(g), P(acc)))
+        k = steps.value
+        return {"steps": k, "alpha": a[:k], "ess": e[:k], "resampled": rs[:k],
+                "logz": z[:k], "integrand": g[:k], "accept": acc[:k],
+                "path_sampling": float(np.sum(0.5 * (g[1:k] + g[:k - 1]) * np.diff(a[:k])))}
+
+    def move(self, kind, alpha, iteration, m):
+        check(lib().vsmch_gmm_move(self._h, kind, alpha, iteration, m))
+
+    def state(self):
+        return self.particle.state()
+
+    def weights(self):
+        return self.particle.weights()
 
 
 class CvBatch:
