@@ -313,6 +313,58 @@ def batched_filters(va, torch, F=1024, seed=2024, plain_sample=24):
             "speedup_vs_plain": (updates / secs) / (p_updates / p_secs)}
 
 
+def tempered_gmm(va, torch, n=1 << 20, moves=5, cpu_budget=4.0):
+    """Tempered SMC for the 4-component mixture on 1,000 seeded points:
+    N = 2^20 FP32 12-D particles, adaptive tempering to conditional ESS 0.9,
+    5 RW-MH moves per step (each move one likelihood evaluation of 1,000
+    points x 4 components). The CPU baseline is the oracle's move (FP64) on
+    the host cores for a sample of the final particles."""
+    import ctypes
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    from vsmc_amd.models import gmm_simulate
+    x = gmm_simulate(1000, 2016)
+    s = va.GmmSampler(n, x, ncomp=4, seed=101, cess=0.9, moves=moves)
+    s.run(max_steps=3)   # code load and allocation
+    r, secs = _timed(torch, lambda: s.run(max_steps=1000))
+    st = s.state().cpu().numpy().astype(np.float64)
+    s.close()
+    steps = r["steps"]
+    gpu_moves = float(n) * (steps - 1) * moves
+    # CPU: the same move on a growing sample of the final particles
+    L = ol._gmm_sig()
+    L.vo_gmm_moves.restype = ctypes.c_int64
+    P = np.ctypeslib.ndpointer
+    L.vo_gmm_moves.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                               P(np.float64), P(np.float64), P(np.float64), ctypes.c_double,
+                               P(np.float64), P(np.float64), ctypes.c_int64, ctypes.c_int]
+    h = ol.gmm_hyper(x)
+    threads = min(os.cpu_count() or 1, 16)
+    rs = np.random.default_rng(0)
+    m = 4096
+    while True:
+        p = np.ascontiguousarray(st[:, :m].T)
+        z = rs.normal(size=(m, 4))
+        lu = np.log(rs.random(m))
+        t0 = time.perf_counter()
+        L.vo_gmm_moves(0, 4, m, p, z, lu, 1.0, h, x, len(x), threads)
+        cs = time.perf_counter() - t0
+        if cs >= cpu_budget / 4 or m >= n:
+            break
+        m = min(4 * m, n)
+    return {"n": n, "dim": 12, "data_points": len(x), "moves_per_step": moves,
+            "steps": steps, "gpu_s": secs, "ms_per_step": secs * 1e3 / max(steps - 1, 1),
+            "gpu_moves_per_s": gpu_moves / secs,
+            "gpu_component_evals_per_s": gpu_moves * len(x) * 4 / secs,
+            "log_evidence_smc": float(r["logz"][-1]),
+            "log_evidence_path_sampling": r["path_sampling"],
+            "accept_last": [float(a) for a in r["accept"][-1]],
+            "cpu_baseline": {"moves_per_s": m / cs, "cores": threads, "kind": "port",
+                             "sample": f"oracle mu move (FP64) on {m} final particles, "
+                                       f"{cs:.2f}s"}}
+
+
 def measured_traffic():
     """HBM bytes per move launch from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy kernel), if any."""
@@ -413,6 +465,7 @@ def main():
         out["microbench"] = microbench(va, torch)
         out["learned_proposal"] = learned_vs_plain(va, torch)
         out["batched_filters"] = batched_filters(va, torch)
+        out["tempered_gmm"] = tempered_gmm(va, torch)
     out["cpu_baseline"] = None if a.no_cpu else cpu_baseline(a.cpu_seconds)
     print(json.dumps(out), flush=True)
 

@@ -31,6 +31,9 @@
 
 #include <math.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define LN_2PI 1.8378770664093454835606594728112352797227949472755668
 
@@ -167,4 +170,23 @@ double vo_cess_solve(int64_t n, const double *W, const double *ll,
             hi = mid;
     }
     return lo;
+}
+
+/* vo_gmm_move over n particles (rows of 3c + 2 in p, c normals per particle
+ * in z, log-uniforms in lu) on `threads` OpenMP threads (0: all); returns
+ * the number accepted. The CPU baseline of the device's moves. */
+int64_t vo_gmm_moves(int kind, int c, int64_t n, double *p, const double *z,
+    const double *lu, double alpha, const double h[4], const double *x,
+    int64_t nd, int threads)
+{
+    int64_t acc = 0;
+#ifdef _OPENMP
+    if (threads > 0)
+        omp_set_num_threads(threads);
+#pragma omp parallel for reduction(+ : acc) schedule(static)
+#endif
+    for (int64_t i = 0; i < n; ++i)
+        acc += vo_gmm_move(kind, c, p + i * (3 * c + 2), z + i * c, lu[i], alpha,
+            h, x, nd);
+    return acc;
 }

@@ -231,6 +231,9 @@ double vo_gmm_log_lik(int c, const double *mu, const double *lambda,
 void vo_gmm_scales(double alpha, double sd[3]);
 int vo_gmm_move(int kind, int c, double *p, const double *z, double log_u,
     double alpha, const double h[4], const double *x, int64_t n);
+int64_t vo_gmm_moves(int kind, int c, int64_t n, double *p, const double *z,
+    const double *lu, double alpha, const double h[4], const double *x,
+    int64_t nd, int threads);
 double vo_cess(int64_t n, const double *W, const double *ll, double d);
 double vo_cess_solve(int64_t n, const double *W, const double *ll,
     double dmax, double target, double tol);

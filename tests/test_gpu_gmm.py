@@ -130,3 +130,21 @@ def test_run_end_to_end(dev):
     mu = np.sort(st[0:4], axis=0)
     post = (mu * W[None, :]).sum(axis=1)
     np.testing.assert_allclose(post, [-3.0, 0.0, 2.0, 5.0], atol=0.6)
+
+
+def test_large_n_finite(dev):
+    """N = 2^20 draws some FP32 uniforms of exactly 0 (infinite Box-Muller
+    normals): their proposals are rejected, the state stays finite and the
+    run reaches alpha = 1 in a few dozen steps."""
+    import vsmc_amd as va
+    from vsmc_amd.models import gmm_simulate
+    x = gmm_simulate(1000, 2016)
+    s = va.GmmSampler(1 << 20, x, ncomp=4, seed=101, cess=0.9, moves=5)
+    r = s.run(max_steps=200)
+    st = s.state().cpu().numpy()
+    s.close()
+    print("steps", r["steps"], "logz", r["logz"][-1], "path", r["path_sampling"], flush=True)
+    assert r["alpha"][-1] == 1.0
+    assert np.isfinite(st).all()
+    assert np.isfinite(r["integrand"]).all()
+    assert abs(r["logz"][-1] - r["path_sampling"]) < 2.0

@@ -261,7 +261,11 @@ __global__ void __launch_bounds__(kGB) k_gmm_move(float *state, long long n,
         const float lp1 = gmm_log_prior(c, mu2, lam2, g);
         const float ll1 = gmm_log_lik(c, mu2, lam2, w2, xs, ndata);
         const float pr = jac + lp1 + alpha * ll1 - (lp0 + alpha * ll0);
-        if (!(pr < log_u)) {
+        // The reference rejects when pr < log u (mh_reject_common). A float
+        // uniform of exactly 0 (probability 2^-24) makes an infinite normal
+        // and can make pr NaN (inf - inf); such a proposal is rejected here
+        // instead of accepted.
+        if (pr >= log_u) {
             acc = 1;
             for (int k = 0; k < c; ++k) {
                 state[k * n + i] = mu2[k];
