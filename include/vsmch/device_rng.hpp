@@ -68,7 +68,9 @@ __host__ __device__ inline u32x4 philox_block(uint32_t key0, uint64_t b)
 __host__ __device__ inline uint32_t philox_word(uint32_t key0, uint64_t k)
 {
     const u32x4 blk = philox_block(key0, k / 4 + 1);
-    return blk.v[k % 4];
+    // a select, not blk.v[k % 4]: a run-time index puts blk in scratch
+    const unsigned r = static_cast<unsigned>(k & 3);
+    return r == 0 ? blk.v[0] : r == 1 ? blk.v[1] : r == 2 ? blk.v[2] : blk.v[3];
 }
 
 // ---------------------------------------------------------- Threefry4x64-20

@@ -134,6 +134,15 @@ __device__ inline T block_reduce(T v, Op op, T *lds)
     return r;
 }
 
+// x through an empty asm statement: values the compiler derived from x
+// before this point cannot be reused after it, so a second pass that
+// recomputes them keeps no array of intermediates live in between.
+__device__ inline double opaque(double x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 struct OpAddD {
     __device__ double operator()(double a, double b) const { return a + b; }
 };
@@ -148,6 +157,9 @@ struct OpMinL {
     {
         return a < b ? a : b;
     }
+};
+struct OpMinI {
+    __device__ int operator()(int a, int b) const { return a < b ? a : b; }
 };
 struct OpMaxD {
     __device__ double operator()(double a, double b) const
@@ -164,9 +176,10 @@ struct OpMaxD {
 __device__ inline int lds_pad(int e) { return e + (e / kItems); }
 constexpr int kTileLds = kTile + kTile / kItems;
 
+// Stage the tile into LDS only (element k at lds_pad(k)), `pad` past n.
 template <typename Src>
-__device__ inline void load_tile_blocked(Src src, long long base,
-    long long n, double (&w)[kItems], double *lds_tile, double pad)
+__device__ __forceinline__ void load_tile_lds(Src src, long long base, long long n,
+    double *lds_tile, double pad)
 {
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
@@ -175,6 +188,13 @@ __device__ inline void load_tile_blocked(Src src, long long base,
         lds_tile[lds_pad(e)] = g < n ? src(g) : pad;
     }
     __syncthreads();
+}
+
+template <typename Src>
+__device__ __forceinline__ void load_tile_blocked(Src src, long long base,
+    long long n, double (&w)[kItems], double *lds_tile, double pad)
+{
+    load_tile_lds(src, base, n, lds_tile, pad);
 #pragma unroll
     for (int j = 0; j < kItems; ++j)
         w[j] = lds_tile[lds_pad(threadIdx.x * kItems + j)];

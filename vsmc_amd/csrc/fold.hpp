@@ -137,17 +137,20 @@ __device__ inline bool apply_segments(const TileSeg &sg, double &x)
 
 struct ExactShared {
     FMap scan[kWaves];
-    long long red[kWaves];
-    double b0;
-    double b1;
+    int red[kWaves];
+    double dred[kWaves];
 };
 
-// Exact inclusive fold of one tile held in blocked order, from the exact
-// start x0. All threads of the block must call it (it synchronises).
-__device__ inline void tile_exact(const double (&w)[kItems], double x0,
-    double (&s)[kItems], ExactShared &sh)
+// Exact inclusive fold of one tile, in place: t (element k at lds_pad(k))
+// holds the weights on entry and s_k on return, from the exact start x0.
+// Thread r owns elements r*kItems .. r*kItems+kItems-1. Sums are written
+// only below the first crossing not yet done, so the weights from there on
+// stay readable. All threads of the block must call it (it synchronises,
+// also before returning).
+__device__ __forceinline__ void tile_exact(double *t, double x0, ExactShared &sh)
 {
     const int base = threadIdx.x * kItems;
+    double *const mine = t + lds_pad(base);
     int pos = 0;
     double cur = x0;
     while (pos < kTile) {
@@ -155,65 +158,65 @@ __device__ inline void tile_exact(const double (&w)[kItems], double x0,
 #pragma unroll
             for (int j = 0; j < kItems; ++j)
                 if (base + j >= pos)
-                    s[j] = cur;
+                    mine[j] = cur;
             break;
         }
         if (cur == 0.0) {
-            long long lf = kTile;
+            // the sum stays 0 up to the first non-zero weight k; s_k = w_k
+            int lf = kTile;
 #pragma unroll
             for (int j = 0; j < kItems; ++j)
-                if (base + j >= pos && w[j] != 0.0 && lf == kTile)
+                if (base + j >= pos && lf == kTile && mine[j] != 0.0)
                     lf = base + j;
-            const long long k = block_reduce(lf, OpMinL(), sh.red);
+            const int k = block_reduce(lf, OpMinI(), sh.red);
 #pragma unroll
             for (int j = 0; j < kItems; ++j) {
                 const int idx = base + j;
                 if (idx >= pos && idx < k)
-                    s[j] = 0.0;
-                if (idx == k) {
-                    s[j] = w[j];
-                    sh.b0 = w[j];
-                }
+                    mine[j] = 0.0;
             }
             if (k >= kTile)
                 break;
-            __syncthreads();
-            cur = sh.b0;
-            __syncthreads();
-            pos = static_cast<int>(k) + 1;
+            cur = t[lds_pad(k)];
+            pos = k + 1;
             continue;
         }
         const int e = binade(cur);
         const long long a0 = mant(cur);
         const int p0 = static_cast<int>(a0 & 1);
         // Without an exact half-ulp tie the maps do not depend on parity:
-        // the fold is a saturating integer prefix sum of the rounded steps.
-        long long agg = 0;
+        // the fold is a prefix sum of the rounded steps d_j = rint(w_j /
+        // ulp). They are integers, exact in FP64 below 2^53, and a sum at or
+        // above 2^53 (exact or not) marks the binade crossing, so this path
+        // runs in FP64 arithmetic (a non-finite sum also counts as a
+        // crossing, which the real add then carries).
+        const double a0d = static_cast<double>(a0);
+        const double two53 = static_cast<double>(kTwo53);
+        double dsum = 0.0;
         bool tie = false;
 #pragma unroll
         for (int j = 0; j < kItems; ++j) {
             if (base + j >= pos) {
-                const FMap m = elem_map(w[j], e);
-                agg = sat(agg + m.d0);
-                tie = tie || (m.d0 != m.d1);
+                const double q = ldexp(mine[j], 52 - e);
+                tie = tie || (q - trunc(q) == 0.5);
+                dsum += rint(q);
             }
         }
-        long long lc = kTile;
+        int lc = kTile;
         if (!__syncthreads_or(tie)) {
-            long long ltot;
-            long long run = block_exclusive_scan(agg, OpAddSatL(), 0LL, sh.red,
-                &ltot);
+            double dtot;
+            double run = block_exclusive_scan(dsum, OpAddD(), 0.0, sh.dred, &dtot);
 #pragma unroll
             for (int j = 0; j < kItems; ++j) {
                 const int idx = base + j;
                 if (idx >= pos) {
-                    run = sat(run + elem_map(w[j], e).d0);
-                    const long long a = a0 + run;
-                    if (a >= kTwo53) {
+                    run += rint(ldexp(mine[j], 52 - e));
+                    const double a = a0d + run;
+                    if (!(a < two53)) {
                         if (lc == kTile)
                             lc = idx;
                     } else {
-                        s[j] = from_mant(a, e);
+                        mine[j] = ldexp(a, e - 52);
                     }
                 }
             }
@@ -222,7 +225,7 @@ __device__ inline void tile_exact(const double (&w)[kItems], double x0,
 #pragma unroll
             for (int j = 0; j < kItems; ++j)
                 if (base + j >= pos)
-                    magg = compose(magg, elem_map(w[j], e));
+                    magg = compose(magg, elem_map(mine[j], e));
             FMap total;
             FMap run = block_exclusive_scan(magg, OpCompose(), ident(), sh.scan,
                 &total);
@@ -230,38 +233,31 @@ __device__ inline void tile_exact(const double (&w)[kItems], double x0,
             for (int j = 0; j < kItems; ++j) {
                 const int idx = base + j;
                 if (idx >= pos) {
-                    run = compose(run, elem_map(w[j], e));
+                    run = compose(run, elem_map(mine[j], e));
                     const long long a = a0 + (p0 ? run.d1 : run.d0);
                     if (a >= kTwo53) {
                         if (lc == kTile)
                             lc = idx;
                     } else {
-                        s[j] = from_mant(a, e);
+                        mine[j] = from_mant(a, e);
                     }
                 }
             }
         }
-        const long long c = block_reduce(lc, OpMinL(), sh.red);
+        // the sums are non-decreasing, so every element from the first
+        // crossing c on is still a weight
+        const int c = block_reduce(lc, OpMinI(), sh.red);
         if (c >= kTile)
             break;
-#pragma unroll
-        for (int j = 0; j < kItems; ++j)
-            if (base + j == c - 1)
-                sh.b0 = s[j];
+        // the crossing element: one real add from the exact previous sum
+        const double prev = (c == pos) ? cur : t[lds_pad(c - 1)];
+        cur = prev + t[lds_pad(c)];
         __syncthreads();
-        const double prev = (c == pos) ? cur : sh.b0;
-#pragma unroll
-        for (int j = 0; j < kItems; ++j) {
-            if (base + j == c) {
-                s[j] = prev + w[j];
-                sh.b1 = s[j];
-            }
-        }
-        __syncthreads();
-        cur = sh.b1;
-        __syncthreads();
-        pos = static_cast<int>(c) + 1;
+        if (c >= base && c < base + kItems)
+            mine[c - base] = cur;
+        pos = c + 1;
     }
+    __syncthreads();
 }
 
 // Segmented composition of (flag, map) pairs: a set flag starts a run.
@@ -293,7 +289,7 @@ struct SegShared {
 
 // Tile summary from the approximate prefix tpre (start of this tile).
 // w holds the tile in blocked order, nvalid elements valid.
-__device__ inline void tile_segments(const double (&w)[kItems], int nvalid,
+__device__ __forceinline__ void tile_segments(const double (&w)[kItems], int nvalid,
     double tpre, TileSeg *out, SegShared &sh)
 {
     const int base = threadIdx.x * kItems;
@@ -357,19 +353,23 @@ __device__ inline void tile_segments(const double (&w)[kItems], int nvalid,
     __syncthreads();
     if (ltot == 0) {
         // one run in one binade (every prev > 0 and finite); without a
-        // half-ulp tie its map is the saturating sum of the rounded steps
-        long long dsum = 0;
+        // half-ulp tie its map is the sum of the rounded steps, exact in
+        // FP64 below 2^53 (larger or non-finite sums saturate: the resolver
+        // then sees the crossing)
+        double dsum = 0.0;
         bool tie = false;
 #pragma unroll
         for (int j = 0; j < kItems; ++j) {
             if (base + j < nvalid) {
-                const FMap m = elem_map(w[j], eb[j]);
-                dsum = sat(dsum + m.d0);
-                tie = tie || (m.d0 != m.d1);
+                const double q = ldexp(w[j], 52 - eb[j]);
+                tie = tie || (q - trunc(q) == 0.5);
+                dsum += rint(q);
             }
         }
         if (!__syncthreads_or(tie)) {
-            const long long d = block_reduce(dsum, OpAddSatL(), sh.lscan);
+            const double dd = block_reduce(dsum, OpAddD(), sh.dscan);
+            const long long d = dd < static_cast<double>(kCap) ?
+                static_cast<long long>(dd) : kCap;
             if (threadIdx.x == 0) {
                 sh.seg.e[0] = eb[0];
                 sh.seg.map[0] = FMap{d, d};

@@ -465,17 +465,11 @@ __global__ void __launch_bounds__(kBlock) k_resolve(WSrc src, long long nh,
             __syncthreads();
             if (!ok) {
                 const long long base = t * kTile;
-                double w[kItems], s[kItems];
-                load_tile_blocked(src, base, n, w, sh.tile, 0.0);
-                fold::tile_exact(w, x, s, sh.ex);
+                load_tile_lds(src, base, n, sh.tile, 0.0);
+                fold::tile_exact(sh.tile, x, sh.ex);
                 const long long rem = n - base;
                 const int last = (rem < kTile ? static_cast<int>(rem) : kTile) - 1;
-#pragma unroll
-                for (int j = 0; j < kItems; ++j)
-                    if (static_cast<int>(threadIdx.x) * kItems + j == last)
-                        sh.bx = s[j];
-                __syncthreads();
-                y = sh.bx;
+                y = sh.tile[lds_pad(last)];
                 __syncthreads();
             }
             x = y;
@@ -525,7 +519,7 @@ struct FoldBufs {
 };
 
 template <class WSrc, class USrc>
-__device__ inline double count_tile(WSrc src, USrc us, long long M,
+__device__ __forceinline__ double count_tile(WSrc src, USrc us, long long M,
     long long tidx, double x0, int *rep, long long *tagg);
 
 // Replication counts (and residual integer parts) for one tile.
@@ -547,46 +541,49 @@ __global__ void __launch_bounds__(kBlock) k_count(WSrc src, USrc us,
 // aggregates to tagg[3 tidx ..] when tagg is non-null. Returns the exact
 // sum after the tile's last element (the next tile's start).
 template <class WSrc, class USrc>
-__device__ inline double count_tile(WSrc src, USrc us, long long M,
+__device__ __forceinline__ double count_tile(WSrc src, USrc us, long long M,
     long long tidx, double x0, int *rep, long long *tagg)
 {
     const long long base = tidx * kTile;
     __shared__ double tile[kTileLds];
     __shared__ fold::ExactShared ex;
-    __shared__ double lasts[kBlock];
+    __shared__ long long gfirst;
     __shared__ double xend;
     __shared__ long long red[kWaves];
-    double w[kItems], s[kItems];
-    load_tile_blocked(src, base, M, w, tile, 0.0);
-    fold::tile_exact(w, x0, s, ex);
-    lasts[threadIdx.x] = s[kItems - 1];
-    __syncthreads();
-    const long long ib = base + static_cast<long long>(threadIdx.x) * kItems;
+    load_tile_lds(src, base, M, tile, 0.0);
+    fold::tile_exact(tile, x0, ex);
     const long long nd = us.ndraw();
-    // count before this thread's first element: the previous element's sum
+    const int nvalid = M - base < kTile ? static_cast<int>(M - base) : kTile;
+    // count before the tile's first element: that of the previous sum
     // (0 before particle 0: the merge starts at j = 0)
-    long long gp = 0;
-    if (ib < M) {
-        if (threadIdx.x == 0)
-            gp = (tidx == 0) ? 0 : us.count(x0);
-        else
-            gp = us.count(lasts[threadIdx.x - 1]);
-    }
-    const long long last = (M - base < kTile ? M : base + kTile) - 1;
-    long long z = 0, e = 0, c = 0;
-#pragma unroll
+    if (threadIdx.x == 0)
+        gfirst = (tidx == 0) ? 0 : us.count(x0);
+    // each sum s_k is replaced by its count G(s_k); the last particle takes
+    // every remaining draw
     for (int j = 0; j < kItems; ++j) {
-        const long long i = ib + j;
-        if (i < M) {
-            const long long g = (i == M - 1) ? nd : us.count(s[j]);
-            const long long r = g - gp + src.integ(i);
-            gp = g;
-            rep[i] = static_cast<int>(r);
+        const int k = j * kBlock + static_cast<int>(threadIdx.x);
+        if (k < nvalid) {
+            const double s = tile[lds_pad(k)];
+            if (k == nvalid - 1)
+                xend = s;
+            const long long g = (base + k == M - 1) ? nd : us.count(s);
+            tile[lds_pad(k)] = __builtin_bit_cast(double, g);
+        }
+    }
+    __syncthreads();
+    long long z = 0, e = 0, c = 0;
+#pragma unroll 4
+    for (int j = 0; j < kItems; ++j) {
+        const int k = j * kBlock + static_cast<int>(threadIdx.x);
+        if (k < nvalid) {
+            const long long g = __builtin_bit_cast(long long, tile[lds_pad(k)]);
+            const long long gp = (k == 0) ? gfirst :
+                __builtin_bit_cast(long long, tile[lds_pad(k - 1)]);
+            const long long r = g - gp + src.integ(base + k);
+            rep[base + k] = static_cast<int>(r);
             z += r == 0;
             e += r > 1 ? r - 1 : 0;
             c += r > 1;
-            if (i == last)
-                xend = s[j];
         }
     }
     if (tagg) {
@@ -633,18 +630,14 @@ __global__ void __launch_bounds__(kBlock) k_prefix(WSrc src, long long nh,
         return;
     __shared__ double tile[kTileLds];
     __shared__ fold::ExactShared ex;
-    __shared__ double x0s;
-    double w[kItems], s[kItems];
-    load_tile_blocked(src, base, n, w, tile, 0.0);
-    if (threadIdx.x == 0)
-        x0s = fb.xstart[blockIdx.x];
-    __syncthreads();
-    fold::tile_exact(w, x0s, s, ex);
-    const long long ib = base + static_cast<long long>(threadIdx.x) * kItems;
+    load_tile_lds(src, base, n, tile, 0.0);
+    fold::tile_exact(tile, fb.xstart[blockIdx.x], ex);
 #pragma unroll
-    for (int j = 0; j < kItems; ++j)
-        if (ib + j < n)
-            epi(ib + j, s[j]);
+    for (int j = 0; j < kItems; ++j) {
+        const int k = j * kBlock + static_cast<int>(threadIdx.x);
+        if (base + k < n)
+            epi(base + k, tile[lds_pad(k)]);
+    }
 }
 
 // ------------------------------------------------- running maximum (v)
@@ -772,7 +765,7 @@ struct OpAddL3 {
     }
 };
 
-__device__ inline void scan_tiles_block(const long long *tagg, long long ntiles,
+__device__ __forceinline__ void scan_tiles_block(const long long *tagg, long long ntiles,
     long long *toff);
 
 __global__ void __launch_bounds__(kBlock) k_ridx_scan(const long long *tagg,
@@ -785,7 +778,7 @@ __global__ void __launch_bounds__(kBlock) k_ridx_scan(const long long *tagg,
 
 // Exclusive offsets of the per-tile (childless, extras, sources) triples,
 // totals at toff[3 ntiles ..]. One block.
-__device__ inline void scan_tiles_block(const long long *tagg, long long ntiles,
+__device__ __forceinline__ void scan_tiles_block(const long long *tagg, long long ntiles,
     long long *toff)
 {
     __shared__ L3 lds[kWaves];
@@ -812,7 +805,7 @@ __device__ inline void scan_tiles_block(const long long *tagg, long long ntiles,
 }
 
 template <bool kExpand>
-__device__ inline void scatter_tile(const int *rep, long long M, long long tidx,
+__device__ __forceinline__ void scatter_tile(const int *rep, long long M, long long tidx,
     const long long *toff, int *zdst, int *cE, int *csrc, int *idx);
 
 template <bool kExpand>
@@ -829,7 +822,7 @@ __global__ void __launch_bounds__(kBlock) k_ridx_scatter(const int *rep,
 // copies (csrc, with cE the exclusive prefix of their extra copies) of tile
 // tidx of rep, at the offsets toff[3 tidx ..] of the tile.
 template <bool kExpand>
-__device__ inline void scatter_tile(const int *rep, long long M, long long tidx,
+__device__ __forceinline__ void scatter_tile(const int *rep, long long M, long long tidx,
     const long long *toff, int *zdst, int *cE, int *csrc, int *idx)
 {
     const long long base = tidx * kTile;
@@ -911,7 +904,7 @@ __device__ inline long long upper_bound_i(const int *a, long long n, long long x
 // expand mode): the source whose copies cover k. Optionally gathers an SoA
 // state in place: x[d][dst] = x[d][src] (sources are never destinations).
 template <bool kExpand, typename E>
-__device__ inline void expand_chunk(long long chunk, long long K, long long C,
+__device__ __forceinline__ void expand_chunk(long long chunk, long long K, long long C,
     const int *zdst, const int *cE, const int *csrc, int *idx, E *state,
     int dim, long long ld);
 
@@ -933,7 +926,7 @@ __global__ void __launch_bounds__(kBlock) k_expand(const ResampleDev *prm,
 // Ranks [chunk kTile, (chunk + 1) kTile) of the K ranks: the source of each
 // rank is the one of the C sources whose copies cover it.
 template <bool kExpand, typename E>
-__device__ inline void expand_chunk(long long chunk, long long K, long long C,
+__device__ __forceinline__ void expand_chunk(long long chunk, long long K, long long C,
     const int *zdst, const int *cE, const int *csrc, int *idx, E *state,
     int dim, long long ld)
 {
