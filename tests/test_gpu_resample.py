@@ -205,3 +205,40 @@ def test_full_size_counts(dev, scheme, s):
     assert rep.sum() == n
     bad = np.nonzero(rep != erep)[0]
     assert len(bad) == 0, (len(bad), bad[:5])
+
+
+def test_growing_weights_counts(dev):
+    """Weights growing by a factor 2 every 2^18 particles (N = 2^24): the
+    running sum crosses a binade in every group of 64 tiles, more crossing
+    groups and tiles than the resolver holds in LDS, so the rest are staged
+    as the walk reaches them. Counts bit-exact with the oracle."""
+    import vsmc_amd as va
+    n = 1 << 24
+    i = np.arange(n, dtype=np.float64)
+    rs = np.random.default_rng(3)
+    W = np.exp2(i / (1 << 18)) * (0.5 + rs.random(n))
+    W /= W.sum()
+    for scheme in (ol.SYSTEMATIC, ol.STRATIFIED):
+        print(f"scheme={scheme} n={n} growing", flush=True)
+        rep, _ = va.resample(scheme, _t(W), n, 17, 0)
+        rep = _np(rep)
+        erep, _ = ol.resample(scheme, W, n, 17)
+        bad = np.nonzero(rep != erep)[0]
+        assert len(bad) == 0, (len(bad), bad[:5])
+
+
+def test_several_resolver_windows(dev):
+    """N = 2^26 + 12345 (16,388 tiles, 257 groups): the resolver stages its
+    summaries in five windows of 64 groups. Counts bit-exact with the oracle."""
+    import vsmc_amd as va
+    n = (1 << 26) + 12345
+    print(f"n={n} systematic", flush=True)
+    rs = np.random.default_rng(26)
+    W = np.exp(rs.normal(0, 1.0, n))
+    W /= W.sum()
+    rep, _ = va.resample(ol.SYSTEMATIC, _t(W), n, 5, 0)
+    rep = _np(rep)
+    erep, _ = ol.resample(ol.SYSTEMATIC, W, n, 5)
+    assert rep.sum() == n
+    bad = np.nonzero(rep != erep)[0]
+    assert len(bad) == 0, (len(bad), bad[:5])

@@ -107,6 +107,37 @@ __device__ inline T block_exclusive_scan(T v, Op op, T identity, T *lds,
     return op(wpre, ex);
 }
 
+// Exclusive scan of n values get(i) by one block, put(i, prefix) for each;
+// returns the total. Each thread reads kItems consecutive values per chunk
+// of kTile (all its loads in flight at once), so a chunk costs one round of
+// loads and one block scan. All threads must call it.
+template <typename T, typename Op, typename Get, typename Put>
+__device__ __forceinline__ T block_scan_chunked(long long n, Get get, Put put,
+    Op op, T identity, T *lds)
+{
+    T carry = identity;
+    for (long long c0 = 0; c0 < n; c0 += kTile) {
+        const long long b0 = c0 + static_cast<long long>(threadIdx.x) * kItems;
+        T v[kItems];
+        T loc = identity;
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            v[k] = (b0 + k < n) ? get(b0 + k) : identity;
+            loc = op(loc, v[k]);
+        }
+        T tot;
+        T run = op(carry, block_exclusive_scan(loc, op, identity, lds, &tot));
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            if (b0 + k < n)
+                put(b0 + k, run);
+            run = op(run, v[k]);
+        }
+        carry = op(carry, tot);
+    }
+    return carry;
+}
+
 template <typename T, typename Op>
 __device__ inline T wave_reduce(T v, Op op)
 {

@@ -135,24 +135,342 @@ __device__ inline bool apply_segments(const TileSeg &sg, double &x)
     return true;
 }
 
+// Runs of one tile's multi-run pass (run starts at possible crossings).
+constexpr int kMaxRuns = 256;
+constexpr int kZeroTag = -4000;    // sum_tag of 0
+constexpr int kInfTag = 4000;      // sum_tag of +inf
+constexpr int kNoTag = -8000;      // run of one element
+// Relative distance between an FP64 prefix of a tile (exact start + block
+// scan) and the sequential fold: at most (kTile + 32) * 2^-53 < 2^-41;
+// twice that.
+constexpr double kSumMargin = 0x1p-40;
+
+// Segmented (run count, rounded-step sum) pairs: a run start resets the sum.
+struct SegCnt {
+    int cnt;
+    double d;
+};
+struct OpSegCnt {
+    __device__ SegCnt operator()(SegCnt a, SegCnt b) const
+    {
+        return SegCnt{a.cnt + b.cnt, b.cnt ? b.d : a.d + b.d};
+    }
+};
+
+// Segmented composition of (flag, map) pairs: a set flag starts a run.
+struct SegPair {
+    long long flag;
+    FMap m;
+};
+struct OpSeg {
+    __device__ SegPair operator()(SegPair a, SegPair b) const
+    {
+        SegPair r;
+        r.flag = a.flag | b.flag;
+        r.m = b.flag ? b.m : compose(a.m, b.m);
+        return r;
+    }
+};
+
 struct ExactShared {
     FMap scan[kWaves];
     int red[kWaves];
     double dred[kWaves];
+    SegCnt sred[kWaves];
+    SegPair pred[kWaves];
+    int first_start[kBlock];       // thread's first element starts a run
+    int rf[kMaxRuns + 1];          // first element of each run
+    int rpe[kMaxRuns];             // binade tag of the run after its start
+    FMap rM[kMaxRuns];             // map of the run after its start
+    double rs[kMaxRuns];           // exact sum at the run's start
+    int pos;                       // first element not yet folded
+    double cur;                    // exact sum before it
 };
 
-// Exact inclusive fold of one tile, in place: t (element k at lds_pad(k))
-// holds the weights on entry and s_k on return, from the exact start x0.
-// Thread r owns elements r*kItems .. r*kItems+kItems-1. Sums are written
-// only below the first crossing not yet done, so the weights from there on
-// stay readable. All threads of the block must call it (it synchronises,
-// also before returning).
-__device__ __forceinline__ void tile_exact(double *t, double x0, ExactShared &sh)
+// Binade of a sum, with tags of their own for 0 and +inf.
+__device__ inline int sum_tag(double s)
+{
+    if (s == 0.0)
+        return kZeroTag;
+    if (!(s < INFINITY))
+        return kInfTag;
+    return binade(s);
+}
+
+// Element k may start a binade: the smallest value the exact s_{k-1} can
+// take and the largest value s_k can take lie in different binades.
+__device__ inline bool may_cross(double sprev, double s)
+{
+    return sum_tag(sprev * (1.0 - kSumMargin)) != sum_tag(s * (1.0 + kSumMargin));
+}
+
+__device__ inline bool finite_tag(int tg) { return tg != kZeroTag && tg != kInfTag; }
+
+// Rounded-step total as a map (saturating: at or above 2^54 the run
+// certainly leaves its binade).
+__device__ inline FMap steps_map(double d)
+{
+    const long long k = d < static_cast<double>(kCap) ? static_cast<long long>(d) : kCap;
+    return FMap{k, k};
+}
+
+// Multi-run pass of tile_exact over elements pos0.. (the ones before hold
+// their sums), from the exact sum cur0 before element pos0. An FP64 prefix
+// marks every element where the sum may change binade as a run start
+// (s = s_prev + w there: one real add). Inside a run of binade e the fold is
+// a composition of the per-element maps of adding w at ulp 2^(e-52) (the
+// prefix sum of rint(w / ulp) when no half-ulp tie occurs), so one
+// segmented scan gives every run's map and a serial chain over the runs
+// gives each run's exact start, verifying that it lies in the predicted
+// binade and that the run ends below 2^53. The sums of the verified runs
+// are written in place; sh.pos / sh.cur receive the first element left and
+// the exact sum before it (kTile when the tile is done).
+__device__ __forceinline__ void tile_runs(double *t, int pos0, double cur0,
+    ExactShared &sh)
 {
     const int base = threadIdx.x * kItems;
     double *const mine = t + lds_pad(base);
-    int pos = 0;
-    double cur = x0;
+    double wsum = 0.0;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+        if (base + j >= pos0)
+            wsum += mine[j];
+    double wtot;
+    const double wpre = block_exclusive_scan(wsum, OpAddD(), 0.0, sh.dred, &wtot);
+    // pass 1: run starts and rounded steps; ties
+    SegCnt loc{0, 0.0};
+    bool tie = false;
+    {
+        double acc = wpre;
+        double sprev = cur0 + wpre;
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const int idx = base + j;
+            bool start = false;
+            if (idx >= pos0) {
+                const double w = mine[j];
+                acc += w;
+                const double s = cur0 + acc;
+                start = (idx == pos0) || may_cross(sprev, s);
+                if (start) {
+                    loc.cnt += 1;
+                    loc.d = 0.0;
+                } else {
+                    const int tg = sum_tag(s);
+                    if (finite_tag(tg)) {
+                        const double q = ldexp(w, 52 - tg);
+                        tie = tie || (q - trunc(q) == 0.5);
+                        loc.d += rint(q);
+                    }
+                }
+                sprev = s;
+            }
+            if (j == 0)
+                sh.first_start[threadIdx.x] = start ? 1 : 0;
+        }
+    }
+    SegCnt tot;
+    const SegCnt carry = block_exclusive_scan(loc, OpSegCnt(), SegCnt{0, 0.0},
+        sh.sred, &tot);
+    const bool anytie = __syncthreads_or(tie);
+    // with a tie the steps depend on the parity of the sum: compose maps
+    SegPair mcarry{0, ident()};
+    if (anytie) {
+        SegPair lm{0, ident()};
+        double acc = wpre;
+        double sprev = cur0 + wpre;
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const int idx = base + j;
+            if (idx >= pos0) {
+                const double w = mine[j];
+                acc += w;
+                const double s = cur0 + acc;
+                if ((idx == pos0) || may_cross(sprev, s)) {
+                    lm.flag = 1;
+                    lm.m = ident();
+                } else {
+                    const int tg = sum_tag(s);
+                    if (finite_tag(tg))
+                        lm.m = compose(lm.m, elem_map(w, tg));
+                }
+                sprev = s;
+            }
+        }
+        SegPair mtot;
+        mcarry = block_exclusive_scan(lm, OpSeg(), SegPair{0, ident()}, sh.pred,
+            &mtot);
+    }
+    // pass 2: each run's first element, and at its last element the binade
+    // tag and map
+    {
+        int r = carry.cnt - 1;
+        double dacc = carry.d;
+        FMap macc = mcarry.m;
+        double acc = wpre;
+        double sprev = cur0 + wpre;
+        bool pstart = false;
+        int ptg = kNoTag;
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const int idx = base + j;
+            if (idx >= pos0) {
+                const double w = mine[j];
+                acc += w;
+                const double s = cur0 + acc;
+                const bool start = (idx == pos0) || may_cross(sprev, s);
+                int tg = kNoTag;
+                if (start) {
+                    if (idx > base && idx > pos0 && r >= 0 && r < kMaxRuns) {
+                        sh.rM[r] = anytie ? macc : steps_map(dacc);
+                        sh.rpe[r] = pstart ? kNoTag : ptg;
+                    }
+                    ++r;
+                    dacc = 0.0;
+                    macc = ident();
+                    if (r <= kMaxRuns)
+                        sh.rf[r] = idx;
+                } else {
+                    tg = sum_tag(s);
+                    if (finite_tag(tg)) {
+                        if (anytie)
+                            macc = compose(macc, elem_map(w, tg));
+                        else
+                            dacc += rint(ldexp(w, 52 - tg));
+                    }
+                }
+                pstart = start;
+                ptg = tg;
+                sprev = s;
+            }
+        }
+        const bool ends = base + kItems - 1 >= pos0 &&
+            (threadIdx.x == kBlock - 1 || sh.first_start[threadIdx.x + 1] != 0);
+        if (ends && r >= 0 && r < kMaxRuns) {
+            sh.rM[r] = anytie ? macc : steps_map(dacc);
+            sh.rpe[r] = pstart ? kNoTag : ptg;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nr = tot.cnt < kMaxRuns ? tot.cnt : kMaxRuns;
+        double x = cur0;
+        int ok = 0;
+        for (; ok < nr; ++ok) {
+            const double s = x + t[lds_pad(sh.rf[ok])];
+            const int pe = sh.rpe[ok];
+            if (pe != kNoTag) {
+                if (sum_tag(s) != pe)
+                    break;
+                if (finite_tag(pe)) {
+                    const long long a0 = mant(s);
+                    const FMap m = sh.rM[ok];
+                    const long long a = a0 + ((a0 & 1) ? m.d1 : m.d0);
+                    if (a >= kTwo53)
+                        break;
+                    sh.rs[ok] = s;
+                    x = from_mant(a, pe);
+                    continue;
+                }
+            }
+            sh.rs[ok] = s;
+            x = s;
+        }
+        sh.pos = ok < tot.cnt ? sh.rf[ok] : kTile;
+        sh.cur = x;
+        sh.red[0] = ok;
+    }
+    __syncthreads();
+    const int nok = sh.red[0];
+    // pass 3: the sums of the verified runs
+    {
+        int r = carry.cnt - 1;
+        double dacc = carry.d;
+        FMap macc = mcarry.m;
+        double acc = wpre;
+        double sprev = cur0 + wpre;
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const int idx = base + j;
+            if (idx >= pos0) {
+                const double w = mine[j];
+                acc += w;
+                const double s = cur0 + acc;
+                const bool start = (idx == pos0) || may_cross(sprev, s);
+                sprev = s;
+                if (start) {
+                    ++r;
+                    dacc = 0.0;
+                    macc = ident();
+                    if (r < nok)
+                        mine[j] = sh.rs[r];
+                } else if (r < nok) {
+                    const int tg = sum_tag(s);
+                    const double x = sh.rs[r];
+                    if (finite_tag(tg)) {
+                        const long long a0 = mant(x);
+                        double a;
+                        if (anytie) {
+                            macc = compose(macc, elem_map(w, tg));
+                            a = static_cast<double>(a0 + ((a0 & 1) ? macc.d1 : macc.d0));
+                        } else {
+                            dacc += rint(ldexp(w, 52 - tg));
+                            a = static_cast<double>(a0) + dacc;
+                        }
+                        mine[j] = ldexp(a, tg - 52);
+                    } else {
+                        mine[j] = x;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// The common tile: every sum stays in the binade e of the exact start x0
+// and no step is a half-ulp tie. Then s_k = x0 advanced by the prefix sum
+// of the rounded steps rint(w / 2^(e-52)), and whether that holds is
+// known from the total (the sums only grow). Writes the tile and returns
+// true, or writes nothing and returns false. All threads call it.
+__device__ __forceinline__ bool tile_single(double *t, double x0, ExactShared &sh)
+{
+    const int e = sum_tag(x0);
+    if (!finite_tag(e))
+        return false;
+    const int base = threadIdx.x * kItems;
+    double *const mine = t + lds_pad(base);
+    double dsum = 0.0;
+    bool tie = false;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const double q = ldexp(mine[j], 52 - e);
+        tie = tie || (q - trunc(q) == 0.5);
+        dsum += rint(q);
+    }
+    double dtot;
+    double run = block_exclusive_scan(dsum, OpAddD(), 0.0, sh.dred, &dtot);
+    const double a0d = static_cast<double>(mant(x0));
+    if (__syncthreads_or(tie) || !(a0d + dtot < static_cast<double>(kTwo53)))
+        return false;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        run += rint(ldexp(mine[j], 52 - e));
+        mine[j] = ldexp(a0d + run, e - 52);
+    }
+    __syncthreads();
+    return true;
+}
+
+// Element-by-element exact fold of one tile from element pos, whose
+// preceding exact sum is cur (the elements before pos hold their sums).
+// All threads of the block must call it (it synchronises, also before
+// returning).
+__device__ __forceinline__ void tile_exact_from(double *t, int pos, double cur,
+    ExactShared &sh)
+{
+    const int base = threadIdx.x * kItems;
+    double *const mine = t + lds_pad(base);
     while (pos < kTile) {
         if (!(cur < INFINITY)) {
 #pragma unroll
@@ -260,20 +578,29 @@ __device__ __forceinline__ void tile_exact(double *t, double x0, ExactShared &sh
     __syncthreads();
 }
 
-// Segmented composition of (flag, map) pairs: a set flag starts a run.
-struct SegPair {
-    long long flag;
-    FMap m;
-};
-struct OpSeg {
-    __device__ SegPair operator()(SegPair a, SegPair b) const
-    {
-        SegPair r;
-        r.flag = a.flag | b.flag;
-        r.m = b.flag ? b.m : compose(a.m, b.m);
-        return r;
+// Exact inclusive fold of one tile, in place: t (element k at lds_pad(k))
+// holds the weights on entry and s_k on return, from the exact start x0.
+// Multi-run passes fold the tile (another pass continues after
+// kMaxRuns runs or a failed prediction); the element-by-element fold
+// finishes from a run whose prediction fails twice. All threads
+// of the block must call it (it synchronises, also before returning).
+__device__ __forceinline__ void tile_exact(double *t, double x0, ExactShared &sh)
+{
+    if (tile_single(t, x0, sh))
+        return;
+    int pos = 0;
+    double cur = x0;
+    while (pos < kTile) {
+        const int p0 = pos;
+        tile_runs(t, p0, cur, sh);
+        pos = sh.pos;
+        cur = sh.cur;
+        if (pos == p0) {
+            tile_exact_from(t, pos, cur, sh);
+            break;
+        }
     }
-};
+}
 
 struct SegShared {
     double tile[kTileLds];

@@ -250,25 +250,16 @@ __global__ void __launch_bounds__(kPB) k_prepare(PDev *pd, ResampleDev *prm,
     const double lmax = pd->lmax;
     const double inv = equal ? 1.0 : 1 / pd->sum;
     const double inv_n = 1 / static_cast<double>(n);
-    double carry = 0.0;
-    for (long long c = 0; c < nb; c += kPB) {
-        const long long b = c + threadIdx.x;
-        double v = 0.0;
-        if (b < nb) {
+    block_scan_chunked(nb,
+        [&](long long b) {
             if (equal) {
                 const long long rem = n - b * kPTile;
-                v = static_cast<double>(rem < kPTile ? rem : kPTile) * inv_n;
-            } else {
-                const Part3 q = part[b];
-                v = q.s > 0.0 ? q.s * exp(q.m - lmax) * inv : 0.0;
+                return static_cast<double>(rem < kPTile ? rem : kPTile) * inv_n;
             }
-        }
-        double tot;
-        const double ex = block_exclusive_scan(v, OpAddD(), 0.0, lds, &tot);
-        if (b < nb)
-            tpre[b] = carry + ex;
-        carry += tot;
-    }
+            const Part3 q = part[b];
+            return q.s > 0.0 ? q.s * exp(q.m - lmax) * inv : 0.0;
+        },
+        [&](long long b, double ex) { tpre[b] = ex; }, OpAddD(), 0.0, lds);
 }
 
 // After the event: equal weights, advance the resampling stream. With
@@ -946,6 +937,10 @@ int particle_create(ParticleSet **out, long long n, int dim, int bytes,
 {
     if (n < 1 || dim < 1 || dim > 64 || (bytes != 4 && bytes != 8)) {
         set_error("particle_create: invalid configuration");
+        return kErrInvalid;
+    }
+    if (n > 2147483647LL) {
+        set_error("particle_create: at most 2^31 - 1 particles (indices are int32)");
         return kErrInvalid;
     }
     ParticleSet *p = new ParticleSet();

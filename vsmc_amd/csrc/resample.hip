@@ -105,25 +105,31 @@ struct USys {
     {
         return __dadd_rn(p->u0, __dmul_rn(static_cast<double>(j), p->delta));
     }
+    __device__ double ui(int j) const
+    {
+        return __dadd_rn(p->u0, __dmul_rn(static_cast<double>(j), p->delta));
+    }
     __device__ long long ndraw() const { return p->ndraw; }
+    // Number of draws u_j <= x. Draw counts are below 2^31 (particle
+    // indices are int32), so the search runs on 32-bit integers.
     __device__ long long count(double x) const
     {
-        const long long n = p->ndraw;
+        const int n = static_cast<int>(p->ndraw);
         if (n <= 0)
             return 0;
         if (!(x < INFINITY))
             return n;
         const double est = (x - p->u0) * static_cast<double>(n);
-        long long g;
+        int g;
         if (!(est > 0.0))
             g = 0;
         else if (est >= static_cast<double>(n))
             g = n;
         else
-            g = static_cast<long long>(est) + 1;
-        while (g > 0 && u(g - 1) > x)
+            g = static_cast<int>(est) + 1;
+        while (g > 0 && ui(g - 1) > x)
             --g;
-        while (g < n && u(g) <= x)
+        while (g < n && ui(g) <= x)
             ++g;
         return g;
     }
@@ -148,25 +154,41 @@ struct UStrat {
         const double b = u(j - 1);
         return b > a ? b : a;
     }
+    __device__ double ui(int j) const
+    {
+        const double U = u01_u32d(philox_word(p->key,
+            p->word0 + static_cast<unsigned long long>(j)));
+        return __dadd_rn(__dmul_rn(U, p->delta),
+            __dmul_rn(static_cast<double>(j), p->delta));
+    }
+    __device__ double vi(int j) const
+    {
+        const double a = ui(j);
+        if (j == 0)
+            return a;
+        const double b = ui(j - 1);
+        return b > a ? b : a;
+    }
     __device__ long long ndraw() const { return p->ndraw; }
+    // Number of draws v_j <= x (32-bit search, as for USys).
     __device__ long long count(double x) const
     {
-        const long long n = p->ndraw;
+        const int n = static_cast<int>(p->ndraw);
         if (n <= 0)
             return 0;
         if (!(x < INFINITY))
             return n;
         const double est = x * static_cast<double>(n);
-        long long g;
+        int g;
         if (!(est > 0.0))
             g = 0;
         else if (est >= static_cast<double>(n))
             g = n;
         else
-            g = static_cast<long long>(est);
-        while (g > 0 && v(g - 1) > x)
+            g = static_cast<int>(est);
+        while (g > 0 && vi(g - 1) > x)
             --g;
-        while (g < n && v(g) <= x)
+        while (g < n && vi(g) <= x)
             ++g;
         return g;
     }
@@ -308,18 +330,71 @@ __global__ void __launch_bounds__(kWave) k_groups(long long nh,
     }
 }
 
+constexpr int kRGroups = kGroup;   // group summaries held per window
+constexpr int kRSlots = 16;        // non-flat groups whose tile summaries are held
+constexpr int kRSegs = 16;         // crossing tiles whose full summaries are held
+
+struct TileSeg1 {
+    int nc;
+    int e;
+    FMap m;
+};
+
 struct ResolveShared {
     double tile[kTileLds];
     fold::ExactShared ex;
-    // single-run summaries of the group being walked, and one full summary
-    int gnc[kGroup];
-    int ge[kGroup];
-    FMap gmap[kGroup];
-    fold::TileSeg one;
-    double bx;
-    int bL;
-    int bok;
+    long long lscan[kWaves];
+    // the window's group summaries; gslot: slot of a non-flat group's tile
+    // summaries (kRSlots: staged when the walk reaches it)
+    int gflat[kRGroups];
+    int ge[kRGroups];
+    FMap gmap[kRGroups];
+    int gslot[kRGroups];
+    int slotg[kRSlots];
+    // single-run tile summaries per slot (slot kRSlots stages one group);
+    // tseg: a crossing tile's full summary in seg[] (kRSegs: staged)
+    int tnc[kRSlots + 1][kGroup];
+    int te[kRSlots + 1][kGroup];
+    FMap tmap[kRSlots + 1][kGroup];
+    int tseg[kRSlots][kGroup];
+    // segments (consecutive advanceable items in one binade): start flag and
+    // the composed map from the segment's first item through each item
+    int gst[kRGroups];
+    FMap gP[kRGroups];
+    int tst[kRSlots + 1][kGroup];
+    FMap tP[kRSlots + 1][kGroup];
+    long long segt[kRSegs];
+    TileSeg seg[kRSegs + 1];
+    // walk state
+    double x;
+    long long g;
+    long long t;     // tile inside the non-flat group g; -1 at group level
+    int status;      // 1: tile t needs the exact fold
 };
+
+// Orders LDS writes of this wave before its later LDS reads.
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Segment start flags and prefix maps of 64 items for one wave (lane k:
+// item k): an item is advanceable when `ok`; a segment is a run of
+// advanceable items in one binade, each other item a segment of its own.
+__device__ inline void wave_segments(bool valid, bool ok, int e, FMap m,
+    int *st, FMap *P)
+{
+    const int lane = threadIdx.x % kWave;
+    ok = valid && ok;
+    const int eprev = __shfl_up(e, 1, kWave);
+    const int okprev = __shfl_up(ok ? 1 : 0, 1, kWave);
+    const bool start = lane == 0 || !ok || !okprev || e != eprev;
+    const fold::SegPair sp = wave_inclusive_scan(
+        fold::SegPair{start ? 1LL : 0LL, ok ? m : fold::ident()}, fold::OpSeg());
+    st[lane] = start ? 1 : 0;
+    P[lane] = sp.m;
+}
 
 // Batched advance over up to 64 items (groups or tiles) whose summaries are
 // single maps in binade binade(x): returns how many were applied, writes the
@@ -368,6 +443,275 @@ __device__ inline int wave_advance(double x, long long first, long long count,
     return L2;
 }
 
+// wave_advance from the first item of a segment, with the segment's prefix
+// maps precomputed: item(it, ok, e, st, P). Same results as wave_advance.
+// Called by wave 0 only.
+template <class Item>
+__device__ inline int wave_advance_seg(double x, long long first,
+    long long count, Item item, double *starts, int *modes, double *xnew)
+{
+    const int lane = threadIdx.x;
+    const bool xpos = x > 0.0 && x < INFINITY;
+    const int ex = xpos ? fold::binade(x) : fold::kEmpty;
+    const long long it = first + lane;
+    bool ok = false;
+    FMap P = fold::ident();
+    if (xpos && lane < count) {
+        bool iok, st;
+        int e;
+        item(it, iok, e, st, P);
+        ok = iok && e == ex && (lane == 0 || !st);
+    }
+    const unsigned long long bal = __ballot(ok);
+    const int L = (~bal == 0ull) ? kWave : (__ffsll(~bal) - 1);
+    const long long a0 = xpos ? fold::mant(x) : 0;
+    const long long a = a0 + ((a0 & 1) ? P.d1 : P.d0);
+    const unsigned long long bo = __ballot(lane < L && a >= fold::kTwo53);
+    int L2 = L;
+    if (bo != 0ull) {
+        const int f = __ffsll(bo) - 1;
+        L2 = f < L2 ? f : L2;
+    }
+    const double val = (lane < L2) ? fold::from_mant(a, ex) : 0.0;
+    double prevv = shfl_up_any(val, 1);
+    if (lane == 0)
+        prevv = x;
+    if (lane < L2) {
+        starts[it] = prevv;
+        if (modes)
+            modes[it] = 1;
+    }
+    const double last = shfl_any(val, L2 > 0 ? L2 - 1 : 0);
+    *xnew = L2 > 0 ? last : x;
+    return L2;
+}
+
+// Stages the summaries of groups [gw0, gw0 + nwg) into LDS: every group's
+// (flat, binade, map), the single-run tile summaries of the first kRSlots
+// non-flat groups and the full summaries of the first kRSegs crossing tiles
+// among them. All threads call it; it ends with a barrier.
+__device__ __forceinline__ void resolve_stage(ResolveShared &sh,
+    const TileSeg *segs, const GroupSeg *groups, long long gw0, int nwg,
+    long long ntiles)
+{
+    const int i = threadIdx.x;
+    long long nonflat = 0;
+    {
+        GroupSeg gs{0, fold::kEmpty, fold::ident()};
+        if (i < nwg) {
+            gs = groups[gw0 + i];
+            sh.gflat[i] = gs.flat;
+            sh.ge[i] = gs.e;
+            sh.gmap[i] = gs.map;
+            nonflat = gs.flat ? 0 : 1;
+        }
+        if (i < kWave)
+            wave_segments(i < nwg, gs.flat != 0, gs.e, gs.map, sh.gst, sh.gP);
+    }
+    long long nslot;
+    const long long s = block_exclusive_scan(nonflat, OpAddL(), 0LL, sh.lscan,
+        &nslot);
+    if (i < nwg) {
+        const bool held = nonflat && s < kRSlots;
+        sh.gslot[i] = held ? static_cast<int>(s) : kRSlots;
+        if (held)
+            sh.slotg[s] = i;
+    }
+    __syncthreads();
+    const int ns = nslot < kRSlots ? static_cast<int>(nslot) : kRSlots;
+    constexpr int kPer = kRSlots * kGroup / kBlock;
+    static_assert(kPer * kBlock == kRSlots * kGroup, "slot entries per thread");
+    int nc[kPer];
+    long long cnt = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int q = i * kPer + k;
+        const int sl = q / kGroup, lane = q % kGroup;
+        nc[k] = 0;
+        if (sl < ns) {
+            const long long tile = (gw0 + sh.slotg[sl]) * kGroup + lane;
+            if (tile < ntiles) {
+                const TileSeg &sg = segs[tile];
+                nc[k] = sg.ncross;
+                sh.tnc[sl][lane] = sg.ncross;
+                sh.te[sl][lane] = sg.e[0];
+                sh.tmap[sl][lane] = sg.map[0];
+                cnt += nc[k] > 0;
+            }
+        }
+    }
+    long long nseg;
+    long long si = block_exclusive_scan(cnt, OpAddL(), 0LL, sh.lscan, &nseg);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int q = i * kPer + k;
+        const int sl = q / kGroup, lane = q % kGroup;
+        if (sl < ns) {
+            int idx = kRSegs;
+            if (nc[k] > 0) {
+                if (si < kRSegs) {
+                    idx = static_cast<int>(si);
+                    sh.segt[si] = (gw0 + sh.slotg[sl]) * kGroup + lane;
+                }
+                ++si;
+            }
+            sh.tseg[sl][lane] = idx;
+        }
+    }
+    __syncthreads();
+    for (int sl = i / kWave; sl < ns; sl += kWaves) {
+        const int lane = i % kWave;
+        const long long tile = (gw0 + sh.slotg[sl]) * kGroup + lane;
+        const bool valid = tile < ntiles;
+        wave_segments(valid, valid && sh.tnc[sl][lane] == 0,
+            valid ? sh.te[sl][lane] : 0, valid ? sh.tmap[sl][lane] : fold::ident(),
+            sh.tst[sl], sh.tP[sl]);
+    }
+    const int nsg = nseg < kRSegs ? static_cast<int>(nseg) : kRSegs;
+    constexpr int kWords = static_cast<int>(sizeof(TileSeg) / 4);
+    for (int q = i; q < nsg * kWords; q += kBlock) {
+        const int k = q / kWords, w = q % kWords;
+        reinterpret_cast<int *>(&sh.seg[k])[w] =
+            reinterpret_cast<const int *>(&segs[sh.segt[k]])[w];
+    }
+    __syncthreads();
+}
+
+// Wave 0 walks from the state in sh (x, g, t) over the groups up to gend,
+// with every summary read from LDS: flat groups and single-run tiles 64 at
+// a time, crossing tiles through their full summary. Stops at the end of
+// the window (status 0) or at a tile whose summary does not apply
+// (status 1); the state goes back to sh.
+__device__ __forceinline__ void resolve_walk(ResolveShared &sh,
+    const TileSeg *segs, long long gw0, long long gend, long long ntiles,
+    double *xstart, double *xgstart, int *gmode)
+{
+    constexpr int kWords = static_cast<int>(sizeof(TileSeg) / 4);
+    const int lane = threadIdx.x;
+    double x = sh.x;
+    long long g = sh.g, t = sh.t;
+    int status = 0;
+    while (true) {
+        if (t < 0) {
+            if (g >= gend)
+                break;
+            double xn;
+            int L;
+            if (sh.gst[g - gw0]) {
+                auto item = [&](long long gg, bool &ok, int &e, bool &st, FMap &P) {
+                    const int k = static_cast<int>(gg - gw0);
+                    ok = sh.gflat[k] != 0;
+                    e = sh.ge[k];
+                    st = sh.gst[k] != 0;
+                    P = sh.gP[k];
+                };
+                L = wave_advance_seg(x, g, gend - g, item, xgstart, gmode, &xn);
+            } else {
+                auto get = [&](long long gg, int &e, FMap &m) {
+                    const int k = static_cast<int>(gg - gw0);
+                    e = sh.ge[k];
+                    m = sh.gmap[k];
+                    return sh.gflat[k] != 0;
+                };
+                L = wave_advance(x, g, gend - g, get, xgstart, gmode, &xn);
+            }
+            x = xn;
+            if (L > 0) {
+                g += L;
+                continue;
+            }
+            // group g walked tile by tile
+            if (lane == 0) {
+                gmode[g] = 0;
+                xgstart[g] = x;
+            }
+            t = g * kGroup;
+            if (sh.gslot[g - gw0] == kRSlots) {
+                const long long tile = t + lane;
+                const bool valid = tile < ntiles;
+                TileSeg1 one{0, 0, fold::ident()};
+                if (valid) {
+                    const TileSeg &sg = segs[tile];
+                    one = TileSeg1{sg.ncross, sg.e[0], sg.map[0]};
+                    sh.tnc[kRSlots][lane] = one.nc;
+                    sh.te[kRSlots][lane] = one.e;
+                    sh.tmap[kRSlots][lane] = one.m;
+                }
+                wave_segments(valid, one.nc == 0, one.e, one.m, sh.tst[kRSlots],
+                    sh.tP[kRSlots]);
+                wave_lds_sync();
+            }
+        }
+        const long long t0 = g * kGroup;
+        const long long tend = (t0 + kGroup < ntiles) ? t0 + kGroup : ntiles;
+        if (t >= tend) {
+            ++g;
+            t = -1;
+            continue;
+        }
+        const int sl = sh.gslot[g - gw0];
+        double xn;
+        int L2;
+        if (sh.tst[sl][t - t0]) {
+            auto item = [&](long long tt, bool &ok, int &e, bool &st, FMap &P) {
+                const int k = static_cast<int>(tt - t0);
+                ok = sh.tnc[sl][k] == 0;
+                e = sh.te[sl][k];
+                st = sh.tst[sl][k] != 0;
+                P = sh.tP[sl][k];
+            };
+            L2 = wave_advance_seg(x, t, tend - t, item, xstart,
+                static_cast<int *>(nullptr), &xn);
+        } else {
+            auto get = [&](long long tt, int &e, FMap &m) {
+                const int k = static_cast<int>(tt - t0);
+                e = sh.te[sl][k];
+                m = sh.tmap[sl][k];
+                return sh.tnc[sl][k] == 0;
+            };
+            L2 = wave_advance(x, t, tend - t, get, xstart,
+                static_cast<int *>(nullptr), &xn);
+        }
+        x = xn;
+        if (L2 > 0) {
+            t += L2;
+            continue;
+        }
+        if (lane == 0)
+            xstart[t] = x;
+        const int k = static_cast<int>(t - t0);
+        if (sh.tnc[sl][k] > 0) {
+            const int idx = sl < kRSlots ? sh.tseg[sl][k] : kRSegs;
+            if (idx == kRSegs) {
+                const int *from = reinterpret_cast<const int *>(&segs[t]);
+                int *to = reinterpret_cast<int *>(&sh.seg[kRSegs]);
+                for (int q = lane; q < kWords; q += kWave)
+                    to[q] = from[q];
+                wave_lds_sync();
+            }
+            double y = x;
+            if (fold::apply_segments(sh.seg[idx], y)) {
+                x = y;
+                ++t;
+                continue;
+            }
+        }
+        status = 1;
+        break;
+    }
+    if (lane == 0) {
+        sh.x = x;
+        sh.g = g;
+        sh.t = t;
+        sh.status = status;
+    }
+}
+
+// Exact start of every tile (xstart) and of every group (xgstart, gmode 1
+// where the group was advanced whole and k_starts fills its tiles). One
+// workgroup: per window of kRGroups groups, the summaries are staged in LDS,
+// wave 0 walks them, and the whole block evaluates a tile exactly where a
+// prediction fails.
 template <class WSrc>
 __global__ void __launch_bounds__(kBlock) k_resolve(WSrc src, long long nh,
     const long long *ndev, const ResampleDev *prm, const TileSeg *segs,
@@ -379,107 +723,40 @@ __global__ void __launch_bounds__(kBlock) k_resolve(WSrc src, long long nh,
     const long long ntiles = ceil_div(n, kTile);
     const long long ngroups = ceil_div(ntiles, kGroup);
     __shared__ ResolveShared sh;
-    double x = 0.0;
-    long long g = 0;
-    while (g < ngroups) {
-        if (threadIdx.x < kWave) {
-            double xn;
-            auto get = [&](long long gg, int &e, FMap &m) {
-                const GroupSeg gs = groups[gg];
-                e = gs.e;
-                m = gs.map;
-                return gs.flat != 0;
-            };
-            const int L = wave_advance(x, g, ngroups - g, get, xgstart, gmode,
-                &xn);
-            if (threadIdx.x == 0) {
-                sh.bL = L;
-                sh.bx = xn;
-            }
-        }
-        __syncthreads();
-        const int L = sh.bL;
-        x = sh.bx;
-        __syncthreads();
-        if (L > 0) {
-            g += L;
-            continue;
-        }
-        // group g walked tile by tile
+    if (threadIdx.x == 0)
+        sh.x = 0.0;
+    for (long long gw0 = 0; gw0 < ngroups; gw0 += kRGroups) {
+        const long long gend = (gw0 + kRGroups < ngroups) ? gw0 + kRGroups : ngroups;
+        resolve_stage(sh, segs, groups, gw0, static_cast<int>(gend - gw0), ntiles);
         if (threadIdx.x == 0) {
-            gmode[g] = 0;
-            xgstart[g] = x;
-        }
-        long long t = g * kGroup;
-        const long long t0 = t;
-        const long long tend = (t + kGroup < ntiles) ? t + kGroup : ntiles;
-        // one round trip for the group's single-run summaries
-        if (threadIdx.x < tend - t0) {
-            const TileSeg &sg = segs[t0 + threadIdx.x];
-            sh.gnc[threadIdx.x] = sg.ncross;
-            sh.ge[threadIdx.x] = sg.e[0];
-            sh.gmap[threadIdx.x] = sg.map[0];
+            sh.g = gw0;
+            sh.t = -1;
         }
         __syncthreads();
-        while (t < tend) {
-            if (threadIdx.x < kWave) {
-                double xn;
-                auto get = [&](long long tt, int &e, FMap &m) {
-                    e = sh.ge[tt - t0];
-                    m = sh.gmap[tt - t0];
-                    return sh.gnc[tt - t0] == 0;
-                };
-                const int L2 = wave_advance(x, t, tend - t, get, xstart,
-                    static_cast<int *>(nullptr), &xn);
-                if (threadIdx.x == 0) {
-                    sh.bL = L2;
-                    sh.bx = xn;
-                }
-            }
+        while (true) {
+            if (threadIdx.x < kWave)
+                resolve_walk(sh, segs, gw0, gend, ntiles, xstart, xgstart, gmode);
             __syncthreads();
-            const int L2 = sh.bL;
-            x = sh.bx;
-            __syncthreads();
-            if (L2 > 0) {
-                t += L2;
-                continue;
-            }
-            {
-                // stage the full summary in LDS, then apply it
-                const int words = static_cast<int>(sizeof(TileSeg) / 4);
-                const int *src_w = reinterpret_cast<const int *>(&segs[t]);
-                int *dst_w = reinterpret_cast<int *>(&sh.one);
-                for (int q = threadIdx.x; q < words; q += kBlock)
-                    dst_w[q] = src_w[q];
-            }
+            if (sh.status == 0)
+                break;
+            const long long t = sh.t;
+            const long long base = t * kTile;
+            load_tile_lds(src, base, n, sh.tile, 0.0);
+            fold::tile_exact(sh.tile, sh.x, sh.ex);
+            const long long rem = n - base;
+            const int last = (rem < kTile ? static_cast<int>(rem) : kTile) - 1;
+            const double y = sh.tile[lds_pad(last)];
             __syncthreads();
             if (threadIdx.x == 0) {
-                xstart[t] = x;
-                double y = x;
-                sh.bok = fold::apply_segments(sh.one, y) ? 1 : 0;
-                sh.bx = y;
+                sh.x = y;
+                sh.t = t + 1;
             }
             __syncthreads();
-            const int ok = sh.bok;
-            double y = sh.bx;
-            __syncthreads();
-            if (!ok) {
-                const long long base = t * kTile;
-                load_tile_lds(src, base, n, sh.tile, 0.0);
-                fold::tile_exact(sh.tile, x, sh.ex);
-                const long long rem = n - base;
-                const int last = (rem < kTile ? static_cast<int>(rem) : kTile) - 1;
-                y = sh.tile[lds_pad(last)];
-                __syncthreads();
-            }
-            x = y;
-            ++t;
         }
-        ++g;
     }
     if (threadIdx.x == 0) {
-        xstart[ntiles] = x;
-        xgstart[ngroups] = x;
+        xstart[ntiles] = sh.x;
+        xgstart[ngroups] = sh.x;
     }
 }
 
@@ -782,21 +1059,14 @@ __device__ __forceinline__ void scan_tiles_block(const long long *tagg, long lon
     long long *toff)
 {
     __shared__ L3 lds[kWaves];
-    L3 carry{0, 0, 0};
-    for (long long c0 = 0; c0 < ntiles; c0 += kBlock) {
-        const long long t = c0 + threadIdx.x;
-        L3 v{0, 0, 0};
-        if (t < ntiles)
-            v = L3{tagg[3 * t], tagg[3 * t + 1], tagg[3 * t + 2]};
-        L3 tot;
-        const L3 ex = block_exclusive_scan(v, OpAddL3(), L3{0, 0, 0}, lds, &tot);
-        if (t < ntiles) {
-            toff[3 * t + 0] = carry.a + ex.a;
-            toff[3 * t + 1] = carry.b + ex.b;
-            toff[3 * t + 2] = carry.c + ex.c;
-        }
-        carry = OpAddL3()(carry, tot);
-    }
+    const L3 carry = block_scan_chunked(ntiles,
+        [&](long long t) { return L3{tagg[3 * t], tagg[3 * t + 1], tagg[3 * t + 2]}; },
+        [&](long long t, L3 ex) {
+            toff[3 * t + 0] = ex.a;
+            toff[3 * t + 1] = ex.b;
+            toff[3 * t + 2] = ex.c;
+        },
+        OpAddL3(), L3{0, 0, 0}, lds);
     if (threadIdx.x == 0) {
         toff[3 * ntiles + 0] = carry.a;
         toff[3 * ntiles + 1] = carry.b;
