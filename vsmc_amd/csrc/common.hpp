@@ -165,15 +165,6 @@ __device__ inline T block_reduce(T v, Op op, T *lds)
     return r;
 }
 
-// x through an empty asm statement: values the compiler derived from x
-// before this point cannot be reused after it, so a second pass that
-// recomputes them keeps no array of intermediates live in between.
-__device__ inline double opaque(double x)
-{
-    asm volatile("" : "+v"(x));
-    return x;
-}
-
 struct OpAddD {
     __device__ double operator()(double a, double b) const { return a + b; }
 };
@@ -192,6 +183,9 @@ struct OpMinL {
 struct OpMinI {
     __device__ int operator()(int a, int b) const { return a < b ? a : b; }
 };
+struct OpMaxI {
+    __device__ int operator()(int a, int b) const { return a > b ? a : b; }
+};
 struct OpMaxD {
     __device__ double operator()(double a, double b) const
     {
@@ -207,17 +201,25 @@ struct OpMaxD {
 __device__ inline int lds_pad(int e) { return e + (e / kItems); }
 constexpr int kTileLds = kTile + kTile / kItems;
 
-// Stage the tile into LDS only (element k at lds_pad(k)), `pad` past n.
+// Stage the tile into LDS only (element k at lds_pad(k)), `pad` past n
+// (base < n). The loads are unconditional (index clamped to n - 1) and all
+// issued before the first LDS store, so they are in flight together: a
+// branch or an interleaved store per element would serialise them.
 template <typename Src>
 __device__ __forceinline__ void load_tile_lds(Src src, long long base, long long n,
     double *lds_tile, double pad)
 {
+    const auto s = src.loaded();
+    double v[kItems];
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
-        const int e = j * kBlock + threadIdx.x;
-        const long long g = base + e;
-        lds_tile[lds_pad(e)] = g < n ? src(g) : pad;
+        const long long g = base + j * kBlock + threadIdx.x;
+        const double x = s(g < n ? g : n - 1);
+        v[j] = g < n ? x : pad;
     }
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+        lds_tile[lds_pad(j * kBlock + threadIdx.x)] = v[j];
     __syncthreads();
 }
 

@@ -231,7 +231,7 @@ __device__ __forceinline__ void tile_runs(double *t, int pos0, double cur0,
     const int base = threadIdx.x * kItems;
     double *const mine = t + lds_pad(base);
     double wsum = 0.0;
-#pragma unroll
+#pragma unroll 1
     for (int j = 0; j < kItems; ++j)
         if (base + j >= pos0)
             wsum += mine[j];
@@ -243,7 +243,7 @@ __device__ __forceinline__ void tile_runs(double *t, int pos0, double cur0,
     {
         double acc = wpre;
         double sprev = cur0 + wpre;
-#pragma unroll
+#pragma unroll 1
         for (int j = 0; j < kItems; ++j) {
             const int idx = base + j;
             bool start = false;
@@ -279,7 +279,7 @@ __device__ __forceinline__ void tile_runs(double *t, int pos0, double cur0,
         SegPair lm{0, ident()};
         double acc = wpre;
         double sprev = cur0 + wpre;
-#pragma unroll
+#pragma unroll 1
         for (int j = 0; j < kItems; ++j) {
             const int idx = base + j;
             if (idx >= pos0) {
@@ -311,7 +311,7 @@ __device__ __forceinline__ void tile_runs(double *t, int pos0, double cur0,
         double sprev = cur0 + wpre;
         bool pstart = false;
         int ptg = kNoTag;
-#pragma unroll
+#pragma unroll 1
         for (int j = 0; j < kItems; ++j) {
             const int idx = base + j;
             if (idx >= pos0) {
@@ -389,7 +389,7 @@ __device__ __forceinline__ void tile_runs(double *t, int pos0, double cur0,
         FMap macc = mcarry.m;
         double acc = wpre;
         double sprev = cur0 + wpre;
-#pragma unroll
+#pragma unroll 1
         for (int j = 0; j < kItems; ++j) {
             const int idx = base + j;
             if (idx >= pos0) {
@@ -433,7 +433,7 @@ __device__ __forceinline__ void tile_runs(double *t, int pos0, double cur0,
 // of the rounded steps rint(w / 2^(e-52)), and whether that holds is
 // known from the total (the sums only grow). Writes the tile and returns
 // true, or writes nothing and returns false. All threads call it.
-__device__ __forceinline__ bool tile_single(double *t, double x0, ExactShared &sh)
+__device__ __forceinline__ bool tile_single(double *t, double x0, double *dred)
 {
     const int e = sum_tag(x0);
     if (!finite_tag(e))
@@ -449,7 +449,7 @@ __device__ __forceinline__ bool tile_single(double *t, double x0, ExactShared &s
         dsum += rint(q);
     }
     double dtot;
-    double run = block_exclusive_scan(dsum, OpAddD(), 0.0, sh.dred, &dtot);
+    double run = block_exclusive_scan(dsum, OpAddD(), 0.0, dred, &dtot);
     const double a0d = static_cast<double>(mant(x0));
     if (__syncthreads_or(tie) || !(a0d + dtot < static_cast<double>(kTwo53)))
         return false;
@@ -473,7 +473,7 @@ __device__ __forceinline__ void tile_exact_from(double *t, int pos, double cur,
     double *const mine = t + lds_pad(base);
     while (pos < kTile) {
         if (!(cur < INFINITY)) {
-#pragma unroll
+#pragma unroll 1
             for (int j = 0; j < kItems; ++j)
                 if (base + j >= pos)
                     mine[j] = cur;
@@ -482,12 +482,12 @@ __device__ __forceinline__ void tile_exact_from(double *t, int pos, double cur,
         if (cur == 0.0) {
             // the sum stays 0 up to the first non-zero weight k; s_k = w_k
             int lf = kTile;
-#pragma unroll
+#pragma unroll 1
             for (int j = 0; j < kItems; ++j)
                 if (base + j >= pos && lf == kTile && mine[j] != 0.0)
                     lf = base + j;
             const int k = block_reduce(lf, OpMinI(), sh.red);
-#pragma unroll
+#pragma unroll 1
             for (int j = 0; j < kItems; ++j) {
                 const int idx = base + j;
                 if (idx >= pos && idx < k)
@@ -512,7 +512,7 @@ __device__ __forceinline__ void tile_exact_from(double *t, int pos, double cur,
         const double two53 = static_cast<double>(kTwo53);
         double dsum = 0.0;
         bool tie = false;
-#pragma unroll
+#pragma unroll 1
         for (int j = 0; j < kItems; ++j) {
             if (base + j >= pos) {
                 const double q = ldexp(mine[j], 52 - e);
@@ -524,7 +524,7 @@ __device__ __forceinline__ void tile_exact_from(double *t, int pos, double cur,
         if (!__syncthreads_or(tie)) {
             double dtot;
             double run = block_exclusive_scan(dsum, OpAddD(), 0.0, sh.dred, &dtot);
-#pragma unroll
+#pragma unroll 1
             for (int j = 0; j < kItems; ++j) {
                 const int idx = base + j;
                 if (idx >= pos) {
@@ -540,14 +540,14 @@ __device__ __forceinline__ void tile_exact_from(double *t, int pos, double cur,
             }
         } else {
             FMap magg = ident();
-#pragma unroll
+#pragma unroll 1
             for (int j = 0; j < kItems; ++j)
                 if (base + j >= pos)
                     magg = compose(magg, elem_map(mine[j], e));
             FMap total;
             FMap run = block_exclusive_scan(magg, OpCompose(), ident(), sh.scan,
                 &total);
-#pragma unroll
+#pragma unroll 1
             for (int j = 0; j < kItems; ++j) {
                 const int idx = base + j;
                 if (idx >= pos) {
@@ -586,7 +586,7 @@ __device__ __forceinline__ void tile_exact_from(double *t, int pos, double cur,
 // of the block must call it (it synchronises, also before returning).
 __device__ __forceinline__ void tile_exact(double *t, double x0, ExactShared &sh)
 {
-    if (tile_single(t, x0, sh))
+    if (tile_single(t, x0, sh.dred))
         return;
     int pos = 0;
     double cur = x0;
@@ -607,43 +607,38 @@ struct SegShared {
     double dscan[kWaves];
     long long lscan[kWaves];
     FMap fscan[kWaves];
-    int fflag[kWaves];
     SegPair pscan[kWaves];
     double tlast[kBlock];
-    unsigned char cross[kTile + 1];
+    unsigned short cross[kBlock];   // crossing flags, bit j of entry t: element t*kItems+j
     TileSeg seg;
 };
 
-// Tile summary from the approximate prefix tpre (start of this tile).
-// w holds the tile in blocked order, nvalid elements valid.
-__device__ __forceinline__ void tile_segments(const double (&w)[kItems], int nvalid,
-    double tpre, TileSeg *out, SegShared &sh)
+__device__ inline int prev_tag(double prev)
+{
+    return (prev > 0.0 && prev < INFINITY) ? binade(prev) : kEmpty;
+}
+
+// Tile summary with crossings (the rare case): the approximate running sums
+// from tpre + ex (ex: this thread's exclusive share of the block scan) mark
+// the elements whose add changes binade; runs between them carry the
+// composition of their elements' maps. Per-element values are recomputed in
+// each pass from the same running sums.
+__device__ __forceinline__ void tile_segments_multi(int nvalid, double tpre,
+    double ex, TileSeg *out, SegShared &sh)
 {
     const int base = threadIdx.x * kItems;
-    double tsum = 0;
-#pragma unroll
-    for (int j = 0; j < kItems; ++j)
-        tsum += w[j];
-    double dtot;
-    const double ex = block_exclusive_scan(tsum, OpAddD(), 0.0, sh.dscan, &dtot);
-    double prev = tpre + ex;
-    // crossing flags and binades of the state before each element
-    bool cr[kItems];
-    int eb[kItems];
-    int ncl = 0;
-#pragma unroll
+    const double *const mine = sh.tile + lds_pad(base);
+    const double start = tpre + ex;
+    unsigned mask = 0;
+    double prev = start;
+#pragma unroll 1
     for (int j = 0; j < kItems; ++j) {
-        const double t = prev + w[j];
-        const bool valid = base + j < nvalid;
-        bool c = false;
-        if (valid) {
-            if (!(prev > 0.0) || !(prev < INFINITY) || !(t < INFINITY))
-                c = true;
-            else
-                c = binade(t) != binade(prev);
+        const double t = prev + mine[j];
+        if (base + j < nvalid) {
+            const bool c = !(prev > 0.0) || !(prev < INFINITY) || !(t < INFINITY) ||
+                binade(t) != binade(prev);
+            mask |= c ? (1u << j) : 0u;
         }
-        cr[j] = c;
-        eb[j] = (prev > 0.0 && prev < INFINITY) ? binade(prev) : kEmpty;
         prev = t;
     }
     // The block scan and the previous thread's running sum approximate the
@@ -652,22 +647,13 @@ __device__ __forceinline__ void tile_segments(const double (&w)[kItems], int nva
     // no run mixes binades (each run's binade is verified at its start).
     sh.tlast[threadIdx.x] = prev;
     __syncthreads();
-    if (threadIdx.x > 0 && base < nvalid && !cr[0]) {
-        const double tl = sh.tlast[threadIdx.x - 1];
-        const int el = (tl > 0.0 && tl < INFINITY) ? binade(tl) : kEmpty;
-        if (el != eb[0])
-            cr[0] = true;
-    }
-#pragma unroll
-    for (int j = 0; j < kItems; ++j) {
-        sh.cross[base + j] = cr[j] ? 1 : 0;
-        ncl += cr[j] ? 1 : 0;
-    }
-    if (threadIdx.x == 0)
-        sh.cross[kTile] = 1;
+    if (threadIdx.x > 0 && base < nvalid && !(mask & 1u) &&
+        prev_tag(sh.tlast[threadIdx.x - 1]) != prev_tag(start))
+        mask |= 1u;
+    sh.cross[threadIdx.x] = static_cast<unsigned short>(mask);
     long long ltot;
-    long long rank = block_exclusive_scan(static_cast<long long>(ncl), OpAddL(),
-        0LL, sh.lscan, &ltot);
+    long long rank = block_exclusive_scan(static_cast<long long>(__popc(mask)),
+        OpAddL(), 0LL, sh.lscan, &ltot);
     if (threadIdx.x == 0) {
         sh.seg.ncross = ltot > kSegMax ? -1 : static_cast<int>(ltot);
         for (int r = 0; r <= kSegMax; ++r) {
@@ -678,83 +664,113 @@ __device__ __forceinline__ void tile_segments(const double (&w)[kItems], int nva
             sh.seg.cross[r] = 0.0;
     }
     __syncthreads();
-    if (ltot == 0) {
-        // one run in one binade (every prev > 0 and finite); without a
-        // half-ulp tie its map is the sum of the rounded steps, exact in
-        // FP64 below 2^53 (larger or non-finite sums saturate: the resolver
-        // then sees the crossing)
-        double dsum = 0.0;
-        bool tie = false;
-#pragma unroll
-        for (int j = 0; j < kItems; ++j) {
-            if (base + j < nvalid) {
-                const double q = ldexp(w[j], 52 - eb[j]);
-                tie = tie || (q - trunc(q) == 0.5);
-                dsum += rint(q);
-            }
-        }
-        if (!__syncthreads_or(tie)) {
-            const double dd = block_reduce(dsum, OpAddD(), sh.dscan);
-            const long long d = dd < static_cast<double>(kCap) ?
-                static_cast<long long>(dd) : kCap;
-            if (threadIdx.x == 0) {
-                sh.seg.e[0] = eb[0];
-                sh.seg.map[0] = FMap{d, d};
-            }
-            __syncthreads();
-            const int words = static_cast<int>(sizeof(TileSeg) / 4);
-            const int *src = reinterpret_cast<const int *>(&sh.seg);
-            int *dst = reinterpret_cast<int *>(out);
-            for (int i = threadIdx.x; i < words; i += kBlock)
-                dst[i] = src[i];
-            return;
-        }
-    }
+    auto cross_at = [&](int k) {
+        return k >= nvalid || ((sh.cross[k / kItems] >> (k % kItems)) & 1u) != 0;
+    };
     // segmented scan: a run starts at element 0, at a crossing, and right
     // after a crossing; crossing elements carry the identity
-    SegPair agg{0, ident()};
-    SegPair pj[kItems];
-#pragma unroll
-    for (int j = 0; j < kItems; ++j) {
+    auto pair_at = [&](int j, double pv) {
         const int idx = base + j;
-        const bool prevc = idx == 0 ? true : (sh.cross[idx - 1] != 0);
+        const bool cr = ((mask >> j) & 1u) != 0;
+        const bool prevc = idx == 0 ? true : cross_at(idx - 1);
+        const int eb = prev_tag(pv);
         SegPair p;
-        p.flag = (prevc || cr[j]) ? 1 : 0;
-        p.m = (cr[j] || idx >= nvalid || eb[j] == kEmpty) ? ident() :
-                                                            elem_map(w[j], eb[j]);
-        pj[j] = p;
-        agg = OpSeg()(agg, p);
+        p.flag = (prevc || cr) ? 1 : 0;
+        p.m = (cr || idx >= nvalid || eb == kEmpty) ? ident() : elem_map(mine[j], eb);
+        return p;
+    };
+    SegPair agg{0, ident()};
+    prev = start;
+#pragma unroll 1
+    for (int j = 0; j < kItems; ++j) {
+        agg = OpSeg()(agg, pair_at(j, prev));
+        prev += mine[j];
     }
     SegPair stot;
     SegPair run = block_exclusive_scan(agg, OpSeg(), SegPair{0, ident()},
         sh.pscan, &stot);
     if (ltot <= kSegMax) {
-#pragma unroll
+        prev = start;
+#pragma unroll 1
         for (int j = 0; j < kItems; ++j) {
             const int idx = base + j;
-            run = OpSeg()(run, pj[j]);
+            run = OpSeg()(run, pair_at(j, prev));
             if (idx < nvalid) {
-                if (cr[j]) {
-                    sh.seg.cross[rank] = w[j];
+                if ((mask >> j) & 1u) {
+                    sh.seg.cross[rank] = mine[j];
                     ++rank;
-                } else {
-                    const bool last = (idx + 1 >= nvalid) || sh.cross[idx + 1];
-                    if (last) {
-                        sh.seg.e[rank] = eb[j];
-                        sh.seg.map[rank] = run.m;
-                    }
+                } else if (cross_at(idx + 1)) {
+                    sh.seg.e[rank] = prev_tag(prev);
+                    sh.seg.map[rank] = run.m;
                 }
             }
+            prev += mine[j];
         }
     }
     __syncthreads();
-    // copy the summary out
     const int words = static_cast<int>(sizeof(TileSeg) / 4);
     const int *src = reinterpret_cast<const int *>(&sh.seg);
     int *dst = reinterpret_cast<int *>(out);
     for (int i = threadIdx.x; i < words; i += kBlock)
         dst[i] = src[i];
     __syncthreads();
+}
+
+// Tile summary from the approximate prefix tpre (start of this tile); the
+// tile is in sh.tile (element k at lds_pad(k)), nvalid elements valid.
+// Common case: the approximate start and end of the tile lie in one binade,
+// so the summary is one run whose map is the composition of the elements'
+// maps (the prefix sum of rounded steps when no half-ulp tie occurs). Only
+// ncross, e[0] and map[0] are written then; the resolver verifies every
+// prediction against the exact sums.
+__device__ __forceinline__ void tile_segments(int nvalid, double tpre,
+    TileSeg *out, SegShared &sh)
+{
+    const int base = threadIdx.x * kItems;
+    const double *const mine = sh.tile + lds_pad(base);
+    double tsum = 0.0;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+        tsum += mine[j];
+    double dtot;
+    const double ex = block_exclusive_scan(tsum, OpAddD(), 0.0, sh.dscan, &dtot);
+    const double end = tpre + dtot;
+    if (!(tpre > 0.0) || !(end < INFINITY) || binade(tpre) != binade(end)) {
+        tile_segments_multi(nvalid, tpre, ex, out, sh);
+        return;
+    }
+    const int e = binade(tpre);
+    double dsum = 0.0;
+    bool tie = false;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        if (base + j < nvalid) {
+            const double q = ldexp(mine[j], 52 - e);
+            tie = tie || (q - trunc(q) == 0.5);
+            dsum += rint(q);
+        }
+    }
+    FMap m;
+    if (!__syncthreads_or(tie)) {
+        // rounded steps are integers, exact in FP64 below 2^53 (larger or
+        // non-finite sums saturate: the resolver then sees the crossing)
+        double dd;
+        block_exclusive_scan(dsum, OpAddD(), 0.0, sh.dscan, &dd);
+        const long long d = dd < static_cast<double>(kCap) ? static_cast<long long>(dd) : kCap;
+        m = FMap{d, d};
+    } else {
+        FMap agg = ident();
+#pragma unroll 1
+        for (int j = 0; j < kItems; ++j)
+            if (base + j < nvalid)
+                agg = compose(agg, elem_map(mine[j], e));
+        block_exclusive_scan(agg, OpCompose(), ident(), sh.fscan, &m);
+    }
+    if (threadIdx.x == 0) {
+        out->ncross = 0;
+        out->e[0] = e;
+        out->map[0] = m;
+    }
 }
 
 } // namespace fold

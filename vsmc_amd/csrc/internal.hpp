@@ -65,6 +65,7 @@ struct Workspace {
     double *v = nullptr;         // materialised uniforms (running max)
     double *aux = nullptr;       // scratch N doubles
     int *rep = nullptr;          // scratch counts
+    int *hard = nullptr;         // [0]: count, then tiles left to the full fold
     int *integ_dummy = nullptr;
     ResampleDev *prm = nullptr;
     long long *scal = nullptr;   // small device scalars

@@ -30,6 +30,24 @@ struct WArray {
     const double *w;
     __device__ double operator()(long long i) const { return w[i]; }
     __device__ long long integ(long long) const { return 0; }
+    // the source with its event parameters in registers (see WLog)
+    __device__ WArray loaded() const { return *this; }
+};
+
+// WLog with its parameters read once: no per-element parameter loads or
+// branches, so a tile's loads can all be in flight at once.
+struct WLogV {
+    const double *l;
+    double inv_sum;
+    double lmax;
+    double eqw;
+    int equal;
+    __device__ double operator()(long long i) const
+    {
+        const double v = inv_sum * exp(l[i] - lmax);
+        return equal ? eqw : v;
+    }
+    __device__ long long integ(long long) const { return 0; }
 };
 
 // W_i = inv_sum * exp(l_i - lmax) (Weight::normalize, weight.hpp:208-222)
@@ -43,6 +61,11 @@ struct WLog {
         return p->inv_sum * exp(l[i] - p->lmax);
     }
     __device__ long long integ(long long) const { return 0; }
+    __device__ WLogV loaded() const
+    {
+        return WLogV{l, p->inv_sum, p->lmax, 1.0 / static_cast<double>(p->M),
+            p->equal};
+    }
 };
 
 // resid_i = modf(N * W_i) (transform.hpp:159-161)
@@ -54,6 +77,32 @@ struct WResidRaw {
     {
         double ip;
         return modf(coeff * b(i), &ip);
+    }
+    __device__ long long integ(long long i) const
+    {
+        double ip;
+        modf(coeff * b(i), &ip);
+        return static_cast<long long>(ip);
+    }
+    __device__ auto loaded() const
+    {
+        using BV = decltype(b.loaded());
+        return WResidRaw<BV>{b.loaded(), coeff};
+    }
+};
+
+// WResid with its parameters read once (see WLogV).
+template <class B>
+struct WResidV {
+    B b;
+    double coeff;
+    double inv_resid;
+    int active;
+    __device__ double operator()(long long i) const
+    {
+        double ip;
+        const double r = modf(coeff * b(i), &ip);
+        return active ? inv_resid * r : 0.0;
     }
     __device__ long long integ(long long i) const
     {
@@ -84,10 +133,26 @@ struct WResid {
         modf(coeff * b(i), &ip);
         return static_cast<long long>(ip);
     }
+    __device__ auto loaded() const
+    {
+        using BV = decltype(b.loaded());
+        return WResidV<BV>{b.loaded(), coeff, p->inv_resid, p->ndraw > 0 ? 1 : 0};
+    }
 };
 
 // Sequence terms of U01SequenceSorted (u01_sequence.hpp:149-151), negated:
 // a_k = -(log(U_k) / (n - k)) >= 0, so the fold of a is -lmax.
+struct WSortedTermsV {
+    unsigned int key;
+    unsigned long long word0;
+    long long ndraw;
+    __device__ double operator()(long long k) const
+    {
+        const double u = u01_u32d(philox_word(key, word0 + k));
+        return -(log(u) / static_cast<double>(ndraw - k));
+    }
+    __device__ long long integ(long long) const { return 0; }
+};
 struct WSortedTerms {
     const ResampleDev *p;
     __device__ double operator()(long long k) const
@@ -96,6 +161,10 @@ struct WSortedTerms {
         return -(log(u) / static_cast<double>(p->ndraw - k));
     }
     __device__ long long integ(long long) const { return 0; }
+    __device__ WSortedTermsV loaded() const
+    {
+        return WSortedTermsV{p->key, p->word0, p->ndraw};
+    }
 };
 
 // U01SequenceSystematic: u_j = u0 + j * delta (u01_sequence.hpp:246), no FMA.
@@ -287,11 +356,10 @@ __global__ void __launch_bounds__(kBlock) k_segments(WSrc src, long long nh,
     if (base >= n)
         return;
     __shared__ fold::SegShared sh;
-    double w[kItems];
-    load_tile_blocked(src, base, n, w, sh.tile, 0.0);
+    load_tile_lds(src, base, n, sh.tile, 0.0);
     const long long rem = n - base;
     const int nvalid = rem < kTile ? static_cast<int>(rem) : kTile;
-    fold::tile_segments(w, nvalid, blockIdx.x == 0 ? 0.0 : tpre[blockIdx.x],
+    fold::tile_segments(nvalid, blockIdx.x == 0 ? 0.0 : tpre[blockIdx.x],
         &segs[blockIdx.x], sh);
 }
 
@@ -715,10 +783,13 @@ __device__ __forceinline__ void resolve_walk(ResolveShared &sh,
 template <class WSrc>
 __global__ void __launch_bounds__(kBlock) k_resolve(WSrc src, long long nh,
     const long long *ndev, const ResampleDev *prm, const TileSeg *segs,
-    const GroupSeg *groups, double *xstart, double *xgstart, int *gmode)
+    const GroupSeg *groups, double *xstart, double *xgstart, int *gmode,
+    int *hard)
 {
     if (!prm->active)
         return;
+    if (threadIdx.x == 0)
+        hard[0] = 0;   // the list k_count fills after this pass
     const long long n = eff_n(ndev, nh);
     const long long ntiles = ceil_div(n, kTile);
     const long long ngroups = ceil_div(ntiles, kGroup);
@@ -795,40 +866,70 @@ struct FoldBufs {
     const double *xstart;   // exact start of every tile, total at [ntiles]
 };
 
-template <class WSrc, class USrc>
+template <bool kFull, class WSrc, class USrc>
 __device__ __forceinline__ double count_tile(WSrc src, USrc us, long long M,
-    long long tidx, double x0, int *rep, long long *tagg);
+    long long tidx, double x0, int *rep, long long *tagg, int *hard);
 
-// Replication counts (and residual integer parts) for one tile.
+// Replication counts (and residual integer parts) for one tile whose sums
+// stay in one binade; any other tile goes to the list `hard`.
 template <class WSrc, class USrc>
 __global__ void __launch_bounds__(kBlock) k_count(WSrc src, USrc us,
     long long nh, const long long *ndev, const ResampleDev *prm, FoldBufs fb,
-    int *rep, long long *tagg)
+    int *rep, long long *tagg, int *hard)
 {
     if (!prm->active)
         return;
     const long long M = eff_n(ndev, nh);
     if (static_cast<long long>(blockIdx.x) * kTile >= M)
         return;
-    count_tile(src, us, M, blockIdx.x, fb.xstart[blockIdx.x], rep, tagg);
+    count_tile<false>(src, us, M, blockIdx.x, fb.xstart[blockIdx.x], rep, tagg,
+        hard);
+}
+
+// The tiles k_count left (hard[1 .. hard[0]]), with the full exact fold.
+template <class WSrc, class USrc>
+__global__ void __launch_bounds__(kBlock) k_count_hard(WSrc src, USrc us,
+    long long nh, const long long *ndev, const ResampleDev *prm, FoldBufs fb,
+    int *rep, long long *tagg, const int *hard)
+{
+    if (!prm->active)
+        return;
+    const long long M = eff_n(ndev, nh);
+    const int nhard = hard[0];
+    for (int q = blockIdx.x; q < nhard; q += gridDim.x) {
+        const long long t = hard[1 + q];
+        count_tile<true>(src, us, M, t, fb.xstart[t], rep, tagg,
+            static_cast<int *>(nullptr));
+    }
 }
 
 // Replication counts (and residual integer parts) of tile `tidx` of M
 // weights from the exact start x0 of the tile; per-tile rep -> index
 // aggregates to tagg[3 tidx ..] when tagg is non-null. Returns the exact
-// sum after the tile's last element (the next tile's start).
-template <class WSrc, class USrc>
+// sum after the tile's last element (the next tile's start). Without kFull
+// only a tile whose sums stay in x0's binade is done; another is appended
+// to `hard` and nothing is written.
+template <bool kFull, class WSrc, class USrc>
 __device__ __forceinline__ double count_tile(WSrc src, USrc us, long long M,
-    long long tidx, double x0, int *rep, long long *tagg)
+    long long tidx, double x0, int *rep, long long *tagg, int *hard)
 {
     const long long base = tidx * kTile;
     __shared__ double tile[kTileLds];
-    __shared__ fold::ExactShared ex;
     __shared__ long long gfirst;
     __shared__ double xend;
     __shared__ long long red[kWaves];
     load_tile_lds(src, base, M, tile, 0.0);
-    fold::tile_exact(tile, x0, ex);
+    if constexpr (kFull) {
+        __shared__ fold::ExactShared ex;
+        fold::tile_exact(tile, x0, ex);
+    } else {
+        __shared__ double dred[kWaves];
+        if (!fold::tile_single(tile, x0, dred)) {
+            if (threadIdx.x == 0)
+                hard[1 + atomicAdd(hard, 1)] = static_cast<int>(tidx);
+            return 0.0;
+        }
+    }
     const long long nd = us.ndraw();
     const int nvalid = M - base < kTile ? static_cast<int>(M - base) : kTile;
     // count before the tile's first element: that of the previous sum
@@ -848,6 +949,7 @@ __device__ __forceinline__ double count_tile(WSrc src, USrc us, long long M,
         }
     }
     __syncthreads();
+    const auto sv = src.loaded();
     long long z = 0, e = 0, c = 0;
 #pragma unroll 4
     for (int j = 0; j < kItems; ++j) {
@@ -856,7 +958,7 @@ __device__ __forceinline__ double count_tile(WSrc src, USrc us, long long M,
             const long long g = __builtin_bit_cast(long long, tile[lds_pad(k)]);
             const long long gp = (k == 0) ? gfirst :
                 __builtin_bit_cast(long long, tile[lds_pad(k - 1)]);
-            const long long r = g - gp + src.integ(base + k);
+            const long long r = g - gp + sv.integ(base + k);
             rep[base + k] = static_cast<int>(r);
             z += r == 0;
             e += r > 1 ? r - 1 : 0;
@@ -1097,21 +1199,27 @@ __device__ __forceinline__ void scatter_tile(const int *rep, long long M, long l
 {
     const long long base = tidx * kTile;
     __shared__ L3 lds[kWaves];
-    __shared__ int tile[kTile];
+    __shared__ int tile[kTileLds];   // rep, element k at lds_pad(k)
+    __shared__ int stage[kTile];     // one output list at a time
+    const int nvalid = M - base < kTile ? static_cast<int>(M - base) : kTile;
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
-        const long long g = base + j * kBlock + threadIdx.x;
-        tile[j * kBlock + threadIdx.x] = g < M ? rep[g] : 0;
+        const int k = j * kBlock + static_cast<int>(threadIdx.x);
+        const int rv = k < nvalid ? rep[base + k] : 0;
+        tile[lds_pad(k)] = rv;
+        if (!kExpand && idx && k < nvalid && rv > 0)
+            idx[base + k] = static_cast<int>(base + k);
     }
     __syncthreads();
+    // blocked order: thread t owns elements t*kItems ..
+    const int kb = static_cast<int>(threadIdx.x) * kItems;
     int r[kItems];
     L3 loc{0, 0, 0};
-    const long long ib = base + static_cast<long long>(threadIdx.x) * kItems;
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
-        r[j] = tile[threadIdx.x * kItems + j];
-        const bool valid = ib + j < M;
-        const long long rr = valid ? r[j] : 0;
+        const bool valid = kb + j < nvalid;
+        r[j] = valid ? tile[lds_pad(kb + j)] : 0;
+        const long long rr = r[j];
         if (kExpand) {
             loc.b += rr;
             loc.c += rr > 0;
@@ -1122,38 +1230,49 @@ __device__ __forceinline__ void scatter_tile(const int *rep, long long M, long l
         }
     }
     L3 tot;
-    L3 ex = block_exclusive_scan(loc, OpAddL3(), L3{0, 0, 0}, lds, &tot);
-    long long zr = toff[3 * tidx + 0] + ex.a;
-    long long er = toff[3 * tidx + 1] + ex.b;
-    long long cr = toff[3 * tidx + 2] + ex.c;
+    const L3 ex = block_exclusive_scan(loc, OpAddL3(), L3{0, 0, 0}, lds, &tot);
+    // each list is staged in LDS in tile order and written out coalesced
+    if (!kExpand) {
+        int zr = static_cast<int>(ex.a);
 #pragma unroll
-    for (int j = 0; j < kItems; ++j) {
-        const long long i = ib + j;
-        if (i >= M)
-            break;
-        const long long rr = r[j];
-        if (kExpand) {
-            if (rr > 0) {
-                cE[cr] = static_cast<int>(er);
-                csrc[cr] = static_cast<int>(i);
-                ++cr;
-                er += rr;
-            }
-        } else {
-            if (rr == 0) {
-                zdst[zr] = static_cast<int>(i);
-                ++zr;
-            } else {
-                if (idx)
-                    idx[i] = static_cast<int>(i);
-                if (rr > 1) {
-                    cE[cr] = static_cast<int>(er);
-                    csrc[cr] = static_cast<int>(i);
-                    ++cr;
-                    er += rr - 1;
-                }
+        for (int j = 0; j < kItems; ++j)
+            if (kb + j < nvalid && r[j] == 0)
+                stage[zr++] = static_cast<int>(base + kb + j);
+        __syncthreads();
+        const long long z0 = toff[3 * tidx + 0];
+        for (int q = threadIdx.x; q < tot.a; q += kBlock)
+            zdst[z0 + q] = stage[q];
+        __syncthreads();
+    }
+    const long long c0 = toff[3 * tidx + 2];
+    {
+        int cr = static_cast<int>(ex.c);
+        long long er = toff[3 * tidx + 1] + ex.b;
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const int rr = r[j];
+            if (kExpand ? rr > 0 : rr > 1) {
+                stage[cr++] = static_cast<int>(er);
+                er += kExpand ? rr : rr - 1;
             }
         }
+        __syncthreads();
+        for (int q = threadIdx.x; q < tot.c; q += kBlock)
+            cE[c0 + q] = stage[q];
+        __syncthreads();
+    }
+    {
+        int cr = static_cast<int>(ex.c);
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const int rr = r[j];
+            if (kExpand ? rr > 0 : rr > 1)
+                stage[cr++] = static_cast<int>(base + kb + j);
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < tot.c; q += kBlock)
+            csrc[c0 + q] = stage[q];
+        __syncthreads();
     }
 }
 
@@ -1201,41 +1320,91 @@ __device__ __forceinline__ void expand_chunk(long long chunk, long long K, long 
     int dim, long long ld)
 {
     const long long k0 = chunk * kTile;
-    const long long kl = (k0 + kTile < K ? k0 + kTile : K) - 1;
-    __shared__ int win[kTile + 1];
+    const int nk = static_cast<int>((k0 + kTile < K ? k0 + kTile : K) - k0);
+    __shared__ int mark[kTileLds];
+    __shared__ int mred[kWaves];
     __shared__ long long clo_s, chi_s;
     if (threadIdx.x == 0) {
-        long long lo = upper_bound_i(cE, C, k0) - 1;
-        long long hi = upper_bound_i(cE, C, kl) - 1;
+        const long long lo = upper_bound_i(cE, C, k0) - 1;
+        const long long hi = upper_bound_i(cE, C, k0 + nk - 1) - 1;
         clo_s = lo < 0 ? 0 : lo;
         chi_s = hi < 0 ? 0 : hi;
     }
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+        mark[lds_pad(j * kBlock + static_cast<int>(threadIdx.x))] = -1;
     __syncthreads();
     const long long clo = clo_s, chi = chi_s;
-    // each source in the window covers at least one rank, so the window
-    // holds at most kTile entries; search globally if counts break that
-    const long long wn = chi - clo + 1;
-    const bool in_lds = wn >= 1 && wn <= kTile;
-    if (in_lds)
-        for (long long q = threadIdx.x; q < wn; q += kBlock)
-            win[q] = cE[clo + q];
-    __syncthreads();
+    // the sources covering this chunk's ranks start at strictly increasing
+    // ranks (each covers at least one), so there are at most kTile of them;
+    // otherwise (counts that break this) search per rank
+    const bool marked = chi - clo + 1 <= kTile;
+    if (marked) {
+        // each source marks the first rank of the chunk it covers ...
+        for (long long q = clo + threadIdx.x; q <= chi; q += kBlock) {
+            long long p = static_cast<long long>(cE[q]) - k0;
+            p = p < 0 ? 0 : p;
+            if (p < nk)
+                mark[lds_pad(static_cast<int>(p))] = static_cast<int>(q - clo);
+        }
+        __syncthreads();
+        // ... and a running maximum carries it over the ranks it covers
+        const int kb = static_cast<int>(threadIdx.x) * kItems;
+        int run = -1;
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const int v = mark[lds_pad(kb + j)];
+            run = v > run ? v : run;
+        }
+        int tot;
+        run = block_exclusive_scan(run, OpMaxI(), -1, mred, &tot);
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const int v = mark[lds_pad(kb + j)];
+            run = v > run ? v : run;
+            mark[lds_pad(kb + j)] = run;
+        }
+        __syncthreads();
+    }
+    // all index loads of this thread's ranks first, then all state loads,
+    // then the stores: the memory latencies overlap instead of adding up
+    int src[kItems], dst[kItems];
+#pragma unroll
     for (int j = 0; j < kItems; ++j) {
-        const long long k = k0 + j * kBlock + threadIdx.x;
-        if (k > kl)
-            break;
-        long long c = in_lds ? clo + upper_bound_i(win, wn, k) - 1 :
-                               upper_bound_i(cE, C, k) - 1;
-        c = c < 0 ? 0 : (c >= C ? C - 1 : c);
-        const int src = csrc[c];
-        const long long dst = kExpand ? k : zdst[k];
-        if (idx)
-            idx[dst] = src;
-        if (state) {
-            for (int d = 0; d < dim; ++d)
-                state[d * ld + dst] = state[d * ld + src];
+        const int kk = j * kBlock + static_cast<int>(threadIdx.x);
+        src[j] = -1;
+        dst[j] = 0;
+        if (kk < nk) {
+            const long long k = k0 + kk;
+            long long c = marked ? clo + mark[lds_pad(kk)] : upper_bound_i(cE, C, k) - 1;
+            c = c < 0 ? 0 : (c >= C ? C - 1 : c);
+            src[j] = csrc[c];
+            dst[j] = kExpand ? static_cast<int>(k) : zdst[k];
         }
     }
+    if (idx) {
+#pragma unroll
+        for (int j = 0; j < kItems; ++j)
+            if (src[j] >= 0)
+                idx[dst[j]] = src[j];
+    }
+    if (state) {
+        // sources are never destinations, so the loads see the old state;
+        // one component at a time, from a uniform column base
+        for (int d = 0; d < dim; ++d) {
+            E *const col = state + d * ld;
+            E v[kItems];
+#pragma unroll
+            for (int j = 0; j < kItems; ++j)
+                if (src[j] >= 0)
+                    v[j] = col[static_cast<unsigned>(src[j])];
+#pragma unroll
+            for (int j = 0; j < kItems; ++j)
+                if (src[j] >= 0)
+                    col[static_cast<unsigned>(dst[j])] = v[j];
+        }
+    }
+    __syncthreads();
 }
 
 __global__ void k_index_rep(const int *idx, long long N, int *rep)
@@ -1287,7 +1456,7 @@ __global__ void __launch_bounds__(kBlock) k_batch_resample(
     const USrc us{p};
     double x = 0.0;
     for (long long k = 0; k < F.ntiles; ++k)
-        x = count_tile(src, us, n, k, x, repf, tg);
+        x = count_tile<true>(src, us, n, k, x, repf, tg, static_cast<int *>(nullptr));
     __threadfence();
     __syncthreads();
     scan_tiles_block(tg, F.ntiles, to);
@@ -1477,6 +1646,7 @@ bool Workspace::ensure(long long n)
     A(&v, sizeof(double) * nn);
     A(&aux, sizeof(double) * nn);
     A(&rep, sizeof(int) * nn);
+    A(&hard, sizeof(int) * (tiles + 1));
     A(&prm, sizeof(ResampleDev));
     A(&scal, sizeof(long long) * 16);
     if (!ok || !all_allocated()) {
@@ -1492,7 +1662,7 @@ bool Workspace::ensure(long long n)
 bool Workspace::all_allocated() const
 {
     const void *ps[] = {tsum, tpre, segs, xstart, xgstart, gmode, groups, tagg,
-        toff, zdst, cE, csrc, v, aux, rep, prm, scal};
+        toff, zdst, cE, csrc, v, aux, rep, hard, prm, scal};
     for (const void *p : ps)
         if (!p)
             return false;
@@ -1502,13 +1672,13 @@ bool Workspace::all_allocated() const
 void Workspace::release()
 {
     void *ps[] = {tsum, tpre, segs, xstart, xgstart, gmode, groups, tagg, toff,
-        zdst, cE, csrc, v, aux, rep, prm, scal};
+        zdst, cE, csrc, v, aux, rep, hard, prm, scal};
     for (void *p : ps)
         if (p)
             (void)hipFree(p);
     tsum = tpre = xstart = xgstart = v = aux = nullptr;
     segs = nullptr;
-    gmode = zdst = cE = csrc = rep = nullptr;
+    gmode = zdst = cE = csrc = rep = hard = nullptr;
     groups = nullptr;
     tagg = toff = scal = nullptr;
     prm = nullptr;
@@ -1559,7 +1729,7 @@ static int run_fold(Workspace &ws, WSrc src, long long nmax,
         ws.prm, ws.segs, ws.groups);
     hipLaunchKernelGGL(k_resolve<WSrc>, dim3(1), dim3(kBlock), 0, st, src,
         nmax, ndev, ws.prm, ws.segs, ws.groups, ws.xstart, ws.xgstart,
-        ws.gmode);
+        ws.gmode, ws.hard);
     hipLaunchKernelGGL(k_starts, dim3(gg), dim3(kWave), 0, st, nmax, ndev,
         ws.prm, ws.segs, ws.xgstart, ws.gmode, ws.xstart);
     VSMCH_HIP(hipGetLastError());
@@ -1578,10 +1748,14 @@ static int run_count(Workspace &ws, WSrc src, USrc us, long long M, int *rep,
     int rc = run_fold(ws, src, M, nullptr, tpre_ready);
     if (rc != kOk)
         return rc;
-    hipLaunchKernelGGL((k_count<WSrc, USrc>), dim3(grid_tiles(M)),
+    const unsigned int gt = grid_tiles(M);
+    hipLaunchKernelGGL((k_count<WSrc, USrc>), dim3(gt), dim3(kBlock), 0,
+        ws.stream, src, us, M, static_cast<const long long *>(nullptr), ws.prm,
+        fold_bufs(ws), rep, ws.tagg, ws.hard);
+    hipLaunchKernelGGL((k_count_hard<WSrc, USrc>), dim3(gt < 256 ? gt : 256),
         dim3(kBlock), 0, ws.stream, src, us, M,
         static_cast<const long long *>(nullptr), ws.prm, fold_bufs(ws), rep,
-        ws.tagg);
+        ws.tagg, static_cast<const int *>(ws.hard));
     VSMCH_HIP(hipGetLastError());
     return kOk;
 }
