@@ -45,24 +45,32 @@ def parse():
     return ap.parse_args()
 
 
-def run_steps(va, f, scheme, thr, t0, k, ox, oy):
+def run_steps(va, f, scheme, thr, t0, k, ox, oy, host_decides=False):
+    """k steps of move + resample. With host_decides the resampling decision
+    is read on the host (Sampler::do_resample's order) and a step that does
+    not resample queues no further work; otherwise the whole event is queued
+    and decided on the device."""
     for t in range(t0, t0 + k):
         f.move(t, float(ox[t]), float(oy[t]))
-        f.resample_async(scheme, thr)
+        if host_decides:
+            f.resample(scheme, thr)
+        else:
+            f.resample_async(scheme, thr)
     return t0 + k
 
 
-def time_config(va, torch, n, scheme, thr, warmup, steps, ox, oy, seed=1):
+def time_config(va, torch, n, scheme, thr, warmup, steps, ox, oy, seed=1,
+                host_decides=False):
     f = va.CvFilter(n, "float32", seed=seed)
     f.init(float(ox[0]), float(oy[0]))
     f.resample_async(scheme, thr)
-    t = run_steps(va, f, scheme, thr, 1, warmup, ox, oy)
+    t = run_steps(va, f, scheme, thr, 1, warmup, ox, oy, host_decides)
     torch.cuda.synchronize()
     r0 = f.resample_count()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record()
-    t = run_steps(va, f, scheme, thr, t, steps, ox, oy)
+    t = run_steps(va, f, scheme, thr, t, steps, ox, oy, host_decides)
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1)

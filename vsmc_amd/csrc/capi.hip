@@ -390,8 +390,10 @@ vsmch_status vsmch_particle_resample(vsmch_particle *p, int scheme,
     if (!p || scheme < 0 || scheme > 5)
         return VSMCH_ERR_INVALID;
     ParticleSet *ps = p->p;
+    // with a host result the decision is read first: a step that does not
+    // resample then queues no further work
     int rc = resample_step(ps, scheme, threshold, 0, nullptr, nullptr, rep_out,
-        idx_out);
+        idx_out, resampled_host != nullptr);
     if (rc != VSMCH_OK)
         return rc;
     if (resampled_host) {

@@ -172,7 +172,24 @@ struct OpSeg {
     }
 };
 
+constexpr int kMaxTies = 32;   // half-ulp ties a one-run tile can carry
+
+struct SingleShared {
+    double dred[kWaves];
+    int nred[kWaves];
+    int tk[kMaxTies];       // element of each tie, in order
+    double tpre[kMaxTies];  // rounded steps before it
+    double tq[kMaxTies];    // its step without the half
+    int cum[kMaxTies];      // extra ulps from the ties up to it
+    int ok;
+};
+
+struct OpAddI {
+    __device__ int operator()(int a, int b) const { return a + b; }
+};
+
 struct ExactShared {
+    SingleShared single;
     FMap scan[kWaves];
     int red[kWaves];
     double dred[kWaves];
@@ -428,12 +445,15 @@ __device__ __forceinline__ void tile_runs(double *t, int pos0, double cur0,
     __syncthreads();
 }
 
-// The common tile: every sum stays in the binade e of the exact start x0
-// and no step is a half-ulp tie. Then s_k = x0 advanced by the prefix sum
-// of the rounded steps rint(w / 2^(e-52)), and whether that holds is
-// known from the total (the sums only grow). Writes the tile and returns
-// true, or writes nothing and returns false. All threads call it.
-__device__ __forceinline__ bool tile_single(double *t, double x0, double *dred)
+// The common tile: every sum stays in the binade e of the exact start x0.
+// Then s_k = x0 advanced by the prefix sum of the rounded steps
+// rint(w / 2^(e-52)), and whether that holds is known from the total (the
+// sums only grow). A step of exactly f + 1/2 ulp (a tie) rounds to the even
+// one of a + f and a + f + 1, which depends on the running sum; up to
+// kMaxTies of them are resolved in order by one thread. Writes the tile
+// and returns true, or writes nothing and returns false. All threads call
+// it.
+__device__ __forceinline__ bool tile_single(double *t, double x0, SingleShared &sh)
 {
     const int e = sum_tag(x0);
     if (!finite_tag(e))
@@ -441,22 +461,81 @@ __device__ __forceinline__ bool tile_single(double *t, double x0, double *dred)
     const int base = threadIdx.x * kItems;
     double *const mine = t + lds_pad(base);
     double dsum = 0.0;
-    bool tie = false;
+    int ntie = 0;
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
         const double q = ldexp(mine[j], 52 - e);
-        tie = tie || (q - trunc(q) == 0.5);
-        dsum += rint(q);
+        const double f = trunc(q);
+        const bool tie = q - f == 0.5;
+        dsum += tie ? f : rint(q);
+        ntie += tie ? 1 : 0;
     }
     double dtot;
-    double run = block_exclusive_scan(dsum, OpAddD(), 0.0, dred, &dtot);
+    double run = block_exclusive_scan(dsum, OpAddD(), 0.0, sh.dred, &dtot);
     const double a0d = static_cast<double>(mant(x0));
-    if (__syncthreads_or(tie) || !(a0d + dtot < static_cast<double>(kTwo53)))
-        return false;
+    const double two53 = static_cast<double>(kTwo53);
+    if (!__syncthreads_or(ntie > 0)) {
+        if (!(a0d + dtot < two53))
+            return false;
 #pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            run += rint(ldexp(mine[j], 52 - e));
+            mine[j] = ldexp(a0d + run, e - 52);
+        }
+        __syncthreads();
+        return true;
+    }
+    int ntot;
+    const int toff = block_exclusive_scan(ntie, OpAddI(), 0, sh.nred, &ntot);
+    if (ntot > kMaxTies || !(a0d + dtot < two53))
+        return false;
+    if (ntie > 0) {
+        double r = run;
+        int q_i = toff;
+#pragma unroll 1
+        for (int j = 0; j < kItems; ++j) {
+            const double q = ldexp(mine[j], 52 - e);
+            const double f = trunc(q);
+            if (q - f == 0.5) {
+                sh.tk[q_i] = base + j;
+                sh.tpre[q_i] = r;
+                sh.tq[q_i] = f;
+                ++q_i;
+                r += f;
+            } else {
+                r += rint(q);
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // every step total is below 2^53 - a0 here, so exact as an integer
+        const long long a0 = mant(x0);
+        long long adj = 0;
+        for (int i = 0; i < ntot; ++i) {
+            const long long ab = a0 + static_cast<long long>(sh.tpre[i]) + adj;
+            adj += ((ab + static_cast<long long>(sh.tq[i])) & 1) ? 1 : 0;
+            sh.cum[i] = static_cast<int>(adj);
+        }
+        sh.ok = a0 + static_cast<long long>(dtot) + adj < kTwo53 ? 1 : 0;
+    }
+    __syncthreads();
+    if (!sh.ok)
+        return false;
+    int ti = toff;
+    double adj = ti > 0 ? static_cast<double>(sh.cum[ti - 1]) : 0.0;
+#pragma unroll 1
     for (int j = 0; j < kItems; ++j) {
-        run += rint(ldexp(mine[j], 52 - e));
-        mine[j] = ldexp(a0d + run, e - 52);
+        const double q = ldexp(mine[j], 52 - e);
+        const double f = trunc(q);
+        if (q - f == 0.5) {
+            run += f;
+            adj = static_cast<double>(sh.cum[ti]);
+            ++ti;
+        } else {
+            run += rint(q);
+        }
+        mine[j] = ldexp(a0d + run + adj, e - 52);
     }
     __syncthreads();
     return true;
@@ -586,7 +665,7 @@ __device__ __forceinline__ void tile_exact_from(double *t, int pos, double cur,
 // of the block must call it (it synchronises, also before returning).
 __device__ __forceinline__ void tile_exact(double *t, double x0, ExactShared &sh)
 {
-    if (tile_single(t, x0, sh.dred))
+    if (tile_single(t, x0, sh.single))
         return;
     int pos = 0;
     double cur = x0;

@@ -45,6 +45,18 @@ struct BatchFilter {
     int pad;
 };
 
+struct PDev;
+
+// A particle set's update when its resampling event completes, launched
+// after the state gather (pd == nullptr: none).
+struct EventEnd {
+    PDev *pd = nullptr;
+    const int *rep = nullptr;
+    long long n = 0;
+    int scheme = 0;
+    int words = 0;
+};
+
 // Grow-only device workspace shared by the resampling passes.
 struct Workspace {
     hipStream_t stream = nullptr;
@@ -65,7 +77,7 @@ struct Workspace {
     double *v = nullptr;         // materialised uniforms (running max)
     double *aux = nullptr;       // scratch N doubles
     int *rep = nullptr;          // scratch counts
-    int *hard = nullptr;         // [0]: count, then tiles left to the full fold
+    int *hard = nullptr;         // [0]: count, then tiles k_count could not fold
     int *integ_dummy = nullptr;
     ResampleDev *prm = nullptr;
     long long *scal = nullptr;   // small device scalars
@@ -141,7 +153,8 @@ int trans_u01_rep(Workspace &ws, long long M, long long N, const double *W,
 // aggregates in ws.tagg were produced by the counting pass that wrote rep.
 int trans_rep_index(Workspace &ws, long long M, long long N, const int *rep,
     int *idx, void *state, int dim, long long ld, int elem_bytes,
-    bool device_active, bool force_expand = false, bool agg_ready = false);
+    bool device_active, bool force_expand = false, bool agg_ready = false,
+    const EventEnd &end = EventEnd());
 
 int trans_index_rep(Workspace &ws, long long M, long long N, const int *idx,
     int *rep);

@@ -262,42 +262,6 @@ __global__ void __launch_bounds__(kPB) k_prepare(PDev *pd, ResampleDev *prm,
         [&](long long b, double ex) { tpre[b] = ex; }, OpAddD(), 0.0, lds);
 }
 
-// After the event: equal weights, advance the resampling stream. With
-// `words` the event's stream use is computed here (systematic: the one
-// constructor word; stratified: up to and including the first uniform above
-// s_{M-2}, transform.hpp:61-69).
-__device__ inline void after_event(PDev *pd, ResampleDev *prm, long long n,
-    const int *rep, int scheme, int words);
-
-__global__ void k_after_resample(PDev *pd, ResampleDev *prm, long long n,
-    const int *rep, int scheme, int words)
-{
-    after_event(pd, prm, n, rep, scheme, words);
-}
-
-__device__ inline void after_event(PDev *pd, ResampleDev *prm, long long n,
-    const int *rep, int scheme, int words)
-{
-    if (!prm->active)
-        return;
-    if (words) {
-        unsigned long long used = 1;
-        if (scheme == kStratified) {
-            const long long jend = n - rep[n - 1];
-            used = static_cast<unsigned long long>(jend + 1 < n ? jend + 1 : n);
-        }
-        if (n < 2)
-            used = scheme == kSystematic ? 1 : 0;
-        prm->words_used = used;
-    }
-    pd->nresampled += 1;
-    pd->rword += prm->words_used;
-    pd->equal = 1;
-    pd->log_equal = -log(static_cast<double>(n));
-    pd->c = pd->log_equal;
-    pd->ess = static_cast<double>(n);
-}
-
 // Weighted moments r_d = sum_i W_i x_d(i) of components [c0, c0 + dim)
 // (dim <= 8); per-block partials of sum exp(l - lmax) x and the merge.
 template <typename Real>
@@ -991,6 +955,8 @@ void particle_destroy(ParticleSet *p)
     for (void *q : ps)
         if (q)
             (void)hipFree(q);
+    if (p->host_active)
+        (void)hipHostFree(p->host_active);
     p->ws.release();
     delete p;
 }
@@ -1024,9 +990,13 @@ int weight_read(ParticleSet *p, double *W)
     return kOk;
 }
 
-// One resampling event driven from the device decision (no host sync).
+// One resampling event. The decision is made on the device; with
+// host_decides the host reads it right away (one synchronisation) and a
+// step that does not resample launches nothing more. Otherwise the whole
+// event is queued and its kernels return early when it is inactive.
 int resample_step(ParticleSet *p, int scheme, double threshold, long long t,
-    double *ess_hist, int *rs_hist, int *rep_out, int *idx_out)
+    double *ess_hist, int *rs_hist, int *rep_out, int *idx_out,
+    bool host_decides)
 {
     hipStream_t st = p->ws.stream;
     // systematic / stratified with N >= 2: one kernel prepares the decision,
@@ -1040,6 +1010,16 @@ int resample_step(ParticleSet *p, int scheme, double threshold, long long t,
         hipLaunchKernelGGL(k_decide, dim3(1), dim3(1), 0, st, p->pd, p->ws.prm,
             p->n, threshold, t, ess_hist, rs_hist);
     VSMCH_HIP(hipGetLastError());
+    if (host_decides) {
+        if (!p->host_active)
+            VSMCH_HIP(hipHostMalloc(reinterpret_cast<void **>(&p->host_active),
+                sizeof(int), hipHostMallocDefault));
+        VSMCH_HIP(hipMemcpyAsync(p->host_active, &p->ws.prm->active, sizeof(int),
+            hipMemcpyDeviceToHost, st));
+        VSMCH_HIP(hipStreamSynchronize(st));
+        if (!*p->host_active)
+            return kOk;
+    }
     int *rep = rep_out ? rep_out : p->rep;
     int rc = resample_scheme(p->ws, scheme, p->n, p->n, 0, 0, p->logw,
         kWeightLog, rep, true, prep);
@@ -1047,14 +1027,15 @@ int resample_step(ParticleSet *p, int scheme, double threshold, long long t,
         return rc;
     // the counting pass left the rep -> index aggregates in ws.tagg (a
     // single particle takes the fill path, which writes none)
-    rc = trans_rep_index(p->ws, p->n, p->n, rep, idx_out, p->state, p->dim,
-        p->n, p->bytes, true, false, p->n >= 2);
-    if (rc != kOk)
-        return rc;
-    hipLaunchKernelGGL(k_after_resample, dim3(1), dim3(1), 0, st, p->pd,
-        p->ws.prm, p->n, static_cast<const int *>(rep), scheme, prep ? 1 : 0);
-    VSMCH_HIP(hipGetLastError());
-    return kOk;
+    // after_event follows the state gather
+    EventEnd end;
+    end.pd = p->pd;
+    end.rep = rep;
+    end.n = p->n;
+    end.scheme = scheme;
+    end.words = prep ? 1 : 0;
+    return trans_rep_index(p->ws, p->n, p->n, rep, idx_out, p->state, p->dim,
+        p->n, p->bytes, true, false, p->n >= 2, end);
 }
 
 // r_d = sum_i W_i buf[i * dim + d0 + d] for 8 components at a time.

@@ -21,6 +21,33 @@ struct PDev {
     int resampled;             // last decision
 };
 
+// After a resampling event: equal weights, advance the resampling stream.
+// With `words` the event's stream use is computed here (systematic: the one
+// constructor word; stratified: up to and including the first uniform above
+// s_{M-2}, transform.hpp:61-69).
+__device__ inline void after_event(PDev *pd, ResampleDev *prm, long long n,
+    const int *rep, int scheme, int words)
+{
+    if (!prm->active)
+        return;
+    if (words) {
+        unsigned long long used = 1;
+        if (scheme == kStratified) {
+            const long long jend = n - rep[n - 1];
+            used = static_cast<unsigned long long>(jend + 1 < n ? jend + 1 : n);
+        }
+        if (n < 2)
+            used = scheme == kSystematic ? 1 : 0;
+        prm->words_used = used;
+    }
+    pd->nresampled += 1;
+    pd->rword += prm->words_used;
+    pd->equal = 1;
+    pd->log_equal = -log(static_cast<double>(n));
+    pd->c = pd->log_equal;
+    pd->ess = static_cast<double>(n);
+}
+
 // Model constants of the constant-velocity PF, precomputed on the host.
 struct CvModelDev {
     double dt;
@@ -46,6 +73,7 @@ struct ParticleSet {
     double *mout = nullptr;
     PDev *pd = nullptr;
     unsigned long long *rng_words = nullptr;  // allocated on first request
+    int *host_active = nullptr;  // pinned: the decision read by the host
     Workspace ws;
 };
 
@@ -58,7 +86,8 @@ int weight_update(ParticleSet *p, const double *src, int mode);
 int read_pdev(ParticleSet *p, PDev *h);
 int weight_read(ParticleSet *p, double *W);
 int resample_step(ParticleSet *p, int scheme, double threshold, long long t,
-    double *ess_hist, int *rs_hist, int *rep_out, int *idx_out);
+    double *ess_hist, int *rs_hist, int *rep_out, int *idx_out,
+    bool host_decides = false);
 int moments(ParticleSet *p, int c0, int dim, double *out_dev);
 int cv_step(ParticleSet *p, const CvModelDev &m, long long t, double ox,
     double oy, bool with_monitor);

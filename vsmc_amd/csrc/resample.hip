@@ -13,6 +13,7 @@
 // G comes from closed forms for systematic and stratified uniforms and from
 // a search over the materialised sequence otherwise.
 #include "internal.hpp"
+#include "particle.hpp"
 #include "../../include/vsmch/device_rng.hpp"
 
 #include <mutex>
@@ -344,6 +345,34 @@ __global__ void __launch_bounds__(kBlock) k_scan_tiles(const double *tsum,
     }
 }
 
+// Summary of group g (64 tiles), by one wave: flat when every tile is a
+// single run in the same binade; the map is the composition of the tiles'.
+__device__ __forceinline__ void group_summary(const TileSeg *segs,
+    long long ntiles, long long g, GroupSeg *groups)
+{
+    const long long t = g * kGroup + threadIdx.x;
+    const bool valid = t < ntiles;
+    int nc = 0, e = fold::kEmpty;
+    FMap m = fold::ident();
+    if (valid) {
+        nc = segs[t].ncross;
+        e = segs[t].e[0];
+        m = segs[t].map[0];
+    }
+    const int e0 = __shfl(e, 0, kWave);
+    const bool ok = !valid || (nc == 0 && e == e0 && e != fold::kEmpty);
+    const unsigned long long bal = __ballot(ok);
+    const FMap P = wave_inclusive_scan(m, fold::OpCompose());
+    const FMap tot = shfl_any(P, kWave - 1);
+    if (threadIdx.x == 0) {
+        GroupSeg gs;
+        gs.flat = (bal == ~0ull) ? 1 : 0;
+        gs.e = e0;
+        gs.map = tot;
+        groups[g] = gs;
+    }
+}
+
 template <class WSrc>
 __global__ void __launch_bounds__(kBlock) k_segments(WSrc src, long long nh,
     const long long *ndev, const ResampleDev *prm, const double *tpre,
@@ -363,39 +392,17 @@ __global__ void __launch_bounds__(kBlock) k_segments(WSrc src, long long nh,
         &segs[blockIdx.x], sh);
 }
 
-// One wave per group of 64 tiles: flat when every tile is a single run in
-// the same binade.
+// One wave per group of 64 tiles (group_summary).
 __global__ void __launch_bounds__(kWave) k_groups(long long nh,
     const long long *ndev, const ResampleDev *prm, const TileSeg *segs,
     GroupSeg *groups)
 {
     if (!prm->active)
         return;
-    const long long n = eff_n(ndev, nh);
-    const long long ntiles = ceil_div(n, kTile);
-    const long long t = static_cast<long long>(blockIdx.x) * kGroup + threadIdx.x;
+    const long long ntiles = ceil_div(eff_n(ndev, nh), kTile);
     if (static_cast<long long>(blockIdx.x) * kGroup >= ntiles)
         return;
-    const bool valid = t < ntiles;
-    int nc = 0, e = fold::kEmpty;
-    FMap m = fold::ident();
-    if (valid) {
-        nc = segs[t].ncross;
-        e = segs[t].e[0];
-        m = segs[t].map[0];
-    }
-    const int e0 = __shfl(e, 0, kWave);
-    const bool ok = !valid || (nc == 0 && e == e0 && e != fold::kEmpty);
-    const unsigned long long bal = __ballot(ok);
-    const FMap P = wave_inclusive_scan(m, fold::OpCompose());
-    const FMap tot = shfl_any(P, kWave - 1);
-    if (threadIdx.x == 0) {
-        GroupSeg gs;
-        gs.flat = (bal == ~0ull) ? 1 : 0;
-        gs.e = e0;
-        gs.map = tot;
-        groups[blockIdx.x] = gs;
-    }
+    group_summary(segs, ntiles, blockIdx.x, groups);
 }
 
 constexpr int kRGroups = kGroup;   // group summaries held per window
@@ -886,7 +893,7 @@ __global__ void __launch_bounds__(kBlock) k_count(WSrc src, USrc us,
         hard);
 }
 
-// The tiles k_count left (hard[1 .. hard[0]]), with the full exact fold.
+// The tiles of a list (hard[1 .. hard[0]]), with the full exact fold.
 template <class WSrc, class USrc>
 __global__ void __launch_bounds__(kBlock) k_count_hard(WSrc src, USrc us,
     long long nh, const long long *ndev, const ResampleDev *prm, FoldBufs fb,
@@ -923,8 +930,8 @@ __device__ __forceinline__ double count_tile(WSrc src, USrc us, long long M,
         __shared__ fold::ExactShared ex;
         fold::tile_exact(tile, x0, ex);
     } else {
-        __shared__ double dred[kWaves];
-        if (!fold::tile_single(tile, x0, dred)) {
+        __shared__ fold::SingleShared ssh;
+        if (!fold::tile_single(tile, x0, ssh)) {
             if (threadIdx.x == 0)
                 hard[1 + atomicAdd(hard, 1)] = static_cast<int>(tidx);
             return 0.0;
@@ -1310,6 +1317,12 @@ __global__ void __launch_bounds__(kBlock) k_expand(const ResampleDev *prm,
         return;
     expand_chunk<kExpand, E>(blockIdx.x, K, C, zdst, cE, csrc, idx, state, dim,
         ld);
+}
+
+// The particle set's update once the event's gather is done (one thread).
+__global__ void k_event_end(ResampleDev *prm, EventEnd end)
+{
+    after_event(end.pd, prm, end.n, end.rep, end.scheme, end.words);
 }
 
 // Ranks [chunk kTile, (chunk + 1) kTile) of the K ranks: the source of each
@@ -2001,14 +2014,14 @@ int read_words_used(Workspace &ws, unsigned long long *out)
 
 int trans_rep_index(Workspace &ws, long long M, long long N, const int *rep,
     int *idx, void *state, int dim, long long ld, int elem_bytes,
-    bool device_active, bool force_expand, bool agg_ready)
+    bool device_active, bool force_expand, bool agg_ready, const EventEnd &end)
 {
     if (M <= 0 || N <= 0)
         return kOk;
     if (!ws.ensure(M > N ? M : N))
         return kErrAlloc;
     hipStream_t st = ws.stream;
-    const ResampleDev *prm = device_active ? ws.prm : nullptr;
+    ResampleDev *prm = device_active ? ws.prm : nullptr;
     const long long ntiles = (M + kTile - 1) / kTile;
     const bool expand = M != N || force_expand;
     if (expand) {
@@ -2041,6 +2054,8 @@ int trans_rep_index(Workspace &ws, long long M, long long N, const int *rep,
             0, st, prm, ws.toff, ntiles, ws.zdst, ws.cE, ws.csrc, idx,
             static_cast<double *>(state), dim, ld);
     }
+    if (end.pd)
+        hipLaunchKernelGGL(k_event_end, dim3(1), dim3(1), 0, st, ws.prm, end);
     VSMCH_HIP(hipGetLastError());
     return kOk;
 }
