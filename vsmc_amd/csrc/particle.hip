@@ -394,13 +394,18 @@ __device__ inline normal_pair<Real> cv_normals(uint32_t wr, uint32_t wa)
 {
     return box_muller<Real>(wr, wa);
 }
+// FP32 state: the hardware's single-precision log2, square root and sine /
+// cosine (v_log_f32, v_sqrt_f32, v_sin_f32, v_cos_f32; the angle argument is
+// in revolutions, so sin(2 pi a) is one instruction). Each is within a few
+// FP32 ulps; the library routines cost tens of instructions per call, and
+// the move is instruction-bound.
 template <>
 __device__ inline normal_pair<float> cv_normals<float>(uint32_t wr, uint32_t wa)
 {
-    const float r = sqrtf(-2.0f * logf(u01_u32f(wr)));
-    float s, c;
-    sincospif(2.0f * u01_u32f(wa), &s, &c);
-    return normal_pair<float>{r * c, r * s};
+    constexpr float kTwoLn2 = 1.38629436f;   // 2 ln 2: -2 ln u = -2 ln2 log2 u
+    const float r = __builtin_amdgcn_sqrtf(-kTwoLn2 * __builtin_amdgcn_logf(u01_u32f(wr)));
+    const float a = u01_u32f(wa);
+    return normal_pair<float>{r * __builtin_amdgcn_cosf(a), r * __builtin_amdgcn_sinf(a)};
 }
 
 // One particle's move (or initialisation) from its Philox block `blk`.
