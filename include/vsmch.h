@@ -380,6 +380,41 @@ vsmch_status vsmch_cv_run_learned(vsmch_particle *p, const vsmch_cv_model *m,
     const double *obs_y_host, int scheme, double threshold, double *ess_host,
     int *resampled_host, double *pos_host, double *logz_host);
 
+/* ---------------------------------------------------------------------
+ * Ancestry history: ResampleIndex (include/vsmc/resample/index.hpp:46-206)
+ * on the device. Row t holds the ancestor index of every particle at
+ * iteration t; `idx` arguments are device arrays of n ints (NULL: the
+ * identity, a step without resampling), e.g. the ancestors that
+ * vsmch_particle_resample writes to idx_out. An ancestor outside [0, n)
+ * makes index / index_matrix fail with VSMCH_ERR_INVALID. */
+#define VSMCH_ROW_MAJOR 101   /* vsmc::RowMajor (internal/defines.hpp:49) */
+#define VSMCH_COL_MAJOR 102   /* vsmc::ColMajor */
+
+typedef struct vsmch_rindex vsmch_rindex;
+
+/* ResampleIndex(N) (index.hpp:53-57) */
+vsmch_status vsmch_rindex_create(vsmch_rindex **r, int64_t n);
+vsmch_status vsmch_rindex_destroy(vsmch_rindex *r);
+/* iter_size() (:62) */
+vsmch_status vsmch_rindex_iter_size(vsmch_rindex *r, int64_t *iters);
+/* reset() (:65): no iterations, memory kept */
+vsmch_status vsmch_rindex_reset(vsmch_rindex *r);
+/* clear() (:68): releases the rows */
+vsmch_status vsmch_rindex_clear(vsmch_rindex *r);
+/* push_back() / push_back(first) (:70-87) */
+vsmch_status vsmch_rindex_push_back(vsmch_rindex *r, const int *idx);
+/* insert() / insert(first) / insert(iter, first) (:89-119); iter < 0: the
+ * last iteration */
+vsmch_status vsmch_rindex_insert(vsmch_rindex *r, int64_t iter, const int *idx);
+/* index(id, iter) (:121-134): the ancestor at iteration iter of particle id
+ * of the last iteration */
+vsmch_status vsmch_rindex_index(vsmch_rindex *r, int64_t id, int64_t iter,
+    int64_t *idx_host);
+/* index_matrix<Layout> / read_index_matrix (:136-148, :156-204): out is a
+ * device array of n * iter_size ints; RowMajor: row i is particle i's
+ * lineage over the iterations, ColMajor: row t is iteration t. */
+vsmch_status vsmch_rindex_index_matrix(vsmch_rindex *r, int layout, int *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -813,6 +813,40 @@ void vo_trans_index_rep(size_t M, size_t N, const int64_t *idx,
         ++rep[idx[i]];
 }
 
+/* ResampleIndex::index, include/vsmc/resample/index.hpp:121-134: start
+ * from the last row at id, then apply the rows before it down to iter. */
+int64_t vo_rindex_index(size_t N, size_t T, const int64_t *hist, size_t id,
+    size_t iter)
+{
+    size_t cur = T - 1;
+    int64_t idx = hist[cur * N + id];
+    while (cur != iter) {
+        --cur;
+        idx = hist[cur * N + (size_t)idx];
+    }
+    return idx;
+}
+
+/* ResampleIndex::index_matrix_dispatch, index.hpp:156-204: the last
+ * iteration's column is the last row; each earlier one maps the next one
+ * through that iteration's row. */
+void vo_rindex_matrix(size_t N, size_t T, const int64_t *hist, int layout,
+    int64_t *out)
+{
+    if (N * T == 0)
+        return;
+    const int row_major = layout == 101;
+    for (size_t i = 0; i != N; ++i)
+        out[row_major ? i * T + (T - 1) : (T - 1) * N + i] = hist[(T - 1) * N + i];
+    for (size_t t = T - 1; t != 0; --t) {
+        const int64_t *row = hist + (t - 1) * N;
+        for (size_t i = 0; i != N; ++i) {
+            const int64_t last = out[row_major ? i * T + t : t * N + i];
+            out[row_major ? i * T + (t - 1) : (t - 1) * N + i] = row[(size_t)last];
+        }
+    }
+}
+
 /* resample_trans_residual, transform.hpp:154-168: resid_i, integ_i =
  * modf(N * W_i); resid *= 1 / (sequential sum of resid); returns
  * N - sum(integ). */

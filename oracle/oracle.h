@@ -144,6 +144,15 @@ void vo_trans_index_rep(size_t M, size_t N, const int64_t *idx,
 size_t vo_trans_residual(size_t M, size_t N, const double *W, double *resid,
     int64_t *integ);
 
+/* ResampleIndex (include/vsmc/resample/index.hpp:46-206) over a history of
+ * T rows of N ancestors (row t at hist + t N): index(id, iter) and
+ * index_matrix<RowMajor> (layout 101: out[i T + t]) / <ColMajor> (102:
+ * out[t N + i]). */
+int64_t vo_rindex_index(size_t N, size_t T, const int64_t *hist, size_t id,
+    size_t iter);
+void vo_rindex_matrix(size_t N, size_t T, const int64_t *hist, int layout,
+    int64_t *out);
+
 /* Resample schemes, enum order of include/vsmc/internal/defines.hpp
  * (ResampleScheme) and include/vsmc/vsmc.h:277-284 */
 #define VO_MULTINOMIAL 0

@@ -90,6 +90,10 @@ def oracle():
                                           P(np.float64), P(np.int64)]),
             "vo_trans_rep_index": (None, [c.c_size_t, c.c_size_t, P(np.int64), P(np.int64)]),
             "vo_trans_index_rep": (None, [c.c_size_t, c.c_size_t, P(np.int64), P(np.int64)]),
+            "vo_rindex_index": (c.c_int64, [c.c_size_t, c.c_size_t, P(np.int64), c.c_size_t,
+                                            c.c_size_t]),
+            "vo_rindex_matrix": (None, [c.c_size_t, c.c_size_t, P(np.int64), c.c_int,
+                                        P(np.int64)]),
             "vo_trans_residual": (c.c_size_t, [c.c_size_t, c.c_size_t, P(np.float64),
                                                P(np.float64), P(np.int64)]),
             "vo_resample": (c.c_uint64, [c.c_int, c.c_size_t, c.c_size_t,
@@ -179,6 +183,22 @@ def trans_rep_index(rep, N=None):
     idx = np.zeros(max(N, 1), np.int64)
     oracle().vo_trans_rep_index(M, N, rep if M else np.zeros(1, np.int64), idx)
     return idx[:N]
+
+
+def rindex_index(hist, id, iter):
+    """ResampleIndex::index over hist[T][N] (int64)."""
+    hist = np.ascontiguousarray(hist, np.int64)
+    T, N = hist.shape
+    return int(oracle().vo_rindex_index(N, T, hist, id, iter))
+
+
+def rindex_matrix(hist, layout=101):
+    """ResampleIndex::index_matrix<RowMajor (101) / ColMajor (102)>."""
+    hist = np.ascontiguousarray(hist, np.int64)
+    T, N = hist.shape
+    out = np.zeros(max(N * T, 1), np.int64)
+    oracle().vo_rindex_matrix(N, T, hist, layout, out)
+    return out[:N * T].reshape((N, T) if layout == 101 else (T, N))
 
 
 def trans_u01_index(W, u):

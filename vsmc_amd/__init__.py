@@ -15,6 +15,10 @@ LIB_PATH = os.path.join(_HERE, "libvsmch.so")
 PHILOX4X32 = 0
 THREEFRY4X64 = 1
 
+# vsmc::MatrixLayout (include/vsmc/internal/defines.hpp:49)
+ROW_MAJOR = 101
+COL_MAJOR = 102
+
 # vsmc::ResampleScheme order (include/vsmc/internal/defines.hpp:49-56)
 MULTINOMIAL = 0
 STRATIFIED = 1
@@ -79,6 +83,15 @@ SIGNATURES = {
     "vsmch_cv_move": (_c.c_int, [_P, _P, _c.c_int64, _c.c_double, _c.c_double]),
     "vsmch_cv_run": (_c.c_int, [_P, _P, _c.c_int64, _P, _P, _c.c_int, _c.c_double,
                                 _P, _P, _P, _P]),
+    "vsmch_rindex_create": (_c.c_int, [_c.POINTER(_P), _c.c_int64]),
+    "vsmch_rindex_destroy": (_c.c_int, [_P]),
+    "vsmch_rindex_iter_size": (_c.c_int, [_P, _c.POINTER(_c.c_int64)]),
+    "vsmch_rindex_reset": (_c.c_int, [_P]),
+    "vsmch_rindex_clear": (_c.c_int, [_P]),
+    "vsmch_rindex_push_back": (_c.c_int, [_P, _P]),
+    "vsmch_rindex_insert": (_c.c_int, [_P, _c.c_int64, _P]),
+    "vsmch_rindex_index": (_c.c_int, [_P, _c.c_int64, _c.c_int64, _c.POINTER(_c.c_int64)]),
+    "vsmch_rindex_index_matrix": (_c.c_int, [_P, _c.c_int, _P]),
     "vsmch_mlp_create": (_c.c_int, [_c.POINTER(_P)]),
     "vsmch_mlp_destroy": (_c.c_int, [_P]),
     "vsmch_mlp_set_params": (_c.c_int, [_P, _P, _P, _P]),
@@ -306,4 +319,5 @@ def sorted_uniforms(n, key, word_offset=0, device="cuda"):
     return out[:n]
 
 
-from .sampler import Particle, CvFilter, ProposalNet, CvBatch, GmmSampler  # noqa: E402,F401
+from .sampler import (Particle, CvFilter, ProposalNet, CvBatch, GmmSampler,  # noqa: E402,F401
+                      ResampleIndex)

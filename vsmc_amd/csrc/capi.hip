@@ -1,6 +1,7 @@
 // capi.hip -- the extern "C" entry points declared in include/vsmch.h.
 #include "particle.hpp"
 #include "mlp.hpp"
+#include "genealogy.hpp"
 #include "../../include/vsmch.h"
 #include "../../include/vsmch/device_rng.hpp"
 
@@ -68,6 +69,10 @@ struct vsmch_batch {
 struct vsmch_gmm {
     GmmSet *s;
     vsmch_particle view;
+};
+
+struct vsmch_rindex {
+    RIndex *r;
 };
 
 namespace {
@@ -761,6 +766,87 @@ vsmch_status vsmch_cv_run_learned(vsmch_particle *p, const vsmch_cv_model *m,
         return VSMCH_ERR_INVALID;
     return run_filter(p, m, net->m, T, obs_x_host, obs_y_host, scheme,
         threshold, ess_host, resampled_host, pos_host, logz_host);
+}
+
+// ---------------------------------------------------------- genealogy
+
+vsmch_status vsmch_rindex_create(vsmch_rindex **r, int64_t n)
+{
+    if (!r)
+        return VSMCH_ERR_INVALID;
+    RIndex *x = nullptr;
+    const int rc = rindex_create(&x, n, g_stream);
+    if (rc != VSMCH_OK)
+        return rc;
+    *r = new vsmch_rindex{x};
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_rindex_destroy(vsmch_rindex *r)
+{
+    if (!r)
+        return VSMCH_ERR_INVALID;
+    rindex_destroy(r->r);
+    delete r;
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_rindex_iter_size(vsmch_rindex *r, int64_t *iters)
+{
+    if (!r || !iters)
+        return VSMCH_ERR_INVALID;
+    *iters = r->r->iters;
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_rindex_reset(vsmch_rindex *r)
+{
+    if (!r)
+        return VSMCH_ERR_INVALID;
+    r->r->iters = 0;
+    return VSMCH_OK;
+}
+
+vsmch_status vsmch_rindex_clear(vsmch_rindex *r)
+{
+    if (!r)
+        return VSMCH_ERR_INVALID;
+    return rindex_clear(r->r);
+}
+
+vsmch_status vsmch_rindex_push_back(vsmch_rindex *r, const int *idx)
+{
+    if (!r)
+        return VSMCH_ERR_INVALID;
+    return rindex_push_back(r->r, idx);
+}
+
+vsmch_status vsmch_rindex_insert(vsmch_rindex *r, int64_t iter, const int *idx)
+{
+    if (!r)
+        return VSMCH_ERR_INVALID;
+    return rindex_insert(r->r, iter, idx);
+}
+
+vsmch_status vsmch_rindex_index(vsmch_rindex *r, int64_t id, int64_t iter,
+    int64_t *idx_host)
+{
+    if (!r || !idx_host)
+        return VSMCH_ERR_INVALID;
+    long long v = 0;
+    const int rc = rindex_index(r->r, id, iter, &v);
+    if (rc == VSMCH_OK)
+        *idx_host = v;
+    return rc;
+}
+
+vsmch_status vsmch_rindex_index_matrix(vsmch_rindex *r, int layout, int *out)
+{
+    if (!r || !out || (layout != VSMCH_ROW_MAJOR && layout != VSMCH_COL_MAJOR)) {
+        set_error("vsmch_rindex_index_matrix: invalid argument");
+        return VSMCH_ERR_INVALID;
+    }
+    return rindex_matrix(r->r, layout == VSMCH_COL_MAJOR ? 1 : 0, out);
 }
 
 } // extern "C"

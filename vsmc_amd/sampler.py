@@ -423,3 +423,82 @@ class CvBatch:
                                      ctypes.byref(mb), ctypes.byref(ms)))
         return {"tmax": tm.value, "particles": npart.value, "move_blocks": mb.value,
                 "run_ms": ms.value}
+
+
+class ResampleIndex:
+    """ResampleIndex (include/vsmc/resample/index.hpp:46-206) with the
+    history on the device: row t holds the ancestor index of every particle
+    at iteration t. push_back() with no argument records a step without
+    resampling (the identity); push_back(idx) records the ancestors of a
+    resampling step, e.g. the idx that Particle.resample(..., want_rep=True)
+    returns."""
+
+    def __init__(self, n):
+        self.n = n
+        h = ctypes.c_void_p()
+        check(lib().vsmch_rindex_create(ctypes.byref(h), n))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().vsmch_rindex_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _ptr(idx):
+        if idx is None:
+            return None
+        import torch
+        if idx.dtype != torch.int32 or not idx.is_cuda or not idx.is_contiguous():
+            raise VsmchError("ResampleIndex: idx must be a contiguous int32 CUDA tensor")
+        return ctypes.c_void_p(idx.data_ptr())
+
+    def _check_len(self, idx):
+        if idx is not None and idx.numel() != self.n:
+            raise VsmchError(f"ResampleIndex: idx has {idx.numel()} entries, expected {self.n}")
+
+    def iter_size(self):
+        v = ctypes.c_int64()
+        check(lib().vsmch_rindex_iter_size(self._h, ctypes.byref(v)))
+        return v.value
+
+    def reset(self):
+        check(lib().vsmch_rindex_reset(self._h))
+
+    def clear(self):
+        check(lib().vsmch_rindex_clear(self._h))
+
+    def push_back(self, idx=None):
+        self._check_len(idx)
+        check(lib().vsmch_rindex_push_back(self._h, self._ptr(idx)))
+
+    def insert(self, idx=None, iter=-1):
+        """insert() / insert(first) / insert(iter, first): iter -1 is the
+        last iteration."""
+        self._check_len(idx)
+        check(lib().vsmch_rindex_insert(self._h, iter, self._ptr(idx)))
+
+    def index(self, id, iter):
+        v = ctypes.c_int64()
+        check(lib().vsmch_rindex_index(self._h, id, iter, ctypes.byref(v)))
+        return v.value
+
+    def index_matrix(self, layout=None):
+        """index_matrix<RowMajor> (n x iter_size, row i = particle i's
+        lineage) or <ColMajor> (iter_size x n) as an int32 CUDA tensor."""
+        import torch
+        from . import ROW_MAJOR, COL_MAJOR
+        layout = ROW_MAJOR if layout is None else layout
+        T = self.iter_size()
+        shape = (self.n, T) if layout == ROW_MAJOR else (T, self.n)
+        out = torch.empty(shape, dtype=torch.int32, device="cuda")
+        if T > 0:
+            check(lib().vsmch_rindex_index_matrix(self._h, layout,
+                                                  ctypes.c_void_p(out.data_ptr())))
+        return out
