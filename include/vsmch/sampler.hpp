@@ -743,7 +743,12 @@ class Sampler {
     {
         if (accept_history_.empty())
             accept_history_.push_back(std::vector<std::size_t>());
-        const std::size_t n = move_queue_.size() + mcmc_queue_.size();
+        // initialize() records the init functor in column 0 and its MCMC
+        // functors from column 1 (sampler.hpp:735-744), so with no move
+        // functor one more column than moves + mcmc is needed; the
+        // reference sizes to moves + mcmc and writes past the end there.
+        const std::size_t nmove = move_queue_.empty() ? 1 : move_queue_.size();
+        const std::size_t n = nmove + mcmc_queue_.size();
         while (accept_history_.size() < n)
             accept_history_.push_back(std::vector<std::size_t>());
         for (auto &a : accept_history_)

@@ -32,6 +32,11 @@
  *   PF -- T = 1 log-evidence closed form; linear-Gaussian log-evidence
  *          against vo_kalman_cv; Kalman against a brute-force dense
  *          Gaussian marginal likelihood on T <= 3 (numpy).
+ *   random walk (rw.c) -- flat-target chains against the stream's normals
+ *          built from raw words, transform closed forms and finite-
+ *          difference Jacobians, stationary moments of long chains, word
+ *          counts; the Cholesky summation order is PARITY UNPINNED
+ *          (unvendored CBLAS tpmv in the reference).
  *
  * Floating-point: built with -ffp-contract=off (the reference's x86-64
  * build has no FMA, and u01_sequence.hpp:200,:246 must not be contracted).

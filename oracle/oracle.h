@@ -263,6 +263,33 @@ double vo_cv_learned_particle(const double s0[4], double ox, double oy,
     double sd_vel, int obs_model, double obs_scale, double obs_df,
     double s1[4]);
 
+/* ------------------------------------------------- random walk (rw.c) --
+ * RandomWalk / RandomWalkG / NormalProposal / NormalMVProposal
+ * (include/vsmc/rng/random_walk.hpp:49-555). */
+#define VO_RW_NORMAL 0     /* N(0,1), NormalProposal(1.5), one-step, ltx */
+#define VO_RW_GAMMA 1      /* Gamma(2.5), (0, inf), n-step, ltx */
+#define VO_RW_BETA 2       /* Beta(2, 3), (0, 1), one-step, ltx null */
+#define VO_RW_NEG_GAMMA 3  /* -Gamma(2.5), (-inf, 0), n-step, ltx null */
+#define VO_RW_MV3 4        /* 3-D, NormalMVProposal<3> + bounds, n-step */
+#define VO_RW_BLOCK 5      /* 3-D, block x[1..3) NormalMVProposal<2>, n-step */
+#define VO_RW_FLAT 6       /* flat, NormalProposal(1.3), one-step, ltx null */
+#define VO_RW_FLAT_MV 7    /* flat 3-D, NormalMVProposal<3> chol, one-step */
+#define VO_RW_G 8          /* RandomWalkG, g = (x, x^2), one-step */
+#define VO_RW_BLOCK1 9     /* 3-D, block x[2] NormalProposal (0, inf) */
+#define VO_RW_NSCEN 10
+#define VO_RW_OUT 8        /* x[3], log gamma, accepted, g[2], words */
+/* normal_proposal_q / _qb / _qa / _qab (:378-410) by flag (lower << 1) +
+ * upper; returns log q(y, x) / q(x, y) */
+double vo_rw_q(unsigned flag, double x, double z, double a, double b,
+    double *y);
+unsigned vo_rw_flag(double a, double b);
+int vo_rw_dim(int scenario);
+void vo_rw_init(int scenario, size_t i, double *x);
+/* n particles of a scenario, `iters` moves of `steps` updates each;
+ * out[i * VO_RW_OUT + k] */
+int vo_rw_run(int scenario, uint32_t seed, size_t n, size_t iters,
+    size_t steps, double *out);
+
 #ifdef __cplusplus
 }
 #endif

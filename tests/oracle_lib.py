@@ -94,6 +94,13 @@ def oracle():
                                             c.c_size_t]),
             "vo_rindex_matrix": (None, [c.c_size_t, c.c_size_t, P(np.int64), c.c_int,
                                         P(np.int64)]),
+            "vo_rw_q": (c.c_double, [c.c_uint, c.c_double, c.c_double, c.c_double,
+                                     c.c_double, c.POINTER(c.c_double)]),
+            "vo_rw_flag": (c.c_uint, [c.c_double, c.c_double]),
+            "vo_rw_dim": (c.c_int, [c.c_int]),
+            "vo_rw_init": (None, [c.c_int, c.c_size_t, P(np.float64)]),
+            "vo_rw_run": (c.c_int, [c.c_int, c.c_uint32, c.c_size_t, c.c_size_t, c.c_size_t,
+                                    P(np.float64)]),
             "vo_trans_residual": (c.c_size_t, [c.c_size_t, c.c_size_t, P(np.float64),
                                                P(np.float64), P(np.int64)]),
             "vo_resample": (c.c_uint64, [c.c_int, c.c_size_t, c.c_size_t,
@@ -199,6 +206,37 @@ def rindex_matrix(hist, layout=101):
     out = np.zeros(max(N * T, 1), np.int64)
     oracle().vo_rindex_matrix(N, T, hist, layout, out)
     return out[:N * T].reshape((N, T) if layout == 101 else (T, N))
+
+
+# random walk: scenario ids and the per-particle output record
+RW_NORMAL, RW_GAMMA, RW_BETA, RW_NEG_GAMMA, RW_MV3, RW_BLOCK, RW_FLAT, \
+    RW_FLAT_MV, RW_G, RW_BLOCK1 = range(10)
+RW_NSCEN = 10
+RW_OUT = 8   # x[3], log gamma(x), accepted, g[2], words used
+
+
+def rw_q(flag, x, z, a=-np.inf, b=np.inf):
+    y = c.c_double(0.0)
+    q = oracle().vo_rw_q(flag, x, z, a, b, c.byref(y))
+    return y.value, q
+
+
+def rw_flag(a, b):
+    return int(oracle().vo_rw_flag(a, b))
+
+
+def rw_init(scenario, i):
+    x = np.zeros(4)
+    oracle().vo_rw_init(scenario, i, x)
+    return x[:oracle().vo_rw_dim(scenario)]
+
+
+def rw_run(scenario, seed, n, iters, steps):
+    out = np.zeros((max(n, 1), RW_OUT))
+    rc = oracle().vo_rw_run(scenario, seed, n, iters, steps, out)
+    if rc != 0:
+        raise ValueError(f"unknown random-walk scenario {scenario}")
+    return out[:n]
 
 
 def trans_u01_index(W, u):
