@@ -11,8 +11,6 @@
 
 namespace vsmch {
 
-enum MatrixLayout { RowMajor = VSMCH_ROW_MAJOR, ColMajor = VSMCH_COL_MAJOR };
-
 class ResampleIndex
 {
     public:

@@ -24,6 +24,7 @@
 #include <cstring>
 #include <functional>
 #include <iomanip>
+#include <iterator>
 #include <limits>
 #include <map>
 #include <ostream>
@@ -108,6 +109,9 @@ class Seed {
 template <typename T>
 class Particle;
 
+// MatrixLayout (include/vsmc/internal/defines.hpp), with the C API's values.
+enum MatrixLayout { RowMajor = VSMCH_ROW_MAJOR, ColMajor = VSMCH_COL_MAJOR };
+
 // ColMajor StateMatrix<Dim, RealType> in device memory
 // (include/vsmc/core/state_matrix.hpp:319-428): component pos of particle id
 // at data[pos * N + id].
@@ -140,12 +144,40 @@ class StateHIP {
             "read_state");
     }
 
+    // StateMatrix::print (state_matrix.hpp:172-188): one line per particle,
+    // its components separated by sepchar (copies the state to the host).
+    template <typename CharT, typename Traits>
+    std::basic_ostream<CharT, Traits> &print(
+        std::basic_ostream<CharT, Traits> &os, char sepchar = '\t') const
+    {
+        if (Dim == 0 || n_ == 0 || !os.good())
+            return os;
+        std::vector<RealType> h(Dim * n_);
+        check(vsmch_synchronize(), "StateHIP::print");
+        check_hip(hipMemcpy(h.data(), data_, sizeof(RealType) * h.size(),
+                      hipMemcpyDeviceToHost),
+            "StateHIP::print");
+        for (size_type i = 0; i != n_; ++i) {
+            for (std::size_t d = 0; d + 1 < Dim; ++d)
+                os << h[d * n_ + i] << sepchar;
+            os << h[(Dim - 1) * n_ + i] << '\n';
+        }
+        return os;
+    }
+
     void bind_(RealType *p) { data_ = p; }
 
   private:
     size_type n_;
     RealType *data_ = nullptr;
 };
+
+template <typename CharT, typename Traits, std::size_t Dim, typename RealType>
+inline std::basic_ostream<CharT, Traits> &operator<<(
+    std::basic_ostream<CharT, Traits> &os, const StateHIP<Dim, RealType> &s)
+{
+    return s.print(os);
+}
 
 // SingleParticle<T> on the device: id, state access, the particle's engine.
 template <typename T>
@@ -650,6 +682,104 @@ class Sampler {
         }
         do_acch();
         return *this;
+    }
+
+    // Summary header and data (sampler.hpp:551-648, :786-865): integer
+    // columns Size, Resampled, Accept.k; floating-point columns ESS and each
+    // recording monitor's components (NaN where a monitor did not record).
+    std::size_t summary_header_size_int() const
+    {
+        return iter_size() == 0 ? 0 : accept_history_.size() + 2;
+    }
+    std::size_t summary_header_size() const
+    {
+        if (iter_size() == 0)
+            return 0;
+        std::size_t n = 1;
+        for (const auto &m : monitor_)
+            if (m.second.iter_size() > 0)
+                n += m.second.dim();
+        return n;
+    }
+    template <typename OutputIter>
+    void summary_header_int(OutputIter first) const
+    {
+        if (summary_header_size_int() == 0)
+            return;
+        *first++ = std::string("Size");
+        *first++ = std::string("Resampled");
+        for (std::size_t i = 0; i != accept_history_.size(); ++i)
+            *first++ = "Accept." + std::to_string(i);
+    }
+    template <typename OutputIter>
+    void summary_header(OutputIter first) const
+    {
+        if (summary_header_size() == 0)
+            return;
+        *first++ = std::string("ESS");
+        for (const auto &m : monitor_) {
+            if (m.second.iter_size() == 0)
+                continue;
+            for (std::size_t d = 0; d != m.second.dim(); ++d)
+                *first++ = m.second.name(d).empty()
+                    ? m.first + "." + std::to_string(d)
+                    : m.second.name(d);
+        }
+    }
+    std::size_t summary_data_size_int() const
+    {
+        return summary_header_size_int() * iter_size();
+    }
+    std::size_t summary_data_size() const
+    {
+        return summary_header_size() * iter_size();
+    }
+    template <MatrixLayout Layout, typename OutputIter>
+    void summary_data_int(OutputIter first) const
+    {
+        if (summary_data_size_int() == 0)
+            return;
+        using V = typename std::iterator_traits<OutputIter>::value_type;
+        const std::size_t nc = summary_header_size_int();
+        for (std::size_t r = 0; r != iter_size(); ++r) {
+            const auto put = [&](std::size_t c, std::size_t v) {
+                const std::size_t at = Layout == RowMajor ? r * nc + c : c * iter_size() + r;
+                first[at] = static_cast<V>(v);
+            };
+            put(0, size_history_[r]);
+            put(1, resampled_history_[r] ? 1 : 0);
+            for (std::size_t i = 0; i != accept_history_.size(); ++i)
+                put(2 + i, accept_history_[i][r]);
+        }
+    }
+    template <MatrixLayout Layout, typename OutputIter>
+    void summary_data(OutputIter first) const
+    {
+        if (summary_data_size() == 0)
+            return;
+        const std::size_t nc = summary_header_size();
+        const std::size_t nr = iter_size();
+        const auto at = [&](std::size_t r, std::size_t c) {
+            return Layout == RowMajor ? r * nc + c : c * nr + r;
+        };
+        for (std::size_t r = 0; r != nr; ++r)
+            first[at(r, 0)] = ess_history_[r];
+        std::size_t c0 = 1;
+        for (const auto &m : monitor_) {
+            if (m.second.iter_size() == 0)
+                continue;
+            const std::size_t md = m.second.dim();
+            std::size_t k = 0;
+            for (std::size_t r = 0; r != nr; ++r) {
+                // the r-th row holds the k-th record when it was taken at r
+                const bool has = k != m.second.iter_size() && r == m.second.index(k);
+                for (std::size_t d = 0; d != md; ++d)
+                    first[at(r, c0 + d)] = has ? m.second.record(d, k)
+                                               : std::numeric_limits<double>::quiet_NaN();
+                k += has ? 1 : 0;
+            }
+            c0 += md;
+        }
     }
 
     // sampler.hpp:651-689: tab-separated Size, Resampled, Accept.k, ESS,

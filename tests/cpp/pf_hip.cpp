@@ -8,12 +8,16 @@
 // Build (done by __graft_entry__.build()):
 //   hipcc -O3 -ffp-contract=off --offload-arch=gfx950 -Iinclude \
 //     tests/cpp/pf_hip.cpp -Lvsmc_amd -lvsmch -o tests/cpp/pf_hip
-// Usage: pf_hip N T scheme threshold seed obs_file
+// Usage: pf_hip N T scheme threshold seed obs_file [full]
+// With "full", also prints the summary header / data in both layouts
+// (lines SHI, SDIR, SDIC, SH, SDR, SDC) and the final state (STATE, then N
+// lines from the state's operator<<).
 #include <vsmch/sampler.hpp>
 
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
+#include <string>
 #include <vector>
 
 static const std::size_t PosX = 0;
@@ -186,6 +190,31 @@ int main(int argc, char **argv)
         for (std::size_t t = 0; t != T; ++t)
             std::cout << '\t' << sampler.log_zconst_history(t);
         std::cout << '\n';
+        if (argc > 7 && std::string(argv[7]) == "full") {
+            std::vector<std::string> hi(sampler.summary_header_size_int());
+            std::vector<std::string> hf(sampler.summary_header_size());
+            sampler.summary_header_int(hi.begin());
+            sampler.summary_header(hf.begin());
+            std::vector<long long> di(sampler.summary_data_size_int());
+            std::vector<double> df(sampler.summary_data_size());
+            const auto line = [](const char *tag, const auto &v) {
+                std::cout << tag;
+                for (const auto &x : v)
+                    std::cout << '\t' << x;
+                std::cout << '\n';
+            };
+            line("SHI", hi);
+            sampler.summary_data_int<vsmch::RowMajor>(di.begin());
+            line("SDIR", di);
+            sampler.summary_data_int<vsmch::ColMajor>(di.begin());
+            line("SDIC", di);
+            line("SH", hf);
+            sampler.summary_data<vsmch::RowMajor>(df.begin());
+            line("SDR", df);
+            sampler.summary_data<vsmch::ColMajor>(df.begin());
+            line("SDC", df);
+            std::cout << "STATE\n" << sampler.particle().value();
+        }
     } catch (const std::exception &e) {
         std::cerr << "pf_hip: " << e.what() << '\n';
         return 1;
