@@ -1304,25 +1304,24 @@ __device__ __forceinline__ void expand_chunk(long long chunk, long long K, long 
     const int *zdst, const int *cE, const int *csrc, int *idx, E *state,
     int dim, long long ld);
 
+// With end.pd, block 0 also applies the event's end to the particle set
+// (after_event on eprm): nothing in this kernel reads what it writes.
 template <bool kExpand, typename E>
 __global__ void __launch_bounds__(kBlock) k_expand(const ResampleDev *prm,
     const long long *toff, long long ntiles, const int *zdst, const int *cE,
-    const int *csrc, int *idx, E *state, int dim, long long ld)
+    const int *csrc, int *idx, E *state, int dim, long long ld,
+    ResampleDev *eprm, EventEnd end)
 {
     if (prm && !prm->active)
         return;
+    if (end.pd && blockIdx.x == 0 && threadIdx.x == 0)
+        after_event(end.pd, eprm, end.n, end.rep, end.scheme, end.words);
     const long long K = kExpand ? toff[3 * ntiles + 1] : toff[3 * ntiles + 0];
     const long long C = toff[3 * ntiles + 2];
     if (static_cast<long long>(blockIdx.x) * kTile >= K || C <= 0)
         return;
     expand_chunk<kExpand, E>(blockIdx.x, K, C, zdst, cE, csrc, idx, state, dim,
         ld);
-}
-
-// The particle set's update once the event's gather is done (one thread).
-__global__ void k_event_end(ResampleDev *prm, EventEnd end)
-{
-    after_event(end.pd, prm, end.n, end.rep, end.scheme, end.words);
 }
 
 // Ranks [chunk kTile, (chunk + 1) kTile) of the K ranks: the source of each
@@ -2041,21 +2040,20 @@ int trans_rep_index(Workspace &ws, long long M, long long N, const int *rep,
             0, st, rep, M, prm, ws.toff, ws.zdst, ws.cE, ws.csrc, idx);
     }
     const unsigned int ge = grid_tiles(N);
+    // the gather's block 0 ends the event (one launch fewer per step)
     if (expand) {
         hipLaunchKernelGGL((k_expand<true, float>), dim3(ge), dim3(kBlock), 0,
             st, prm, ws.toff, ntiles, ws.zdst, ws.cE, ws.csrc, idx,
-            static_cast<float *>(nullptr), 0, 0LL);
+            static_cast<float *>(nullptr), 0, 0LL, ws.prm, end);
     } else if (state == nullptr || elem_bytes == 4) {
         hipLaunchKernelGGL((k_expand<false, float>), dim3(ge), dim3(kBlock), 0,
             st, prm, ws.toff, ntiles, ws.zdst, ws.cE, ws.csrc, idx,
-            static_cast<float *>(state), dim, ld);
+            static_cast<float *>(state), dim, ld, ws.prm, end);
     } else {
         hipLaunchKernelGGL((k_expand<false, double>), dim3(ge), dim3(kBlock),
             0, st, prm, ws.toff, ntiles, ws.zdst, ws.cE, ws.csrc, idx,
-            static_cast<double *>(state), dim, ld);
+            static_cast<double *>(state), dim, ld, ws.prm, end);
     }
-    if (end.pd)
-        hipLaunchKernelGGL(k_event_end, dim3(1), dim3(1), 0, st, ws.prm, end);
     VSMCH_HIP(hipGetLastError());
     return kOk;
 }
