@@ -199,6 +199,21 @@ vsmch_status vsmch_particle_resample_count(vsmch_particle *p,
 vsmch_status vsmch_monitor_state_moments(vsmch_particle *p, int c0, int dim,
     double *r_host);
 
+/* Adaptive tempering. No counterpart in the reference, which anneals on a
+ * fixed schedule (example/gmm/include/gmm.hpp:567-604); the search the
+ * mixture sampler (vsmch_gmm_*) runs, for any target whose per-particle
+ * log-likelihoods ll (device, n doubles; finite or -inf) the caller holds.
+ * Conditional ESS of the increment d >= 0 under the current weights:
+ * (sum_i W_i e^{d ll_i})^2 / sum_i W_i e^{2 d ll_i}, in (0, 1].
+ * Synchronous. */
+vsmch_status vsmch_particle_cess(vsmch_particle *p, const double *ll,
+    double d, double *cess_host);
+/* The largest d in (0, dmax] with CESS(d) >= target (0 < target <= 1):
+ * dmax itself when it qualifies, else found to within dmax / 16^rounds
+ * (1 <= rounds <= 64) with CESS(d) >= target. Synchronous. */
+vsmch_status vsmch_particle_cess_solve(vsmch_particle *p, const double *ll,
+    double target, double dmax, int rounds, double *d_host);
+
 /* ------------------------------------------- constant-velocity PF model */
 
 /* The constant-velocity bootstrap PF of user_guide/cpp/pf_seq.cpp with the

@@ -148,7 +148,8 @@ double vo_cess(int64_t n, const double *W, const double *ll, double d)
         m = ll[i] > m ? ll[i] : m;
     double s1 = 0, s2 = 0;
     for (int64_t i = 0; i < n; ++i) {
-        const double e = exp(d * (ll[i] - m));
+        /* e^{0 ll} = 1, also for ll = -inf */
+        const double e = d == 0 ? 1.0 : exp(d * (ll[i] - m));
         s1 += W[i] * e;
         s2 += W[i] * e * e;
     }

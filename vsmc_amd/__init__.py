@@ -79,6 +79,9 @@ SIGNATURES = {
     "vsmch_weighted_sum": (_c.c_int, [_P, _c.c_int, _P, _c.POINTER(_c.c_double)]),
     "vsmch_monitor_state_moments": (_c.c_int, [_P, _c.c_int, _c.c_int,
                                                _c.POINTER(_c.c_double)]),
+    "vsmch_particle_cess": (_c.c_int, [_P, _P, _c.c_double, _c.POINTER(_c.c_double)]),
+    "vsmch_particle_cess_solve": (_c.c_int, [_P, _P, _c.c_double, _c.c_double, _c.c_int,
+                                             _c.POINTER(_c.c_double)]),
     "vsmch_cv_init": (_c.c_int, [_P, _P, _c.c_double, _c.c_double]),
     "vsmch_cv_move": (_c.c_int, [_P, _P, _c.c_int64, _c.c_double, _c.c_double]),
     "vsmch_cv_run": (_c.c_int, [_P, _P, _c.c_int64, _P, _P, _c.c_int, _c.c_double,

@@ -473,6 +473,22 @@ vsmch_status vsmch_monitor_state_moments(vsmch_particle *p, int c0, int dim,
     return VSMCH_OK;
 }
 
+vsmch_status vsmch_particle_cess(vsmch_particle *p, const double *ll,
+    double d, double *cess_host)
+{
+    if (!p || !ll || !cess_host)
+        return VSMCH_ERR_INVALID;
+    return particle_cess(p->p, ll, d, cess_host);
+}
+
+vsmch_status vsmch_particle_cess_solve(vsmch_particle *p, const double *ll,
+    double target, double dmax, int rounds, double *d_host)
+{
+    if (!p || !ll || !d_host)
+        return VSMCH_ERR_INVALID;
+    return particle_cess_solve(p->p, ll, target, dmax, rounds, d_host);
+}
+
 vsmch_status vsmch_cv_init(vsmch_particle *p, const vsmch_cv_model *m,
     double obs_x, double obs_y)
 {

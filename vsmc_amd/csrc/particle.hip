@@ -956,7 +956,7 @@ void particle_destroy(ParticleSet *p)
     if (!p)
         return;
     void *ps[] = {p->state, p->logw, p->scratch, p->rep, p->part, p->mpart,
-        p->mout, p->pd, p->rng_words};
+        p->mout, p->pd, p->rng_words, p->cess_buf};
     for (void *q : ps)
         if (q)
             (void)hipFree(q);

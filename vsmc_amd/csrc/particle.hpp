@@ -74,6 +74,7 @@ struct ParticleSet {
     PDev *pd = nullptr;
     unsigned long long *rng_words = nullptr;  // allocated on first request
     int *host_active = nullptr;  // pinned: the decision read by the host
+    void *cess_buf = nullptr;    // conditional-ESS search (temper.hip)
     Workspace ws;
 };
 
@@ -166,6 +167,13 @@ int gmm_set_alpha(GmmSet *s, double alpha);
 int gmm_run(GmmSet *s, long long max_steps, double *alpha_d, double *ess_d,
     int *rs_d, double *logz_d, double *integ_d, double *acc_d,
     long long *steps_out);
+
+// temper.hip -- conditional ESS of an increment d of the log-likelihoods
+// ll (device, n doubles) and the largest d in (0, dmax] that keeps it at
+// or above target (kCand-point search, `rounds` passes after the first)
+int particle_cess(ParticleSet *p, const double *ll, double d, double *out);
+int particle_cess_solve(ParticleSet *p, const double *ll, double target,
+    double dmax, int rounds, double *out);
 
 // rng.hip
 int stream_fill(int kind, unsigned long long key, unsigned long long offset,

@@ -121,6 +121,24 @@ class Particle:
         check(lib().vsmch_particle_resample_count(self._h, ctypes.byref(v)))
         return v.value
 
+    def cess(self, ll, d):
+        """Conditional ESS of the increment d of the log-likelihoods ll (one
+        per particle) under the current weights."""
+        ll = ll.to(device="cuda", dtype=_f64()).contiguous()
+        v = ctypes.c_double()
+        check(lib().vsmch_particle_cess(self._h, ctypes.c_void_p(ll.data_ptr()), float(d),
+                                        ctypes.byref(v)))
+        return v.value
+
+    def cess_solve(self, ll, target, dmax, rounds=10):
+        """The largest increment d in (0, dmax] with cess(ll, d) >= target."""
+        ll = ll.to(device="cuda", dtype=_f64()).contiguous()
+        v = ctypes.c_double()
+        check(lib().vsmch_particle_cess_solve(self._h, ctypes.c_void_p(ll.data_ptr()),
+                                              float(target), float(dmax), int(rounds),
+                                              ctypes.byref(v)))
+        return v.value
+
     def moments(self, c0, dim):
         """Monitor::eval: sum_i W_i x_d(i) for components c0..c0+dim-1."""
         out = (ctypes.c_double * dim)()
