@@ -113,6 +113,34 @@ vsmch_status vsmch_resample(int m, int n, uint32_t key0,
     uint64_t *word_offset_host, const double *weight, int *replication,
     vsmch_resample_scheme scheme);
 
+/* vsmc_resample_<scheme> (include/vsmc/vsmc.h:297-308): vsmch_resample
+ * with the scheme fixed. */
+vsmch_status vsmch_resample_multinomial(int m, int n, uint32_t key0,
+    uint64_t *word_offset_host, const double *weight, int *replication);
+vsmch_status vsmch_resample_stratified(int m, int n, uint32_t key0,
+    uint64_t *word_offset_host, const double *weight, int *replication);
+vsmch_status vsmch_resample_systematic(int m, int n, uint32_t key0,
+    uint64_t *word_offset_host, const double *weight, int *replication);
+vsmch_status vsmch_resample_residual(int m, int n, uint32_t key0,
+    uint64_t *word_offset_host, const double *weight, int *replication);
+vsmch_status vsmch_resample_residual_stratified(int m, int n, uint32_t key0,
+    uint64_t *word_offset_host, const double *weight, int *replication);
+vsmch_status vsmch_resample_residual_systematic(int m, int n, uint32_t key0,
+    uint64_t *word_offset_host, const double *weight, int *replication);
+
+/* U01 sequences of caller uniforms (device arrays of n doubles).
+ * vsmc_rng_u01_sorted (include/vsmc/vsmc.h:76-77, u01_sequence.hpp:54-62):
+ * the sums of log(u_i) / (n - i) in order, u01seq[i] = 1 - exp(sum); the
+ * sums are the exact sequential ones, the logs within an ulp of libm's. */
+vsmch_status vsmch_u01_sorted(int n, const double *u01, double *u01seq);
+/* vsmc_rng_u01_stratified (include/vsmc/vsmc.h:79-80,
+ * u01_sequence.hpp:67-74): u01[i] / n + i / n with delta = 1 / n. */
+vsmch_status vsmch_u01_stratified(int n, const double *u01, double *u01seq);
+/* vsmc_rng_u01_systematic (include/vsmc/vsmc.h:82-83,
+ * u01_sequence.hpp:80-89): u01seq[0] = u01 / n, then repeated additions of
+ * 1 / n, bit-equal to the sequential loop. */
+vsmch_status vsmch_u01_systematic(int n, double u01, double *u01seq);
+
 /* Test hooks (no reference counterpart): the exact sequential prefix sums
  * out[i] = (((w0 + w1) + w2) + ... + wi) and the U01SequenceSorted values
  * (include/vsmc/rng/u01_sequence.hpp:130-164) of a Philox stream. */

@@ -207,10 +207,30 @@ struct normal_pair {
 
 // Round-to-nearest add and multiply that the compiler may not contract into
 // an FMA: the reference's host build evaluates a * b + c with two roundings.
-__device__ inline double rn_add(double a, double b) { return __dadd_rn(a, b); }
-__device__ inline float rn_add(float a, float b) { return __fadd_rn(a, b); }
-__device__ inline double rn_mul(double a, double b) { return __dmul_rn(a, b); }
-__device__ inline float rn_mul(float a, float b) { return __fmul_rn(a, b); }
+// HIP's __dadd_rn / __dmul_rn are plain + and *, so the operations here
+// carry no contraction permission of their own (HIP's default contraction
+// mode honours the pragma; translation units of the library are also built
+// with -ffp-contract=off).
+__device__ inline double rn_add(double a, double b)
+{
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ inline float rn_add(float a, float b)
+{
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ inline double rn_mul(double a, double b)
+{
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ inline float rn_mul(float a, float b)
+{
+#pragma clang fp contract(off)
+    return a * b;
+}
 
 __device__ inline void sincos_t(double a, double *s, double *c)
 {

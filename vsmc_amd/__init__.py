@@ -79,6 +79,21 @@ SIGNATURES = {
     "vsmch_weighted_sum": (_c.c_int, [_P, _c.c_int, _P, _c.POINTER(_c.c_double)]),
     "vsmch_monitor_state_moments": (_c.c_int, [_P, _c.c_int, _c.c_int,
                                                _c.POINTER(_c.c_double)]),
+    "vsmch_resample_multinomial": (_c.c_int, [_c.c_int, _c.c_int, _c.c_uint32,
+                                              _c.POINTER(_c.c_uint64), _P, _P]),
+    "vsmch_resample_stratified": (_c.c_int, [_c.c_int, _c.c_int, _c.c_uint32,
+                                             _c.POINTER(_c.c_uint64), _P, _P]),
+    "vsmch_resample_systematic": (_c.c_int, [_c.c_int, _c.c_int, _c.c_uint32,
+                                             _c.POINTER(_c.c_uint64), _P, _P]),
+    "vsmch_resample_residual": (_c.c_int, [_c.c_int, _c.c_int, _c.c_uint32,
+                                           _c.POINTER(_c.c_uint64), _P, _P]),
+    "vsmch_resample_residual_stratified": (_c.c_int, [_c.c_int, _c.c_int, _c.c_uint32,
+                                                      _c.POINTER(_c.c_uint64), _P, _P]),
+    "vsmch_resample_residual_systematic": (_c.c_int, [_c.c_int, _c.c_int, _c.c_uint32,
+                                                      _c.POINTER(_c.c_uint64), _P, _P]),
+    "vsmch_u01_sorted": (_c.c_int, [_c.c_int, _P, _P]),
+    "vsmch_u01_stratified": (_c.c_int, [_c.c_int, _P, _P]),
+    "vsmch_u01_systematic": (_c.c_int, [_c.c_int, _c.c_double, _P]),
     "vsmch_particle_cess": (_c.c_int, [_P, _P, _c.c_double, _c.POINTER(_c.c_double)]),
     "vsmch_particle_cess_solve": (_c.c_int, [_P, _P, _c.c_double, _c.c_double, _c.c_int,
                                              _c.POINTER(_c.c_double)]),
@@ -318,6 +333,51 @@ def sorted_uniforms(n, key, word_offset=0, device="cuda"):
     import torch
     out = torch.empty(max(n, 1), dtype=torch.float64, device=device)
     check(lib().vsmch_sorted_uniforms(n, key, word_offset, _ptr(out)))
+    _stream_sync()
+    return out[:n]
+
+
+_SCHEME_NAMES = ("multinomial", "stratified", "systematic", "residual",
+                 "residual_stratified", "residual_systematic")
+
+
+def resample_scheme(name, weight, n, key, word_offset=0, m=None):
+    """vsmc_resample_<name>(m, n, rng, W, rep): resample with the scheme in
+    the entry point's name; returns (rep, new word offset)."""
+    import torch
+    if name not in _SCHEME_NAMES:
+        raise ValueError(f"unknown resampling scheme {name!r}")
+    m = weight.numel() if m is None else m
+    rep = torch.empty(max(m, 1), dtype=torch.int32, device=weight.device)
+    off = ctypes.c_uint64(word_offset)
+    check(getattr(lib(), "vsmch_resample_" + name)(m, n, key, ctypes.byref(off),
+                                                    _ptr(weight), _ptr(rep)))
+    return rep[:m], off.value
+
+
+def u01_sorted(u01):
+    """u01_sorted of n uniforms (include/vsmc/rng/u01_sequence.hpp:54-62)."""
+    import torch
+    out = torch.empty(max(u01.numel(), 1), dtype=torch.float64, device=u01.device)
+    check(lib().vsmch_u01_sorted(u01.numel(), _ptr(u01), _ptr(out)))
+    _stream_sync()
+    return out[:u01.numel()]
+
+
+def u01_stratified(u01):
+    """u01_stratified of n uniforms (include/vsmc/rng/u01_sequence.hpp:67-74)."""
+    import torch
+    out = torch.empty(max(u01.numel(), 1), dtype=torch.float64, device=u01.device)
+    check(lib().vsmch_u01_stratified(u01.numel(), _ptr(u01), _ptr(out)))
+    _stream_sync()
+    return out[:u01.numel()]
+
+
+def u01_systematic(n, u01, device="cuda"):
+    """u01_systematic of one uniform (include/vsmc/rng/u01_sequence.hpp:80-89)."""
+    import torch
+    out = torch.empty(max(n, 1), dtype=torch.float64, device=device)
+    check(lib().vsmch_u01_systematic(n, float(u01), _ptr(out)))
     _stream_sync()
     return out[:n]
 

@@ -18,6 +18,10 @@ FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # packed FP32 instructions are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # a * b + c stays two roundings, as in the reference's x86-64 build:
+    # __dmul_rn / __dadd_rn are plain * and + in HIP's headers, which the
+    # default contraction would fuse into an FMA
+    "-ffp-contract=off",
     "-Wall", "-Wshadow", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 

@@ -231,6 +231,48 @@ vsmch_status vsmch_resample(int m, int n, uint32_t key0,
     return VSMCH_OK;
 }
 
+#define VSMCH_RESAMPLE_SCHEME(name, scheme)                                    \
+    vsmch_status vsmch_resample_##name(int m, int n, uint32_t key0,            \
+        uint64_t *word_offset_host, const double *weight, int *replication)    \
+    {                                                                          \
+        return vsmch_resample(m, n, key0, word_offset_host, weight,            \
+            replication, scheme);                                              \
+    }
+VSMCH_RESAMPLE_SCHEME(multinomial, VSMCH_RESAMPLE_MULTINOMIAL)
+VSMCH_RESAMPLE_SCHEME(stratified, VSMCH_RESAMPLE_STRATIFIED)
+VSMCH_RESAMPLE_SCHEME(systematic, VSMCH_RESAMPLE_SYSTEMATIC)
+VSMCH_RESAMPLE_SCHEME(residual, VSMCH_RESAMPLE_RESIDUAL)
+VSMCH_RESAMPLE_SCHEME(residual_stratified, VSMCH_RESAMPLE_RESIDUAL_STRATIFIED)
+VSMCH_RESAMPLE_SCHEME(residual_systematic, VSMCH_RESAMPLE_RESIDUAL_SYSTEMATIC)
+#undef VSMCH_RESAMPLE_SCHEME
+
+vsmch_status vsmch_u01_sorted(int n, const double *u01, double *u01seq)
+{
+    if (n < 0 || (n > 0 && (!u01 || !u01seq))) {
+        set_error("vsmch_u01_sorted: invalid argument");
+        return VSMCH_ERR_INVALID;
+    }
+    return u01_sorted_seq(ws_current(), n, u01, u01seq);
+}
+
+vsmch_status vsmch_u01_stratified(int n, const double *u01, double *u01seq)
+{
+    if (n < 0 || (n > 0 && (!u01 || !u01seq))) {
+        set_error("vsmch_u01_stratified: invalid argument");
+        return VSMCH_ERR_INVALID;
+    }
+    return u01_stratified_seq(ws_current(), n, u01, u01seq);
+}
+
+vsmch_status vsmch_u01_systematic(int n, double u01, double *u01seq)
+{
+    if (n < 0 || (n > 0 && !u01seq)) {
+        set_error("vsmch_u01_systematic: invalid argument");
+        return VSMCH_ERR_INVALID;
+    }
+    return u01_systematic_seq(ws_current(), n, u01, u01seq);
+}
+
 vsmch_status vsmch_exact_cumsum(size_t n, const double *w, double *out)
 {
     if (n > 0 && (!w || !out)) {

@@ -185,6 +185,10 @@ int sorted_uniforms(Workspace &ws, long long ndraw, unsigned int key,
 
 // Exact sequential prefix sums (for tests): out[i] = fl(...fl(w0 + w1)...)
 int exact_cumsum(Workspace &ws, long long n, const double *w, double *out);
+// u01_sorted / u01_stratified / u01_systematic over caller uniforms
+int u01_sorted_seq(Workspace &ws, long long n, const double *u, double *out);
+int u01_stratified_seq(Workspace &ws, long long n, const double *u, double *out);
+int u01_systematic_seq(Workspace &ws, long long n, double u, double *out);
 
 // One resampling event for every filter of a batch whose prm[f].active is
 // set (one workgroup per filter); weights from the log-weights l with

@@ -316,15 +316,15 @@ __global__ void k_weight_read(const double *l, long long n, const PDev *pd,
 template <typename Real>
 __device__ inline Real radd(Real a, Real b);
 template <>
-__device__ inline double radd<double>(double a, double b) { return __dadd_rn(a, b); }
+__device__ inline double radd<double>(double a, double b) { return rn_add(a, b); }
 template <>
-__device__ inline float radd<float>(float a, float b) { return __fadd_rn(a, b); }
+__device__ inline float radd<float>(float a, float b) { return rn_add(a, b); }
 template <typename Real>
 __device__ inline Real rmul(Real a, Real b);
 template <>
-__device__ inline double rmul<double>(double a, double b) { return __dmul_rn(a, b); }
+__device__ inline double rmul<double>(double a, double b) { return rn_mul(a, b); }
 template <>
-__device__ inline float rmul<float>(float a, float b) { return __fmul_rn(a, b); }
+__device__ inline float rmul<float>(float a, float b) { return rn_mul(a, b); }
 
 // log-likelihood of example/pf/include/pf.hpp:78-87 (Student-t with the
 // reference's dropped constants), optionally with the constants, or

@@ -173,11 +173,11 @@ struct USys {
     const ResampleDev *p;
     __device__ double u(long long j) const
     {
-        return __dadd_rn(p->u0, __dmul_rn(static_cast<double>(j), p->delta));
+        return rn_add(p->u0, rn_mul(static_cast<double>(j), p->delta));
     }
     __device__ double ui(int j) const
     {
-        return __dadd_rn(p->u0, __dmul_rn(static_cast<double>(j), p->delta));
+        return rn_add(p->u0, rn_mul(static_cast<double>(j), p->delta));
     }
     __device__ long long ndraw() const { return p->ndraw; }
     // Number of draws u_j <= x. Draw counts are below 2^31 (particle
@@ -213,8 +213,8 @@ struct UStrat {
     __device__ double u(long long j) const
     {
         const double U = u01_u32d(philox_word(p->key, p->word0 + j));
-        return __dadd_rn(__dmul_rn(U, p->delta),
-            __dmul_rn(static_cast<double>(j), p->delta));
+        return rn_add(rn_mul(U, p->delta),
+            rn_mul(static_cast<double>(j), p->delta));
     }
     __device__ double v(long long j) const
     {
@@ -228,8 +228,8 @@ struct UStrat {
     {
         const double U = u01_u32d(philox_word(p->key,
             p->word0 + static_cast<unsigned long long>(j)));
-        return __dadd_rn(__dmul_rn(U, p->delta),
-            __dmul_rn(static_cast<double>(j), p->delta));
+        return rn_add(rn_mul(U, p->delta),
+            rn_mul(static_cast<double>(j), p->delta));
     }
     __device__ double vi(int j) const
     {
@@ -1890,6 +1890,85 @@ int sorted_uniforms(Workspace &ws, long long ndraw, unsigned int key,
     if (rc != kOk)
         return rc;
     return check_err(ws);
+}
+
+// u01_sorted over the caller's uniforms (u01_sequence.hpp:54-62): the
+// terms -log(u_i) / (N - i) folded in order, s_i -> 1 - exp(-s_i) (the
+// reference's lmax = -s_i, rounded the same way by symmetry).
+struct WLogUArr {
+    const double *u;
+    long long n;
+    __device__ double operator()(long long k) const
+    {
+        return -(log(u[k]) / static_cast<double>(n - k));
+    }
+    __device__ long long integ(long long) const { return 0; }
+    __device__ WLogUArr loaded() const { return *this; }
+};
+
+// u01_systematic (u01_sequence.hpp:80-89): u_0 = u delta and u_i = u_{i-1}
+// + delta, the sequential sums of (u delta, delta, delta, ...).
+struct WSysSteps {
+    double first;
+    double delta;
+    __device__ double operator()(long long k) const { return k == 0 ? first : delta; }
+    __device__ long long integ(long long) const { return 0; }
+    __device__ WSysSteps loaded() const { return *this; }
+};
+
+// u01_stratified (u01_sequence.hpp:67-74): u_i delta + i delta, no FMA;
+// delta = 1 / n rounded on the host.
+__global__ void k_u01_stratified(long long n, double delta, const double *u,
+    double *out)
+{
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    out[i] = rn_add(rn_mul(u[i], delta), rn_mul(static_cast<double>(i), delta));
+}
+
+template <class WSrc, class Epi>
+static int fold_prefix(Workspace &ws, long long n, WSrc src, Epi epi)
+{
+    if (!ws.ensure(n))
+        return kErrAlloc;
+    int rc = setup_host(ws, n, n, 0, 0, kMultinomial);
+    if (rc != kOk)
+        return rc;
+    rc = run_fold(ws, src, n, nullptr);
+    if (rc != kOk)
+        return rc;
+    hipLaunchKernelGGL((k_prefix<WSrc, Epi>), dim3(grid_tiles(n)), dim3(kBlock),
+        0, ws.stream, src, n, static_cast<const long long *>(nullptr), ws.prm,
+        fold_bufs(ws), epi);
+    VSMCH_HIP(hipGetLastError());
+    return check_err(ws);
+}
+
+int u01_sorted_seq(Workspace &ws, long long n, const double *u, double *out)
+{
+    if (n <= 0)
+        return kOk;
+    return fold_prefix(ws, n, WLogUArr{u, n}, EpiSorted{out});
+}
+
+int u01_stratified_seq(Workspace &ws, long long n, const double *u, double *out)
+{
+    if (n <= 0)
+        return kOk;
+    const double delta = 1 / static_cast<double>(n);
+    hipLaunchKernelGGL(k_u01_stratified, dim3(static_cast<unsigned>((n + 255) / 256)),
+        dim3(256), 0, ws.stream, n, delta, u, out);
+    VSMCH_HIP(hipGetLastError());
+    return kOk;
+}
+
+int u01_systematic_seq(Workspace &ws, long long n, double u, double *out)
+{
+    if (n <= 0)
+        return kOk;
+    const double delta = 1.0 / static_cast<double>(n);
+    return fold_prefix(ws, n, WSysSteps{u * delta, delta}, EpiRaw{out});
 }
 
 template <class Base>

@@ -151,9 +151,9 @@ __global__ void __launch_bounds__(256) k_normal_batch(unsigned long long key,
         // odd total: r[n-1] from words offset+n, offset+n+1
         const double u = u01_at<kKind>(key, offset + n);
         const double v = u01_at<kKind>(key, offset + n + 1);
-        const double r = __dmul_rn(sd, sqrt(__dmul_rn(-2.0, log(u))));
-        out[n - 1] = __dadd_rn(mean,
-            __dmul_rn(r, cos(__dmul_rn(two_pi<double>(), v))));
+        const double r = rn_mul(sd, sqrt(rn_mul(-2.0, log(u))));
+        out[n - 1] = rn_add(mean,
+            rn_mul(r, cos(rn_mul(two_pi<double>(), v))));
     }
     if (i0 >= nu)
         return;
@@ -174,14 +174,14 @@ __global__ void __launch_bounds__(256) k_normal_batch(unsigned long long key,
         if (i >= nu)
             break;
         double s = log(ur[t]);
-        s = __dmul_rn(-2.0, s);
+        s = rn_mul(-2.0, s);
         s = sqrt(s);
-        const double a = __dmul_rn(two_pi<double>(), ua[t]);
+        const double a = rn_mul(two_pi<double>(), ua[t]);
         double sn, cs;
         sincos(a, &sn, &cs);
-        s = __dmul_rn(sd, s);
-        out[c0 + i] = __dadd_rn(__dmul_rn(s, sn), mean);
-        out[c0 + nu + i] = __dadd_rn(__dmul_rn(s, cs), mean);
+        s = rn_mul(sd, s);
+        out[c0 + i] = rn_add(rn_mul(s, sn), mean);
+        out[c0 + nu + i] = rn_add(rn_mul(s, cs), mean);
     }
 }
 
@@ -198,13 +198,13 @@ __global__ void __launch_bounds__(256) k_normal_scalar(unsigned long long key,
         return;
     const double ua = u01_at<kKind>(key, offset + 2 * p);
     const double ub = u01_at<kKind>(key, offset + 2 * p + 1);
-    const double r = sqrt(__dmul_rn(-2.0, log(ua)));
-    const double a = __dmul_rn(two_pi<double>(), ub);
+    const double r = sqrt(rn_mul(-2.0, log(ua)));
+    const double a = rn_mul(two_pi<double>(), ub);
     double sn, cs;
     sincos(a, &sn, &cs);
-    out[2 * p] = __dadd_rn(mean, __dmul_rn(sd, __dmul_rn(r, cs)));
+    out[2 * p] = rn_add(mean, rn_mul(sd, rn_mul(r, cs)));
     if (2 * p + 1 < n)
-        out[2 * p + 1] = __dadd_rn(mean, __dmul_rn(sd, __dmul_rn(r, sn)));
+        out[2 * p + 1] = rn_add(mean, rn_mul(sd, rn_mul(r, sn)));
 }
 
 int stream_fill(int kind, unsigned long long key, unsigned long long offset,
