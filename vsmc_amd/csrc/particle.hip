@@ -100,34 +100,62 @@ __global__ void __launch_bounds__(kPB) k_weight_pass(double *l,
 
 // Merges block partials; mode 0: set_log, 1: add_log, 2: set_equal.
 // Also prepares the resampling parameters of this step (weights from l)
-// and records the history entry `t` when hist is non-null.
+// and records the history entry `t` when hist is non-null. kThreads: the
+// block size it runs with.
+template <int kThreads = kPB>
 __device__ inline void weight_merge(const Part3 *part, long long nblocks,
     long long n, int mode, PDev *pd);
 
-__global__ void __launch_bounds__(kPB) k_weight_finalize(const Part3 *part,
+// One block of kMergeThreads: four times the threads of a particle block,
+// so each thread merges a quarter of the partials (the merge is a chain of
+// dependent loads and exponentials per thread, latency-bound).
+constexpr int kMergeThreads = 1024;
+
+__global__ void __launch_bounds__(kMergeThreads) k_weight_finalize(const Part3 *part,
     long long nblocks, long long n, int mode, PDev *pd)
 {
-    weight_merge(part, nblocks, n, mode, pd);
+    weight_merge<kMergeThreads>(part, nblocks, n, mode, pd);
 }
 
-// The body of k_weight_finalize (one block).
+// Block reduction over kThreads threads, the result broadcast (lds: one
+// entry per wave).
+template <int kThreads, typename T, typename Op>
+__device__ inline T block_reduce_n(T v, Op op, T *lds)
+{
+    constexpr int nw = kThreads / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    v = wave_reduce(v, op);
+    if (lane == 0)
+        lds[wave] = v;
+    __syncthreads();
+    T r = lds[0];
+#pragma unroll
+    for (int w = 1; w < nw; ++w)
+        r = op(r, lds[w]);
+    __syncthreads();
+    return r;
+}
+
+// The body of k_weight_finalize (one block of kThreads).
+template <int kThreads>
 __device__ inline void weight_merge(const Part3 *part, long long nblocks,
     long long n, int mode, PDev *pd)
 {
-    __shared__ double dlds[kWaves];
+    __shared__ double dlds[kThreads / kWave];
     Part3 acc{-INFINITY, 0.0, 0.0};
     if (mode != 2) {
         // the maximum first, then independent scaled sums (no serial chain
         // of exponentials through a running maximum)
         double mx = -INFINITY;
-        for (long long b = threadIdx.x; b < nblocks; b += kPB) {
+        for (long long b = threadIdx.x; b < nblocks; b += kThreads) {
             const double m = part[b].m;
             mx = m > mx ? m : mx;
         }
-        mx = block_reduce(mx, OpMaxD(), dlds);
+        mx = block_reduce_n<kThreads>(mx, OpMaxD(), dlds);
         double s = 0.0, q = 0.0;
         if (mx > -INFINITY) {
-            for (long long b = threadIdx.x; b < nblocks; b += kPB) {
+            for (long long b = threadIdx.x; b < nblocks; b += kThreads) {
                 const Part3 p = part[b];
                 if (p.m > -INFINITY) {
                     const double r = exp(p.m - mx);
@@ -137,8 +165,8 @@ __device__ inline void weight_merge(const Part3 *part, long long nblocks,
             }
         }
         acc = Part3{mx, s, q};
-        acc.s = block_reduce(acc.s, OpAddD(), dlds);
-        acc.q = block_reduce(acc.q, OpAddD(), dlds);
+        acc.s = block_reduce_n<kThreads>(acc.s, OpAddD(), dlds);
+        acc.q = block_reduce_n<kThreads>(acc.q, OpAddD(), dlds);
     }
     if (threadIdx.x != 0)
         return;
@@ -973,7 +1001,7 @@ int weight_update(ParticleSet *p, const double *src, int mode)
     if (mode != 2)
         hipLaunchKernelGGL(k_weight_pass, dim3(nb), dim3(kPB), 0, st, p->logw,
             src, p->n, mode, p->pd, static_cast<Part3 *>(p->part));
-    hipLaunchKernelGGL(k_weight_finalize, dim3(1), dim3(kPB), 0, st, static_cast<const Part3 *>(p->part),
+    hipLaunchKernelGGL(k_weight_finalize, dim3(1), dim3(kMergeThreads), 0, st, static_cast<const Part3 *>(p->part),
         nb, p->n, mode, p->pd);
     VSMCH_HIP(hipGetLastError());
     return kOk;
@@ -1171,7 +1199,7 @@ int cv_step(ParticleSet *p, const CvModelDev &m, long long t, double ox,
             launch_cv<double, false>(batch, nb, st, s, p->logw, p->n, p->seed,
                 t, ox, oy, m, p->pd, part, mon);
     }
-    hipLaunchKernelGGL(k_weight_finalize, dim3(1), dim3(kPB), 0, st, static_cast<const Part3 *>(p->part),
+    hipLaunchKernelGGL(k_weight_finalize, dim3(1), dim3(kMergeThreads), 0, st, static_cast<const Part3 *>(p->part),
         nb, p->n, t == 0 ? 0 : 1, p->pd);
     VSMCH_HIP(hipGetLastError());
     return kOk;
@@ -1213,7 +1241,7 @@ int cv_step_learned(ParticleSet *p, const CvModelDev &m, const MlpSet *net,
         static_cast<float>(ox), static_cast<float>(oy), m,
         static_cast<const mlp::Frags *>(net->frags), static_cast<const PDev *>(p->pd),
         static_cast<Part3 *>(p->part));
-    hipLaunchKernelGGL(k_weight_finalize, dim3(1), dim3(kPB), 0, st,
+    hipLaunchKernelGGL(k_weight_finalize, dim3(1), dim3(kMergeThreads), 0, st,
         static_cast<const Part3 *>(p->part), nb, p->n, 1, p->pd);
     VSMCH_HIP(hipGetLastError());
     return kOk;
