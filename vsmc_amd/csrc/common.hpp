@@ -78,13 +78,14 @@ __device__ inline T wave_inclusive_scan(T v, Op op)
     return v;
 }
 
-// Block-wide exclusive scan (kBlock threads). Returns the exclusive prefix
-// of this thread; *total receives the block aggregate. `lds` needs kWaves
-// entries. Contains __syncthreads (all threads must call).
-template <typename T, typename Op>
-__device__ inline T block_exclusive_scan(T v, Op op, T identity, T *lds,
+// Block-wide exclusive scan over kThreads threads. Returns the exclusive
+// prefix of this thread; *total receives the block aggregate. `lds` needs
+// kThreads / kWave entries. Contains __syncthreads (all threads must call).
+template <int kThreads, typename T, typename Op>
+__device__ inline T block_exclusive_scan_n(T v, Op op, T identity, T *lds,
     T *total)
 {
+    constexpr int nw = kThreads / kWave;
     const int lane = threadIdx.x % kWave;
     const int wave = threadIdx.x / kWave;
     T inc = wave_inclusive_scan(v, op);
@@ -94,7 +95,7 @@ __device__ inline T block_exclusive_scan(T v, Op op, T identity, T *lds,
     T wpre = identity;
     T agg = identity;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
+    for (int w = 0; w < nw; ++w) {
         if (w < wave)
             wpre = op(wpre, lds[w]);
         agg = op(agg, lds[w]);
@@ -107,28 +108,38 @@ __device__ inline T block_exclusive_scan(T v, Op op, T identity, T *lds,
     return op(wpre, ex);
 }
 
-// Exclusive scan of n values get(i) by one block, put(i, prefix) for each;
-// returns the total. Each thread reads kItems consecutive values per chunk
-// of kTile (all its loads in flight at once), so a chunk costs one round of
-// loads and one block scan. All threads must call it.
-template <typename T, typename Op, typename Get, typename Put>
-__device__ __forceinline__ T block_scan_chunked(long long n, Get get, Put put,
+// The same over the kBlock threads of a tile block.
+template <typename T, typename Op>
+__device__ inline T block_exclusive_scan(T v, Op op, T identity, T *lds,
+    T *total)
+{
+    return block_exclusive_scan_n<kBlock>(v, op, identity, lds, total);
+}
+
+// Exclusive scan of n values get(i) by one block of kThreads threads,
+// put(i, prefix) for each; returns the total. Each thread reads kPer
+// consecutive values per chunk of kThreads * kPer (all its loads in flight
+// at once), so a chunk costs one round of loads and one block scan. All
+// threads must call it; lds needs kThreads / kWave entries.
+template <int kThreads, int kPer, typename T, typename Op, typename Get,
+    typename Put>
+__device__ __forceinline__ T block_scan_chunked_n(long long n, Get get, Put put,
     Op op, T identity, T *lds)
 {
     T carry = identity;
-    for (long long c0 = 0; c0 < n; c0 += kTile) {
-        const long long b0 = c0 + static_cast<long long>(threadIdx.x) * kItems;
-        T v[kItems];
+    for (long long c0 = 0; c0 < n; c0 += static_cast<long long>(kThreads) * kPer) {
+        const long long b0 = c0 + static_cast<long long>(threadIdx.x) * kPer;
+        T v[kPer];
         T loc = identity;
 #pragma unroll
-        for (int k = 0; k < kItems; ++k) {
+        for (int k = 0; k < kPer; ++k) {
             v[k] = (b0 + k < n) ? get(b0 + k) : identity;
             loc = op(loc, v[k]);
         }
         T tot;
-        T run = op(carry, block_exclusive_scan(loc, op, identity, lds, &tot));
+        T run = op(carry, block_exclusive_scan_n<kThreads>(loc, op, identity, lds, &tot));
 #pragma unroll
-        for (int k = 0; k < kItems; ++k) {
+        for (int k = 0; k < kPer; ++k) {
             if (b0 + k < n)
                 put(b0 + k, run);
             run = op(run, v[k]);
@@ -136,6 +147,14 @@ __device__ __forceinline__ T block_scan_chunked(long long n, Get get, Put put,
         carry = op(carry, tot);
     }
     return carry;
+}
+
+// The same over the kBlock threads of a tile block, kItems per thread.
+template <typename T, typename Op, typename Get, typename Put>
+__device__ __forceinline__ T block_scan_chunked(long long n, Get get, Put put,
+    Op op, T identity, T *lds)
+{
+    return block_scan_chunked_n<kBlock, kItems>(n, get, put, op, identity, lds);
 }
 
 template <typename T, typename Op>

@@ -257,12 +257,12 @@ __device__ inline int decide_event(PDev *pd, ResampleDev *prm, long long n,
     return active;
 }
 
-__global__ void __launch_bounds__(kPB) k_prepare(PDev *pd, ResampleDev *prm,
+__global__ void __launch_bounds__(kMergeThreads) k_prepare(PDev *pd, ResampleDev *prm,
     const Part3 *part, long long nb, long long n, double threshold,
     long long t, double *ess_hist, int *rs_hist, int scheme, double *tpre)
 {
     __shared__ int act;
-    __shared__ double lds[kWaves];
+    __shared__ double lds[kMergeThreads / kWave];
     if (threadIdx.x == 0) {
         if (ess_hist)
             ess_hist[t] = pd->ess;
@@ -278,7 +278,7 @@ __global__ void __launch_bounds__(kPB) k_prepare(PDev *pd, ResampleDev *prm,
     const double lmax = pd->lmax;
     const double inv = equal ? 1.0 : 1 / pd->sum;
     const double inv_n = 1 / static_cast<double>(n);
-    block_scan_chunked(nb,
+    block_scan_chunked_n<kMergeThreads, 4>(nb,
         [&](long long b) {
             if (equal) {
                 const long long rem = n - b * kPTile;
@@ -1036,7 +1036,7 @@ int resample_step(ParticleSet *p, int scheme, double threshold, long long t,
     // the draw parameters and the tile prefix from the weight partials
     const bool prep = p->n >= 2 && (scheme == kSystematic || scheme == kStratified);
     if (prep)
-        hipLaunchKernelGGL(k_prepare, dim3(1), dim3(kPB), 0, st, p->pd,
+        hipLaunchKernelGGL(k_prepare, dim3(1), dim3(kMergeThreads), 0, st, p->pd,
             p->ws.prm, static_cast<const Part3 *>(p->part), pblocks(p->n),
             p->n, threshold, t, ess_hist, rs_hist, scheme, p->ws.tpre);
     else

@@ -1151,24 +1151,31 @@ struct OpAddL3 {
     }
 };
 
+template <int kThreads, int kPer>
 __device__ __forceinline__ void scan_tiles_block(const long long *tagg, long long ntiles,
     long long *toff);
 
-__global__ void __launch_bounds__(kBlock) k_ridx_scan(const long long *tagg,
+// One block of kScanThreads: the scan is latency-bound, so four times the
+// threads of a tile block, four tiles each.
+constexpr int kScanThreads = 1024;
+
+__global__ void __launch_bounds__(kScanThreads) k_ridx_scan(const long long *tagg,
     long long ntiles, const ResampleDev *prm, long long *toff)
 {
     if (prm && !prm->active)
         return;
-    scan_tiles_block(tagg, ntiles, toff);
+    scan_tiles_block<kScanThreads, 4>(tagg, ntiles, toff);
 }
 
 // Exclusive offsets of the per-tile (childless, extras, sources) triples,
-// totals at toff[3 ntiles ..]. One block.
+// totals at toff[3 ntiles ..]. One block of kThreads, kPer tiles a thread
+// per chunk.
+template <int kThreads, int kPer>
 __device__ __forceinline__ void scan_tiles_block(const long long *tagg, long long ntiles,
     long long *toff)
 {
-    __shared__ L3 lds[kWaves];
-    const L3 carry = block_scan_chunked(ntiles,
+    __shared__ L3 lds[kThreads / kWave];
+    const L3 carry = block_scan_chunked_n<kThreads, kPer>(ntiles,
         [&](long long t) { return L3{tagg[3 * t], tagg[3 * t + 1], tagg[3 * t + 2]}; },
         [&](long long t, L3 ex) {
             toff[3 * t + 0] = ex.a;
@@ -1471,7 +1478,7 @@ __global__ void __launch_bounds__(kBlock) k_batch_resample(
         x = count_tile<true>(src, us, n, k, x, repf, tg, static_cast<int *>(nullptr));
     __threadfence();
     __syncthreads();
-    scan_tiles_block(tg, F.ntiles, to);
+    scan_tiles_block<kBlock, kItems>(tg, F.ntiles, to);
     __threadfence();
     __syncthreads();
     for (long long k = 0; k < F.ntiles; ++k)
@@ -2109,7 +2116,7 @@ int trans_rep_index(Workspace &ws, long long M, long long N, const int *rep,
         hipLaunchKernelGGL(k_ridx_agg<false>, dim3(ntiles), dim3(kBlock), 0,
             st, rep, M, prm, ws.tagg);
     }
-    hipLaunchKernelGGL(k_ridx_scan, dim3(1), dim3(kBlock), 0, st, ws.tagg,
+    hipLaunchKernelGGL(k_ridx_scan, dim3(1), dim3(kScanThreads), 0, st, ws.tagg,
         ntiles, prm, ws.toff);
     if (expand) {
         hipLaunchKernelGGL(k_ridx_scatter<true>, dim3(ntiles), dim3(kBlock), 0,
