@@ -240,7 +240,12 @@ struct UStrat {
         return b > a ? b : a;
     }
     __device__ long long ndraw() const { return p->ndraw; }
-    // Number of draws v_j <= x (32-bit search, as for USys).
+    // Number of draws v_j <= x (32-bit search, as for USys). The search
+    // starts at g = floor(x n) and almost always ends within one draw of
+    // it, so the uniforms of draws g - 2 .. g + 1 are taken from the two
+    // Philox blocks that hold them (one block evaluation per two words
+    // instead of one per word); a search that leaves that window evaluates
+    // the draws one at a time.
     __device__ long long count(double x) const
     {
         const int n = static_cast<int>(p->ndraw);
@@ -256,9 +261,42 @@ struct UStrat {
             g = n;
         else
             g = static_cast<int>(est);
-        while (g > 0 && vi(g - 1) > x)
+        const int w0 = g >= 2 ? g - 2 : 0;
+        const unsigned long long k0 = p->word0 + static_cast<unsigned long long>(w0);
+        const u32x4 B0 = philox_block(p->key, k0 / 4 + 1);
+        const u32x4 B1 = philox_block(p->key, k0 / 4 + 2);
+        const unsigned off = static_cast<unsigned>(k0 & 3);
+        // the uniforms of draws w0 .. w0 + 3: words off .. off + 3 of the
+        // eight in B0, B1 (value selects: no indexed array in scratch)
+        auto uwin = [&](unsigned q) -> double {
+            const unsigned i = off + q;
+            const unsigned r = i & 3;
+            const uint32_t a = r == 0 ? B0.v[0] : r == 1 ? B0.v[1] : r == 2 ? B0.v[2] : B0.v[3];
+            const uint32_t b = r == 0 ? B1.v[0] : r == 1 ? B1.v[1] : r == 2 ? B1.v[2] : B1.v[3];
+            return rn_add(rn_mul(u01_u32d(i < 4 ? a : b), p->delta),
+                rn_mul(static_cast<double>(w0 + static_cast<int>(q)), p->delta));
+        };
+        const double u0 = uwin(0), u1 = uwin(1), u2 = uwin(2), u3 = uwin(3);
+        const double m1 = u0 > u1 ? u0 : u1;
+        const double m2 = u1 > u2 ? u1 : u2;
+        const double m3 = u2 > u3 ? u2 : u3;
+        // v_j = max(u_{j-1}, u_j) (v_0 = u_0) for j in the window, any
+        // other draw one at a time
+        auto vget = [&](int j) -> double {
+            const int q = j - w0;
+            if (q == 0 && w0 == 0)
+                return u0;
+            if (q == 1)
+                return m1;
+            if (q == 2)
+                return m2;
+            if (q == 3)
+                return m3;
+            return vi(j);
+        };
+        while (g > 0 && vget(g - 1) > x)
             --g;
-        while (g < n && vi(g) <= x)
+        while (g < n && vget(g) <= x)
             ++g;
         return g;
     }
@@ -1473,17 +1511,16 @@ __global__ void __launch_bounds__(kBlock) k_batch_resample(
     long long *to = toff + 3 * (F.tile0 + f);
     const WLog src{l + F.off, p};
     const USrc us{p};
+    // every stage reads only what this workgroup wrote: a barrier (with
+    // its workgroup-scope fences) orders it; no device-scope fence
     double x = 0.0;
     for (long long k = 0; k < F.ntiles; ++k)
         x = count_tile<true>(src, us, n, k, x, repf, tg, static_cast<int *>(nullptr));
-    __threadfence();
     __syncthreads();
     scan_tiles_block<kBlock, kItems>(tg, F.ntiles, to);
-    __threadfence();
     __syncthreads();
     for (long long k = 0; k < F.ntiles; ++k)
         scatter_tile<false>(repf, n, k, to, zf, cf, sf, nullptr);
-    __threadfence();
     __syncthreads();
     const long long K = to[3 * F.ntiles + 0];
     const long long C = to[3 * F.ntiles + 2];
