@@ -191,13 +191,15 @@ int u01_stratified_seq(Workspace &ws, long long n, const double *u, double *out)
 int u01_systematic_seq(Workspace &ws, long long n, double u, double *out);
 
 // One resampling event for every filter of a batch whose prm[f].active is
-// set (one workgroup per filter); weights from the log-weights l with
-// prm[f].lmax / inv_sum / equal; FP32 SoA state with component stride ld;
-// scratch rep / zdst / cE / csrc of the batch's particle count, tagg of 3
-// per tile, toff of 3 per (tile + filter).
+// set (counts: one workgroup per filter; scatter and gather: one per tile
+// of the batch, tile_filter giving each tile's filter, ntiles in all);
+// weights from the log-weights l with prm[f].lmax / inv_sum / equal; FP32
+// SoA state with component stride ld; scratch rep / zdst / cE / csrc of
+// the batch's particle count, tagg of 3 per tile, toff of 3 per (tile +
+// filter).
 int batch_resample(int scheme, int nfilters, const BatchFilter *bf,
-    const ResampleDev *prm, const double *l, float *state, long long ld,
-    int *rep, long long *tagg, long long *toff, int *zdst, int *cE, int *csrc,
-    hipStream_t st);
+    const int *tile_filter, long long ntiles, const ResampleDev *prm,
+    const double *l, float *state, long long ld, int *rep, long long *tagg,
+    long long *toff, int *zdst, int *cE, int *csrc, hipStream_t st);
 
 } // namespace vsmch

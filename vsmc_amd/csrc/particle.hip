@@ -899,8 +899,9 @@ int batch_step(const BatchDev &b, int move_blocks, const MlpSet *net,
         static_cast<const Part3 *>(b.part), b.pd, b.prm, b.threshold,
         b.scheme, b.tmax, b.logz_hist, b.ess_hist, b.rs_hist);
     VSMCH_HIP(hipGetLastError());
-    int rc = batch_resample(b.scheme, b.nfilters, b.bf, b.prm, b.l, b.state,
-        b.ld, b.rep, b.tagg, b.toff, b.zdst, b.cE, b.csrc, st);
+    int rc = batch_resample(b.scheme, b.nfilters, b.bf, b.tile_filter,
+        b.ntiles, b.prm, b.l, b.state, b.ld, b.rep, b.tagg, b.toff, b.zdst,
+        b.cE, b.csrc, st);
     if (rc != kOk)
         return rc;
     hipLaunchKernelGGL(k_batch_after, dim3((b.nfilters + 255) / 256), dim3(256),

@@ -1511,29 +1511,56 @@ __global__ void __launch_bounds__(kBlock) k_batch_resample(
     long long *to = toff + 3 * (F.tile0 + f);
     const WLog src{l + F.off, p};
     const USrc us{p};
-    // every stage reads only what this workgroup wrote: a barrier (with
-    // its workgroup-scope fences) orders it; no device-scope fence
+    // the scan reads only what this workgroup wrote: a barrier (with its
+    // workgroup-scope fences) orders it
     double x = 0.0;
     for (long long k = 0; k < F.ntiles; ++k)
         x = count_tile<true>(src, us, n, k, x, repf, tg, static_cast<int *>(nullptr));
     __syncthreads();
     scan_tiles_block<kBlock, kItems>(tg, F.ntiles, to);
-    __syncthreads();
-    for (long long k = 0; k < F.ntiles; ++k)
-        scatter_tile<false>(repf, n, k, to, zf, cf, sf, nullptr);
-    __syncthreads();
+}
+
+// The scatter of childless slots and extra copies, one workgroup per tile
+// of every filter (tile_filter: the filter of each tile), and the in-place
+// gather, one workgroup per chunk of kTile ranks (at most a filter's tile
+// count): the per-filter stages that do not chain from tile to tile run
+// over the whole device instead of inside the filter's workgroup.
+__global__ void __launch_bounds__(kBlock) k_batch_scatter(
+    const BatchFilter *bf, const int *tile_filter, const ResampleDev *prm,
+    const int *rep, const long long *toff, int *zdst, int *cE, int *csrc)
+{
+    const int f = tile_filter[blockIdx.x];
+    if (!prm[f].active)
+        return;
+    const BatchFilter F = bf[f];
+    const long long k = static_cast<long long>(blockIdx.x) - F.tile0;
+    scatter_tile<false>(rep + F.off, F.n, k, toff + 3 * (F.tile0 + f),
+        zdst + F.off, cE + F.off, csrc + F.off, nullptr);
+}
+
+__global__ void __launch_bounds__(kBlock) k_batch_expand(
+    const BatchFilter *bf, const int *tile_filter, const ResampleDev *prm,
+    const long long *toff, const int *zdst, const int *cE, const int *csrc,
+    float *state, long long ld)
+{
+    const int f = tile_filter[blockIdx.x];
+    if (!prm[f].active)
+        return;
+    const BatchFilter F = bf[f];
+    const long long chunk = static_cast<long long>(blockIdx.x) - F.tile0;
+    const long long *to = toff + 3 * (F.tile0 + f);
     const long long K = to[3 * F.ntiles + 0];
     const long long C = to[3 * F.ntiles + 2];
-    if (C > 0)
-        for (long long chunk = 0; chunk * kTile < K; ++chunk)
-            expand_chunk<false, float>(chunk, K, C, zf, cf, sf, nullptr,
-                state + F.off, 4, ld);
+    if (C <= 0 || chunk * kTile >= K)
+        return;
+    expand_chunk<false, float>(chunk, K, C, zdst + F.off, cE + F.off,
+        csrc + F.off, nullptr, state + F.off, 4, ld);
 }
 
 int batch_resample(int scheme, int nfilters, const BatchFilter *bf,
-    const ResampleDev *prm, const double *l, float *state, long long ld,
-    int *rep, long long *tagg, long long *toff, int *zdst, int *cE, int *csrc,
-    hipStream_t st)
+    const int *tile_filter, long long ntiles, const ResampleDev *prm,
+    const double *l, float *state, long long ld, int *rep, long long *tagg,
+    long long *toff, int *zdst, int *cE, int *csrc, hipStream_t st)
 {
     if (scheme == kSystematic)
         hipLaunchKernelGGL(k_batch_resample<USys>, dim3(nfilters), dim3(kBlock),
@@ -1546,6 +1573,14 @@ int batch_resample(int scheme, int nfilters, const BatchFilter *bf,
         set_error("batched filters resample with the systematic or stratified scheme");
         return kErrUnsupported;
     }
+    const unsigned int gt = static_cast<unsigned int>(ntiles);
+    hipLaunchKernelGGL(k_batch_scatter, dim3(gt), dim3(kBlock), 0, st, bf,
+        tile_filter, prm, static_cast<const int *>(rep),
+        static_cast<const long long *>(toff), zdst, cE, csrc);
+    hipLaunchKernelGGL(k_batch_expand, dim3(gt), dim3(kBlock), 0, st, bf,
+        tile_filter, prm, static_cast<const long long *>(toff),
+        static_cast<const int *>(zdst), static_cast<const int *>(cE),
+        static_cast<const int *>(csrc), state, ld);
     VSMCH_HIP(hipGetLastError());
     return kOk;
 }
