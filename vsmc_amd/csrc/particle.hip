@@ -852,7 +852,7 @@ __global__ void __launch_bounds__(kPB) k_batch_move(float *state, double *l,
 __global__ void __launch_bounds__(kPB) k_batch_finalize(const BatchFilter *bf,
     const long long *tstep, const Part3 *part, PDev *pd, ResampleDev *prm,
     double threshold, int scheme, long long tmax, double *logz_hist,
-    double *ess_hist, int *rs_hist)
+    double *ess_hist, int *rs_hist, double *tpre)
 {
     const int f = blockIdx.x;
     const long long t = *tstep;
@@ -863,13 +863,37 @@ __global__ void __launch_bounds__(kPB) k_batch_finalize(const BatchFilter *bf,
         return;
     }
     weight_merge(part + F.tile0, F.ntiles, F.n, t == 0 ? 0 : 1, pd + f);
+    __shared__ int act;
     if (threadIdx.x == 0) {
         PDev *p = pd + f;
         const long long h = static_cast<long long>(f) * tmax + t;
         logz_hist[h] = p->logz;
         ess_hist[h] = p->ess;
-        rs_hist[h] = decide_event(p, prm + f, F.n, threshold, scheme);
+        act = decide_event(p, prm + f, F.n, threshold, scheme);
+        rs_hist[h] = act;
     }
+    __syncthreads();
+    if (!act)
+        return;
+    // the approximate prefix of the normalised weights over the filter's
+    // tiles (as k_prepare for one particle set), for the tile summaries
+    const PDev *p = pd + f;
+    const int equal = p->equal;
+    const double lmax = p->lmax;
+    const double inv = equal ? 1.0 : 1 / p->sum;
+    const double inv_n = 1 / static_cast<double>(F.n);
+    __shared__ double lds[kWaves];
+    block_scan_chunked(F.ntiles,
+        [&](long long b) {
+            if (equal) {
+                const long long rem = F.n - b * kPTile;
+                return static_cast<double>(rem < kPTile ? rem : kPTile) * inv_n;
+            }
+            const Part3 q = part[F.tile0 + b];
+            return q.s > 0.0 ? q.s * exp(q.m - lmax) * inv : 0.0;
+        },
+        [&](long long b, double ex) { tpre[F.tile0 + b] = ex; }, OpAddD(), 0.0,
+        lds);
 }
 
 __global__ void k_batch_after(const BatchFilter *bf, PDev *pd,
@@ -897,11 +921,11 @@ int batch_step(const BatchDev &b, int move_blocks, const MlpSet *net,
     hipLaunchKernelGGL(k_batch_finalize, dim3(b.nfilters), dim3(kPB), 0, st,
         b.bf, static_cast<const long long *>(b.tstep),
         static_cast<const Part3 *>(b.part), b.pd, b.prm, b.threshold,
-        b.scheme, b.tmax, b.logz_hist, b.ess_hist, b.rs_hist);
+        b.scheme, b.tmax, b.logz_hist, b.ess_hist, b.rs_hist, b.tpre);
     VSMCH_HIP(hipGetLastError());
     int rc = batch_resample(b.scheme, b.nfilters, b.bf, b.tile_filter,
         b.ntiles, b.prm, b.l, b.state, b.ld, b.rep, b.tagg, b.toff, b.zdst,
-        b.cE, b.csrc, st);
+        b.cE, b.csrc, b.tpre, b.segs, b.xstart, st);
     if (rc != kOk)
         return rc;
     hipLaunchKernelGGL(k_batch_after, dim3((b.nfilters + 255) / 256), dim3(256),

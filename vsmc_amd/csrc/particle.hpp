@@ -126,6 +126,9 @@ struct BatchDev {
     int *csrc = nullptr;
     long long *tagg = nullptr;   // 3 per tile
     long long *toff = nullptr;   // 3 per (tile + filter)
+    double *tpre = nullptr;      // approximate prefix of the weights per tile
+    fold::TileSeg *segs = nullptr;  // tile summaries
+    double *xstart = nullptr;    // exact start of every tile
     double *obs = nullptr;       // [nfilters][tmax][2]
     long long *tstep = nullptr;  // device step counter
     double *logz_hist = nullptr; // [nfilters][tmax]

@@ -1483,41 +1483,98 @@ __global__ void k_fill(T *p, long long n, T v, const ResampleDev *prm)
 
 // ------------------------------------------------- batched filters
 
-// The whole resampling event of one filter of a batch in one workgroup:
-// the exact sequential fold tile by tile from 0 (each tile's exact start is
-// the previous tile's exact end, so no tile summaries or resolver are
-// needed), counts of the systematic or stratified uniforms, the rep ->
-// index aggregates, their scan, the scatter of childless slots and extra
-// copies, and the in-place gather of the FP32 SoA state (component stride
-// ld). Same counts, ancestors and state as the multi-kernel path. Filters
+// Batched resampling, the counting stages. Every tile of every resampling
+// filter (tile_filter: the filter of each tile) gets its summary from the
+// approximate prefix tpre (as k_segments); one workgroup per filter walks
+// its tiles' summaries from the exact start 0, folding a tile exactly where
+// a summary does not apply (as k_resolve, without groups: a filter has at
+// most a few hundred tiles); then every tile counts from its exact start
+// with the full exact fold, and one workgroup per filter scans the rep ->
+// index aggregates. Same counts as the single-filter pipeline. Filters
 // whose prm->active is 0 return at once.
-template <class USrc>
-__global__ void __launch_bounds__(kBlock) k_batch_resample(
-    const BatchFilter *bf, const ResampleDev *prm, const double *l,
-    float *state, long long ld, int *rep, long long *tagg, long long *toff,
-    int *zdst, int *cE, int *csrc)
+__global__ void __launch_bounds__(kBlock) k_batch_segments(
+    const BatchFilter *bf, const int *tile_filter, const ResampleDev *prm,
+    const double *l, const double *tpre, TileSeg *segs)
+{
+    const int f = tile_filter[blockIdx.x];
+    if (!prm[f].active)
+        return;
+    const BatchFilter F = bf[f];
+    const long long k = static_cast<long long>(blockIdx.x) - F.tile0;
+    const long long base = k * kTile;
+    __shared__ fold::SegShared sh;
+    load_tile_lds(WLog{l + F.off, prm + f}, base, F.n, sh.tile, 0.0);
+    const long long rem = F.n - base;
+    const int nvalid = rem < kTile ? static_cast<int>(rem) : kTile;
+    fold::tile_segments(nvalid, k == 0 ? 0.0 : tpre[blockIdx.x], &segs[blockIdx.x],
+        sh);
+}
+
+__global__ void __launch_bounds__(kBlock) k_batch_walk(const BatchFilter *bf,
+    const ResampleDev *prm, const double *l, const TileSeg *segs,
+    double *xstart)
 {
     const int f = blockIdx.x;
     const ResampleDev *p = prm + f;
     if (!p->active)
         return;
     const BatchFilter F = bf[f];
-    const long long n = F.n;
-    int *repf = rep + F.off;
-    int *zf = zdst + F.off;
-    int *cf = cE + F.off;
-    int *sf = csrc + F.off;
-    long long *tg = tagg + 3 * F.tile0;
-    long long *to = toff + 3 * (F.tile0 + f);
     const WLog src{l + F.off, p};
-    const USrc us{p};
-    // the scan reads only what this workgroup wrote: a barrier (with its
-    // workgroup-scope fences) orders it
-    double x = 0.0;
-    for (long long k = 0; k < F.ntiles; ++k)
-        x = count_tile<true>(src, us, n, k, x, repf, tg, static_cast<int *>(nullptr));
+    __shared__ double tile[kTileLds];
+    __shared__ fold::ExactShared ex;
+    __shared__ double xs;
+    __shared__ int ok;
+    if (threadIdx.x == 0)
+        xs = 0.0;
     __syncthreads();
-    scan_tiles_block<kBlock, kItems>(tg, F.ntiles, to);
+    for (long long k = 0; k < F.ntiles; ++k) {
+        if (threadIdx.x == 0) {
+            double y = xs;
+            xstart[F.tile0 + k] = y;
+            ok = fold::apply_segments(segs[F.tile0 + k], y) ? 1 : 0;
+            if (ok)
+                xs = y;
+        }
+        __syncthreads();
+        if (!ok) {
+            const long long base = k * kTile;
+            load_tile_lds(src, base, F.n, tile, 0.0);
+            fold::tile_exact(tile, xs, ex);
+            const long long rem = F.n - base;
+            const int last = (rem < kTile ? static_cast<int>(rem) : kTile) - 1;
+            const double y = tile[lds_pad(last)];
+            __syncthreads();
+            if (threadIdx.x == 0)
+                xs = y;
+        }
+        __syncthreads();
+    }
+}
+
+template <class USrc>
+__global__ void __launch_bounds__(kBlock) k_batch_count(const BatchFilter *bf,
+    const int *tile_filter, const ResampleDev *prm, const double *l,
+    const double *xstart, int *rep, long long *tagg)
+{
+    const int f = tile_filter[blockIdx.x];
+    const ResampleDev *p = prm + f;
+    if (!p->active)
+        return;
+    const BatchFilter F = bf[f];
+    const long long k = static_cast<long long>(blockIdx.x) - F.tile0;
+    count_tile<true>(WLog{l + F.off, p}, USrc{p}, F.n, k, xstart[blockIdx.x],
+        rep + F.off, tagg + 3 * F.tile0, static_cast<int *>(nullptr));
+}
+
+__global__ void __launch_bounds__(kBlock) k_batch_scan(const BatchFilter *bf,
+    const ResampleDev *prm, const long long *tagg, long long *toff)
+{
+    const int f = blockIdx.x;
+    if (!prm[f].active)
+        return;
+    const BatchFilter F = bf[f];
+    scan_tiles_block<kBlock, kItems>(tagg + 3 * F.tile0, F.ntiles,
+        toff + 3 * (F.tile0 + f));
 }
 
 // The scatter of childless slots and extra copies, one workgroup per tile
@@ -1560,20 +1617,28 @@ __global__ void __launch_bounds__(kBlock) k_batch_expand(
 int batch_resample(int scheme, int nfilters, const BatchFilter *bf,
     const int *tile_filter, long long ntiles, const ResampleDev *prm,
     const double *l, float *state, long long ld, int *rep, long long *tagg,
-    long long *toff, int *zdst, int *cE, int *csrc, hipStream_t st)
+    long long *toff, int *zdst, int *cE, int *csrc, const double *tpre,
+    TileSeg *segs, double *xstart, hipStream_t st)
 {
-    if (scheme == kSystematic)
-        hipLaunchKernelGGL(k_batch_resample<USys>, dim3(nfilters), dim3(kBlock),
-            0, st, bf, prm, l, state, ld, rep, tagg, toff, zdst, cE, csrc);
-    else if (scheme == kStratified)
-        hipLaunchKernelGGL(k_batch_resample<UStrat>, dim3(nfilters),
-            dim3(kBlock), 0, st, bf, prm, l, state, ld, rep, tagg, toff, zdst,
-            cE, csrc);
-    else {
+    if (scheme != kSystematic && scheme != kStratified) {
         set_error("batched filters resample with the systematic or stratified scheme");
         return kErrUnsupported;
     }
     const unsigned int gt = static_cast<unsigned int>(ntiles);
+    hipLaunchKernelGGL(k_batch_segments, dim3(gt), dim3(kBlock), 0, st, bf,
+        tile_filter, prm, l, tpre, segs);
+    hipLaunchKernelGGL(k_batch_walk, dim3(nfilters), dim3(kBlock), 0, st, bf,
+        prm, l, static_cast<const TileSeg *>(segs), xstart);
+    if (scheme == kSystematic)
+        hipLaunchKernelGGL(k_batch_count<USys>, dim3(gt), dim3(kBlock), 0, st,
+            bf, tile_filter, prm, l, static_cast<const double *>(xstart), rep,
+            tagg);
+    else
+        hipLaunchKernelGGL(k_batch_count<UStrat>, dim3(gt), dim3(kBlock), 0, st,
+            bf, tile_filter, prm, l, static_cast<const double *>(xstart), rep,
+            tagg);
+    hipLaunchKernelGGL(k_batch_scan, dim3(nfilters), dim3(kBlock), 0, st, bf,
+        prm, static_cast<const long long *>(tagg), toff);
     hipLaunchKernelGGL(k_batch_scatter, dim3(gt), dim3(kBlock), 0, st, bf,
         tile_filter, prm, static_cast<const int *>(rep),
         static_cast<const long long *>(toff), zdst, cE, csrc);
