@@ -84,7 +84,7 @@ void batch_destroy(BatchSet *b)
     BatchDev &d = b->d;
     void *ps[] = {d.bf, d.tile_filter, d.state, d.l, d.part, d.pd, d.prm, d.rep,
         d.zdst, d.cE, d.csrc, d.tagg, d.toff, d.obs, d.tstep, d.logz_hist,
-        d.ess_hist, d.rs_hist, d.tpre, d.segs, d.xstart};
+        d.ess_hist, d.rs_hist, d.tpre, d.segs, d.xstart, d.hard};
     for (void *p : ps)
         (void)hipFree(p);
     if (b->stream)
@@ -147,7 +147,7 @@ int batch_create(BatchSet **out, int nfilters, const long long *n,
         dalloc(&d.csrc, d.ld) && dalloc(&d.tagg, 3 * tiles) &&
         dalloc(&d.toff, 3 * (tiles + nfilters)) && dalloc(&d.obs, 2 * ft) &&
         dalloc(&d.tpre, tiles) && dalloc(&d.segs, tiles) &&
-        dalloc(&d.xstart, tiles) &&
+        dalloc(&d.xstart, tiles) && dalloc(&d.hard, tiles + 1) &&
         dalloc(&d.tstep, 1) && dalloc(&d.logz_hist, ft) &&
         dalloc(&d.ess_hist, ft) && dalloc(&d.rs_hist, ft);
     if (!ok) {

@@ -194,7 +194,8 @@ int u01_systematic_seq(Workspace &ws, long long n, double u, double *out);
 // set (tile summaries, counts, scatter and gather: one workgroup per tile
 // of the batch, tile_filter giving each tile's filter, ntiles in all; the
 // walk over the summaries and the scan: one per filter; tpre: the
-// approximate prefix per tile, segs / xstart: scratch of one per tile);
+// approximate prefix per tile, segs / xstart: scratch of one per tile,
+// hard: a list of ntiles + 1 whose count k_batch_finalize clears);
 // weights from the log-weights l with prm[f].lmax / inv_sum / equal; FP32
 // SoA state with component stride ld; scratch rep / zdst / cE / csrc of
 // the batch's particle count, tagg of 3 per tile, toff of 3 per (tile +
@@ -203,6 +204,6 @@ int batch_resample(int scheme, int nfilters, const BatchFilter *bf,
     const int *tile_filter, long long ntiles, const ResampleDev *prm,
     const double *l, float *state, long long ld, int *rep, long long *tagg,
     long long *toff, int *zdst, int *cE, int *csrc, const double *tpre,
-    fold::TileSeg *segs, double *xstart, hipStream_t st);
+    fold::TileSeg *segs, double *xstart, int *hard, hipStream_t st);
 
 } // namespace vsmch

@@ -852,9 +852,12 @@ __global__ void __launch_bounds__(kPB) k_batch_move(float *state, double *l,
 __global__ void __launch_bounds__(kPB) k_batch_finalize(const BatchFilter *bf,
     const long long *tstep, const Part3 *part, PDev *pd, ResampleDev *prm,
     double threshold, int scheme, long long tmax, double *logz_hist,
-    double *ess_hist, int *rs_hist, double *tpre)
+    double *ess_hist, int *rs_hist, double *tpre, int *hard)
 {
     const int f = blockIdx.x;
+    // the batched resampler's list of tiles to count exactly starts empty
+    if (f == 0 && threadIdx.x == 0)
+        hard[0] = 0;
     const long long t = *tstep;
     const BatchFilter F = bf[f];
     if (t >= F.T) {
@@ -921,11 +924,11 @@ int batch_step(const BatchDev &b, int move_blocks, const MlpSet *net,
     hipLaunchKernelGGL(k_batch_finalize, dim3(b.nfilters), dim3(kPB), 0, st,
         b.bf, static_cast<const long long *>(b.tstep),
         static_cast<const Part3 *>(b.part), b.pd, b.prm, b.threshold,
-        b.scheme, b.tmax, b.logz_hist, b.ess_hist, b.rs_hist, b.tpre);
+        b.scheme, b.tmax, b.logz_hist, b.ess_hist, b.rs_hist, b.tpre, b.hard);
     VSMCH_HIP(hipGetLastError());
     int rc = batch_resample(b.scheme, b.nfilters, b.bf, b.tile_filter,
         b.ntiles, b.prm, b.l, b.state, b.ld, b.rep, b.tagg, b.toff, b.zdst,
-        b.cE, b.csrc, b.tpre, b.segs, b.xstart, st);
+        b.cE, b.csrc, b.tpre, b.segs, b.xstart, b.hard, st);
     if (rc != kOk)
         return rc;
     hipLaunchKernelGGL(k_batch_after, dim3((b.nfilters + 255) / 256), dim3(256),

@@ -129,6 +129,7 @@ struct BatchDev {
     double *tpre = nullptr;      // approximate prefix of the weights per tile
     fold::TileSeg *segs = nullptr;  // tile summaries
     double *xstart = nullptr;    // exact start of every tile
+    int *hard = nullptr;         // count, then the tiles counted exactly
     double *obs = nullptr;       // [nfilters][tmax][2]
     long long *tstep = nullptr;  // device step counter
     double *logz_hist = nullptr; // [nfilters][tmax]

@@ -913,7 +913,8 @@ struct FoldBufs {
 
 template <bool kFull, class WSrc, class USrc>
 __device__ __forceinline__ double count_tile(WSrc src, USrc us, long long M,
-    long long tidx, double x0, int *rep, long long *tagg, int *hard);
+    long long tidx, double x0, int *rep, long long *tagg, int *hard,
+    int hard_id = -1);
 
 // Replication counts (and residual integer parts) for one tile whose sums
 // stay in one binade; any other tile goes to the list `hard`.
@@ -953,10 +954,12 @@ __global__ void __launch_bounds__(kBlock) k_count_hard(WSrc src, USrc us,
 // aggregates to tagg[3 tidx ..] when tagg is non-null. Returns the exact
 // sum after the tile's last element (the next tile's start). Without kFull
 // only a tile whose sums stay in x0's binade is done; another is appended
-// to `hard` and nothing is written.
+// to `hard` (as hard_id when that is set, else as tidx) and nothing is
+// written.
 template <bool kFull, class WSrc, class USrc>
 __device__ __forceinline__ double count_tile(WSrc src, USrc us, long long M,
-    long long tidx, double x0, int *rep, long long *tagg, int *hard)
+    long long tidx, double x0, int *rep, long long *tagg, int *hard,
+    int hard_id)
 {
     const long long base = tidx * kTile;
     __shared__ double tile[kTileLds];
@@ -971,7 +974,7 @@ __device__ __forceinline__ double count_tile(WSrc src, USrc us, long long M,
         __shared__ fold::SingleShared ssh;
         if (!fold::tile_single(tile, x0, ssh)) {
             if (threadIdx.x == 0)
-                hard[1 + atomicAdd(hard, 1)] = static_cast<int>(tidx);
+                hard[1 + atomicAdd(hard, 1)] = hard_id >= 0 ? hard_id : static_cast<int>(tidx);
             return 0.0;
         }
     }
@@ -1510,6 +1513,9 @@ __global__ void __launch_bounds__(kBlock) k_batch_segments(
         sh);
 }
 
+// Tile summaries a walk holds in LDS at a time.
+constexpr int kWalkWin = 16;
+
 __global__ void __launch_bounds__(kBlock) k_batch_walk(const BatchFilter *bf,
     const ResampleDev *prm, const double *l, const TileSeg *segs,
     double *xstart)
@@ -1522,21 +1528,49 @@ __global__ void __launch_bounds__(kBlock) k_batch_walk(const BatchFilter *bf,
     const WLog src{l + F.off, p};
     __shared__ double tile[kTileLds];
     __shared__ fold::ExactShared ex;
+    __shared__ TileSeg win[kWalkWin];
     __shared__ double xs;
+    __shared__ long long next;   // first tile not walked yet
     __shared__ int ok;
-    if (threadIdx.x == 0)
+    constexpr int kWords = static_cast<int>(sizeof(TileSeg) / 4);
+    static_assert(sizeof(TileSeg) % 4 == 0, "summaries copy as words");
+    if (threadIdx.x == 0) {
         xs = 0.0;
+        next = 0;
+    }
     __syncthreads();
-    for (long long k = 0; k < F.ntiles; ++k) {
-        if (threadIdx.x == 0) {
-            double y = xs;
-            xstart[F.tile0 + k] = y;
-            ok = fold::apply_segments(segs[F.tile0 + k], y) ? 1 : 0;
-            if (ok)
-                xs = y;
-        }
+    for (long long w0 = 0; w0 < F.ntiles; w0 += kWalkWin) {
+        // stage the window's summaries (all threads, coalesced words)
+        const int nw = F.ntiles - w0 < kWalkWin ? static_cast<int>(F.ntiles - w0) : kWalkWin;
+        const int *from = reinterpret_cast<const int *>(segs + F.tile0 + w0);
+        int *to = reinterpret_cast<int *>(win);
+        for (int q = threadIdx.x; q < nw * kWords; q += kBlock)
+            to[q] = from[q];
         __syncthreads();
-        if (!ok) {
+        while (next < w0 + nw) {
+            // thread 0 applies summaries until one does not hold
+            if (threadIdx.x == 0) {
+                double x = xs;
+                long long k = next;
+                int good = 1;
+                for (; k < w0 + nw; ++k) {
+                    xstart[F.tile0 + k] = x;
+                    double y = x;
+                    if (!fold::apply_segments(win[k - w0], y)) {
+                        good = 0;
+                        break;
+                    }
+                    x = y;
+                }
+                xs = x;
+                next = k;
+                ok = good;
+            }
+            __syncthreads();
+            if (ok)
+                break;
+            // tile `next` folded exactly from its exact start xs
+            const long long k = next;
             const long long base = k * kTile;
             load_tile_lds(src, base, F.n, tile, 0.0);
             fold::tile_exact(tile, xs, ex);
@@ -1544,17 +1578,23 @@ __global__ void __launch_bounds__(kBlock) k_batch_walk(const BatchFilter *bf,
             const int last = (rem < kTile ? static_cast<int>(rem) : kTile) - 1;
             const double y = tile[lds_pad(last)];
             __syncthreads();
-            if (threadIdx.x == 0)
+            if (threadIdx.x == 0) {
                 xs = y;
+                next = k + 1;
+            }
+            __syncthreads();
         }
         __syncthreads();
     }
 }
 
+// Counts of every tile of the resampling filters from their exact starts:
+// the one-binade fast path, a tile that needs more appended (as its batch
+// tile index) to `hard` for k_batch_count_hard.
 template <class USrc>
 __global__ void __launch_bounds__(kBlock) k_batch_count(const BatchFilter *bf,
     const int *tile_filter, const ResampleDev *prm, const double *l,
-    const double *xstart, int *rep, long long *tagg)
+    const double *xstart, int *rep, long long *tagg, int *hard)
 {
     const int f = tile_filter[blockIdx.x];
     const ResampleDev *p = prm + f;
@@ -1562,8 +1602,27 @@ __global__ void __launch_bounds__(kBlock) k_batch_count(const BatchFilter *bf,
         return;
     const BatchFilter F = bf[f];
     const long long k = static_cast<long long>(blockIdx.x) - F.tile0;
-    count_tile<true>(WLog{l + F.off, p}, USrc{p}, F.n, k, xstart[blockIdx.x],
-        rep + F.off, tagg + 3 * F.tile0, static_cast<int *>(nullptr));
+    count_tile<false>(WLog{l + F.off, p}, USrc{p}, F.n, k, xstart[blockIdx.x],
+        rep + F.off, tagg + 3 * F.tile0, hard, static_cast<int>(blockIdx.x));
+}
+
+// The listed tiles (hard[1 .. hard[0]]) with the full exact fold.
+template <class USrc>
+__global__ void __launch_bounds__(kBlock) k_batch_count_hard(
+    const BatchFilter *bf, const int *tile_filter, const ResampleDev *prm,
+    const double *l, const double *xstart, int *rep, long long *tagg,
+    const int *hard)
+{
+    const int nhard = hard[0];
+    for (int q = blockIdx.x; q < nhard; q += gridDim.x) {
+        const int id = hard[1 + q];
+        const int f = tile_filter[id];
+        const ResampleDev *p = prm + f;
+        const BatchFilter F = bf[f];
+        const long long k = static_cast<long long>(id) - F.tile0;
+        count_tile<true>(WLog{l + F.off, p}, USrc{p}, F.n, k, xstart[id],
+            rep + F.off, tagg + 3 * F.tile0, static_cast<int *>(nullptr));
+    }
 }
 
 __global__ void __launch_bounds__(kBlock) k_batch_scan(const BatchFilter *bf,
@@ -1618,7 +1677,7 @@ int batch_resample(int scheme, int nfilters, const BatchFilter *bf,
     const int *tile_filter, long long ntiles, const ResampleDev *prm,
     const double *l, float *state, long long ld, int *rep, long long *tagg,
     long long *toff, int *zdst, int *cE, int *csrc, const double *tpre,
-    TileSeg *segs, double *xstart, hipStream_t st)
+    TileSeg *segs, double *xstart, int *hard, hipStream_t st)
 {
     if (scheme != kSystematic && scheme != kStratified) {
         set_error("batched filters resample with the systematic or stratified scheme");
@@ -1629,14 +1688,24 @@ int batch_resample(int scheme, int nfilters, const BatchFilter *bf,
         tile_filter, prm, l, tpre, segs);
     hipLaunchKernelGGL(k_batch_walk, dim3(nfilters), dim3(kBlock), 0, st, bf,
         prm, l, static_cast<const TileSeg *>(segs), xstart);
-    if (scheme == kSystematic)
+    // hard[0] was cleared by this step's k_batch_finalize; one workgroup
+    // per listed tile (the list can hold many filters' tiles at once)
+    const unsigned int gh = gt;
+    if (scheme == kSystematic) {
         hipLaunchKernelGGL(k_batch_count<USys>, dim3(gt), dim3(kBlock), 0, st,
             bf, tile_filter, prm, l, static_cast<const double *>(xstart), rep,
-            tagg);
-    else
+            tagg, hard);
+        hipLaunchKernelGGL(k_batch_count_hard<USys>, dim3(gh), dim3(kBlock), 0,
+            st, bf, tile_filter, prm, l, static_cast<const double *>(xstart), rep,
+            tagg, static_cast<const int *>(hard));
+    } else {
         hipLaunchKernelGGL(k_batch_count<UStrat>, dim3(gt), dim3(kBlock), 0, st,
             bf, tile_filter, prm, l, static_cast<const double *>(xstart), rep,
-            tagg);
+            tagg, hard);
+        hipLaunchKernelGGL(k_batch_count_hard<UStrat>, dim3(gh), dim3(kBlock),
+            0, st, bf, tile_filter, prm, l, static_cast<const double *>(xstart),
+            rep, tagg, static_cast<const int *>(hard));
+    }
     hipLaunchKernelGGL(k_batch_scan, dim3(nfilters), dim3(kBlock), 0, st, bf,
         prm, static_cast<const long long *>(tagg), toff);
     hipLaunchKernelGGL(k_batch_scatter, dim3(gt), dim3(kBlock), 0, st, bf,
