@@ -223,6 +223,17 @@ __device__ inline bool may_cross(double sprev, double s)
 
 __device__ inline bool finite_tag(int tg) { return tg != kZeroTag && tg != kInfTag; }
 
+// Element idx starts a run of the multi-run pass: the pass's first element,
+// or an add that may change binade. An add of 0 leaves the sum exactly as
+// it is, so it never starts one (the zero padding after a partial tile's
+// last weight would otherwise make a run of every padding element when the
+// total sits at a binade boundary, as the normalised weights' total 1 does).
+__device__ inline bool run_start(int idx, int pos0, double w, double sprev,
+    double s)
+{
+    return idx == pos0 || (w > 0.0 && may_cross(sprev, s));
+}
+
 // Rounded-step total as a map (saturating: at or above 2^54 the run
 // certainly leaves its binade).
 __device__ inline FMap steps_map(double d)
@@ -268,7 +279,7 @@ __device__ __forceinline__ void tile_runs(double *t, int pos0, double cur0,
                 const double w = mine[j];
                 acc += w;
                 const double s = cur0 + acc;
-                start = (idx == pos0) || may_cross(sprev, s);
+                start = run_start(idx, pos0, w, sprev, s);
                 if (start) {
                     loc.cnt += 1;
                     loc.d = 0.0;
@@ -303,7 +314,7 @@ __device__ __forceinline__ void tile_runs(double *t, int pos0, double cur0,
                 const double w = mine[j];
                 acc += w;
                 const double s = cur0 + acc;
-                if ((idx == pos0) || may_cross(sprev, s)) {
+                if (run_start(idx, pos0, w, sprev, s)) {
                     lm.flag = 1;
                     lm.m = ident();
                 } else {
@@ -335,7 +346,7 @@ __device__ __forceinline__ void tile_runs(double *t, int pos0, double cur0,
                 const double w = mine[j];
                 acc += w;
                 const double s = cur0 + acc;
-                const bool start = (idx == pos0) || may_cross(sprev, s);
+                const bool start = run_start(idx, pos0, w, sprev, s);
                 int tg = kNoTag;
                 if (start) {
                     if (idx > base && idx > pos0 && r >= 0 && r < kMaxRuns) {
@@ -413,7 +424,7 @@ __device__ __forceinline__ void tile_runs(double *t, int pos0, double cur0,
                 const double w = mine[j];
                 acc += w;
                 const double s = cur0 + acc;
-                const bool start = (idx == pos0) || may_cross(sprev, s);
+                const bool start = run_start(idx, pos0, w, sprev, s);
                 sprev = s;
                 if (start) {
                     ++r;
