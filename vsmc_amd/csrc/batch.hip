@@ -84,7 +84,7 @@ void batch_destroy(BatchSet *b)
     BatchDev &d = b->d;
     void *ps[] = {d.bf, d.tile_filter, d.state, d.l, d.part, d.pd, d.prm, d.rep,
         d.zdst, d.cE, d.csrc, d.tagg, d.toff, d.obs, d.tstep, d.logz_hist,
-        d.ess_hist, d.rs_hist, d.tpre, d.segs, d.xstart, d.hard};
+        d.ess_hist, d.rs_hist, d.tpre, d.segs, d.xstart, d.hard, d.lsub};
     for (void *p : ps)
         (void)hipFree(p);
     if (b->stream)
@@ -142,6 +142,7 @@ int batch_create(BatchSet **out, int nfilters, const long long *n,
     ok = ok && dalloc(&d.bf, nfilters) && dalloc(&d.tile_filter, tiles) &&
         dalloc(&d.state, 4 * d.ld) && dalloc(&d.l, d.ld) &&
         dalloc(reinterpret_cast<double **>(&d.part), 3 * tiles) &&
+        dalloc(reinterpret_cast<double **>(&d.lsub), 3 * kLearnedChunks * tiles) &&
         dalloc(&d.pd, nfilters) && dalloc(&d.prm, nfilters) &&
         dalloc(&d.rep, d.ld) && dalloc(&d.zdst, d.ld) && dalloc(&d.cE, d.ld) &&
         dalloc(&d.csrc, d.ld) && dalloc(&d.tagg, 3 * tiles) &&
@@ -201,12 +202,14 @@ int batch_run(BatchSet *b, const MlpSet *net, bool use_graph)
             cus = prop.multiProcessorCount;
     }
     // candidates: persistent grids of 1, 2, 4 and 8 blocks per CU, capped
-    // at one block per tile
+    // at one block per unit of the move (a tile; a 256-particle chunk of
+    // one for the learned move)
+    const long long units = net ? d.ntiles * kLearnedChunks : d.ntiles;
     std::vector<int> cand;
     for (int k : {1, 2, 4, 8}) {
         long long g = static_cast<long long>(cus) * k;
-        if (g > d.ntiles)
-            g = d.ntiles;
+        if (g > units)
+            g = units;
         if (cand.empty() || cand.back() != static_cast<int>(g))
             cand.push_back(static_cast<int>(g));
     }

@@ -633,14 +633,32 @@ __global__ void __launch_bounds__(kPB) k_cv_step(Real *__restrict__ state,
 // u = o_mu + exp(ls) z, x_t = m + s u with the transition mean m and scales
 // s, and the weight increment log g + log f - log q
 // = ll(x_t) + sum_d (ls_d + z_d^2 / 2 - u_d^2 / 2).
-// A block is one 4096-particle tile (its weight partial feeds the same
-// merge and resampling tile prefix as the bootstrap move); each wave runs
-// the network on 4 column tiles of 16 particles per 64-particle chunk.
-__device__ inline void cv_learned_tile(float *__restrict__ X,
+// A block runs chunks [ch0, ch0 + nch) of a 4096-particle tile, 256
+// particles (one per thread) per chunk; each wave runs the network on 4
+// column tiles of 16 particles of its 64 particles, and the span's weight
+// partial goes to out. A filter of more than kLearnedChunkedTiles tiles
+// runs whole tiles (nch = kLChunks), its partial the tile partial that the
+// merge and the resampling tile prefix read, as for the bootstrap move; a
+// smaller one runs one chunk per block (so that a few small filters still
+// fill the device), and merge_chunks folds a tile's chunk partials in
+// chunk order into the tile partial. The choice depends on the filter's
+// size alone, so a filter gives the same results alone and in a batch.
+constexpr int kLChunks = kLearnedChunks;
+static_assert(kLChunks * kPB == kPTile, "a learned chunk is one block of particles");
+
+__device__ inline Part3 merge_chunks(const Part3 *c)
+{
+    Part3 acc = c[0];
+    for (int k = 1; k < kLChunks; ++k)
+        acc = part_merge(acc, c[k]);
+    return acc;
+}
+
+__device__ inline void cv_learned_span(float *__restrict__ X,
     float *__restrict__ Y, float *__restrict__ VX, float *__restrict__ VY,
-    double *__restrict__ l, long long n, long long tile, uint32_t seed,
-    long long t, float ox, float oy, const CvModelDev &m,
-    const mlp::Frags *__restrict__ F, const PDev *pd, Part3 *part_out)
+    double *__restrict__ l, long long n, long long tile, int ch0, int nch,
+    uint32_t seed, long long t, float ox, float oy, const CvModelDev &m,
+    const mlp::Frags *__restrict__ F, const PDev *pd, Part3 *out)
 {
     __shared__ float fl[kWaves][64][8];
     __shared__ float ol[kWaves][64][8];
@@ -656,7 +674,7 @@ __device__ inline void cv_learned_tile(float *__restrict__ X,
     const float sv = static_cast<float>(m.sd_vel);
     const unsigned long long blk = static_cast<unsigned long long>(t) + 1;
     Part3 acc{-INFINITY, 0.0, 0.0};
-    for (int ch = 0; ch < kPTile / kPB; ++ch) {
+    for (int ch = ch0; ch < ch0 + nch; ++ch) {
         const long long i = base + ch * kPB + wave * 64 + lane;
         const bool valid = i < n;
         float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f;
@@ -723,16 +741,28 @@ __device__ inline void cv_learned_tile(float *__restrict__ X,
         }
         __syncthreads();
     }
-    block_partials_at(acc, part_out);
+    block_partials_at(acc, out);
 }
 
+// chunked: kLChunks blocks per tile, the partial of block b at out[b];
+// otherwise one block per tile
 __global__ void __launch_bounds__(kPB) k_cv_learned(float *__restrict__ state,
     double *__restrict__ l, long long n, uint32_t seed, long long t, float ox,
     float oy, CvModelDev m, const mlp::Frags *__restrict__ F, const PDev *pd,
-    Part3 *part)
+    int chunked, Part3 *out)
 {
-    cv_learned_tile(state, state + n, state + 2 * n, state + 3 * n, l, n,
-        blockIdx.x, seed, t, ox, oy, m, F, pd, part + blockIdx.x);
+    const long long tile = chunked ? blockIdx.x / kLChunks : blockIdx.x;
+    const int ch0 = chunked ? static_cast<int>(blockIdx.x % kLChunks) : 0;
+    cv_learned_span(state, state + n, state + 2 * n, state + 3 * n, l, n, tile,
+        ch0, chunked ? 1 : kLChunks, seed, t, ox, oy, m, F, pd, out + blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kPB) k_learned_merge(const Part3 *sub,
+    long long ntiles, Part3 *part)
+{
+    const long long tile = static_cast<long long>(blockIdx.x) * kPB + threadIdx.x;
+    if (tile < ntiles)
+        part[tile] = merge_chunks(sub + tile * kLChunks);
 }
 
 // Draws from the word stream (key, first word) in order: Box-Muller pairs
@@ -819,14 +849,28 @@ __global__ void __launch_bounds__(kPB) k_cv_transitions(const float *state,
 // filter runs the same per-tile code as a single filter, so its results
 // are those of a single-filter run with the same seed and observations.
 
+// kLearned: the learned moves t >= 1 only (nothing at t = 0), so that the
+// kernel's registers are the network's alone; the bootstrap variant runs
+// every step, or with init_only the initialisation alone (for a learned
+// batch, which launches both variants every step).
+template <bool kLearned>
 __global__ void __launch_bounds__(kPB) k_batch_move(float *state, double *l,
     long long ld, const BatchFilter *bf, const int *tile_filter,
     long long ntiles, const long long *tstep, const double *obs,
     long long tmax, CvModelDev m, const mlp::Frags *net, const PDev *pd,
-    Part3 *part)
+    Part3 *part, Part3 *sub, int init_only)
 {
     const long long t = *tstep;
-    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    if (kLearned ? t == 0 : (init_only && t != 0))
+        return;
+    // the learned move has kLChunks units per tile: one chunk each for a
+    // chunked filter, else the whole tile in the first unit. Units run
+    // chunk-major (u = ch ntiles + tile), so that the tiles' first units
+    // are the first blocks, spread over every XCD.
+    const int per = kLearned ? kLChunks : 1;
+    for (long long u = blockIdx.x; u < ntiles * per; u += gridDim.x) {
+        const long long tile = u % ntiles;
+        const int ch = static_cast<int>(u / ntiles);
         const int f = tile_filter[tile];
         const BatchFilter F = bf[f];
         if (t >= F.T)
@@ -837,12 +881,16 @@ __global__ void __launch_bounds__(kPB) k_batch_move(float *state, double *l,
         float *X = state + F.off;
         double *lf = l + F.off;
         const long long lt = tile - F.tile0;
-        if (t == 0)
+        if (kLearned && learned_chunked(F.ntiles))
+            cv_learned_span(X, X + ld, X + 2 * ld, X + 3 * ld, lf, F.n, lt,
+                ch, 1, F.seed, t, ox, oy, m, net, pd + f, sub + tile * kLChunks + ch);
+        else if (kLearned) {
+            if (ch == 0)
+                cv_learned_span(X, X + ld, X + 2 * ld, X + 3 * ld, lf, F.n, lt,
+                    0, kLChunks, F.seed, t, ox, oy, m, net, pd + f, part + tile);
+        } else if (t == 0)
             cv_tile<float, true, 1>(X, X + ld, X + 2 * ld, X + 3 * ld, lf, F.n,
                 lt, F.seed, t, ox, oy, m, pd + f, part + tile, nullptr);
-        else if (net)
-            cv_learned_tile(X, X + ld, X + 2 * ld, X + 3 * ld, lf, F.n, lt,
-                F.seed, t, ox, oy, m, net, pd + f, part + tile);
         else
             cv_tile<float, false, 1>(X, X + ld, X + 2 * ld, X + 3 * ld, lf, F.n,
                 lt, F.seed, t, ox, oy, m, pd + f, part + tile, nullptr);
@@ -850,9 +898,9 @@ __global__ void __launch_bounds__(kPB) k_batch_move(float *state, double *l,
 }
 
 __global__ void __launch_bounds__(kPB) k_batch_finalize(const BatchFilter *bf,
-    const long long *tstep, const Part3 *part, PDev *pd, ResampleDev *prm,
-    double threshold, int scheme, long long tmax, double *logz_hist,
-    double *ess_hist, int *rs_hist, double *tpre, int *hard)
+    const long long *tstep, Part3 *part, const Part3 *sub, PDev *pd,
+    ResampleDev *prm, double threshold, int scheme, long long tmax,
+    double *logz_hist, double *ess_hist, int *rs_hist, double *tpre, int *hard)
 {
     const int f = blockIdx.x;
     // the batched resampler's list of tiles to count exactly starts empty
@@ -864,6 +912,12 @@ __global__ void __launch_bounds__(kPB) k_batch_finalize(const BatchFilter *bf,
         if (threadIdx.x == 0)
             prm[f].active = 0;
         return;
+    }
+    if (sub && t > 0 && learned_chunked(F.ntiles)) {
+        // learned move: the tile partials from the chunk partials
+        for (long long k = threadIdx.x; k < F.ntiles; k += kPB)
+            part[F.tile0 + k] = merge_chunks(sub + (F.tile0 + k) * kLChunks);
+        __syncthreads();
     }
     weight_merge(part + F.tile0, F.ntiles, F.n, t == 0 ? 0 : 1, pd + f);
     __shared__ int act;
@@ -916,14 +970,26 @@ __global__ void k_batch_after(const BatchFilter *bf, PDev *pd,
 int batch_step(const BatchDev &b, int move_blocks, const MlpSet *net,
     hipStream_t st)
 {
-    hipLaunchKernelGGL(k_batch_move, dim3(move_blocks), dim3(kPB), 0, st,
+    const mlp::Frags *nf = net ? static_cast<const mlp::Frags *>(net->frags) : nullptr;
+    if (net) {
+        // the initialisation (t = 0) of a learned batch: the bootstrap variant
+        const long long g = move_blocks < b.ntiles ? move_blocks : b.ntiles;
+        hipLaunchKernelGGL(k_batch_move<false>, dim3(g), dim3(kPB), 0, st,
+            b.state, b.l, b.ld, b.bf, b.tile_filter, b.ntiles,
+            static_cast<const long long *>(b.tstep), b.obs, b.tmax, b.m, nf,
+            static_cast<const PDev *>(b.pd), static_cast<Part3 *>(b.part),
+            static_cast<Part3 *>(b.lsub), 1);
+    }
+    hipLaunchKernelGGL(net ? k_batch_move<true> : k_batch_move<false>,
+        dim3(move_blocks), dim3(kPB), 0, st,
         b.state, b.l, b.ld, b.bf, b.tile_filter, b.ntiles,
-        static_cast<const long long *>(b.tstep), b.obs, b.tmax, b.m,
-        net ? static_cast<const mlp::Frags *>(net->frags) : nullptr,
-        static_cast<const PDev *>(b.pd), static_cast<Part3 *>(b.part));
+        static_cast<const long long *>(b.tstep), b.obs, b.tmax, b.m, nf,
+        static_cast<const PDev *>(b.pd), static_cast<Part3 *>(b.part),
+        static_cast<Part3 *>(b.lsub), 0);
     hipLaunchKernelGGL(k_batch_finalize, dim3(b.nfilters), dim3(kPB), 0, st,
         b.bf, static_cast<const long long *>(b.tstep),
-        static_cast<const Part3 *>(b.part), b.pd, b.prm, b.threshold,
+        static_cast<Part3 *>(b.part),
+        net ? static_cast<const Part3 *>(b.lsub) : nullptr, b.pd, b.prm, b.threshold,
         b.scheme, b.tmax, b.logz_hist, b.ess_hist, b.rs_hist, b.tpre, b.hard);
     VSMCH_HIP(hipGetLastError());
     int rc = batch_resample(b.scheme, b.nfilters, b.bf, b.tile_filter,
@@ -1012,7 +1078,7 @@ void particle_destroy(ParticleSet *p)
     if (!p)
         return;
     void *ps[] = {p->state, p->logw, p->scratch, p->rep, p->part, p->mpart,
-        p->mout, p->pd, p->rng_words, p->cess_buf};
+        p->mout, p->pd, p->rng_words, p->cess_buf, p->lsub};
     for (void *q : ps)
         if (q)
             (void)hipFree(q);
@@ -1264,11 +1330,18 @@ int cv_step_learned(ParticleSet *p, const CvModelDev &m, const MlpSet *net,
     }
     hipStream_t st = p->ws.stream;
     const long long nb = pblocks(p->n);
-    hipLaunchKernelGGL(k_cv_learned, dim3(nb), dim3(kPB), 0, st,
-        static_cast<float *>(p->state), p->logw, p->n, p->seed, t,
+    const bool chunked = learned_chunked(nb);
+    if (chunked && !p->lsub)
+        VSMCH_HIP(hipMalloc(&p->lsub, sizeof(Part3) * kLChunks * nb));
+    hipLaunchKernelGGL(k_cv_learned, dim3(chunked ? nb * kLChunks : nb), dim3(kPB),
+        0, st, static_cast<float *>(p->state), p->logw, p->n, p->seed, t,
         static_cast<float>(ox), static_cast<float>(oy), m,
         static_cast<const mlp::Frags *>(net->frags), static_cast<const PDev *>(p->pd),
-        static_cast<Part3 *>(p->part));
+        chunked ? 1 : 0, static_cast<Part3 *>(chunked ? p->lsub : p->part));
+    if (chunked)
+        hipLaunchKernelGGL(k_learned_merge, dim3((nb + kPB - 1) / kPB), dim3(kPB),
+            0, st, static_cast<const Part3 *>(p->lsub), nb,
+            static_cast<Part3 *>(p->part));
     hipLaunchKernelGGL(k_weight_finalize, dim3(1), dim3(kMergeThreads), 0, st,
         static_cast<const Part3 *>(p->part), nb, p->n, 1, p->pd);
     VSMCH_HIP(hipGetLastError());

@@ -75,6 +75,7 @@ struct ParticleSet {
     unsigned long long *rng_words = nullptr;  // allocated on first request
     int *host_active = nullptr;  // pinned: the decision read by the host
     void *cess_buf = nullptr;    // conditional-ESS search (temper.hip)
+    void *lsub = nullptr;        // learned move: weight partial per chunk
     Workspace ws;
 };
 
@@ -107,6 +108,16 @@ int cv_transitions(ParticleSet *p, const CvModelDev &m, long long T,
     const double *ox, const double *oy, int scheme, double threshold,
     uint32_t train_seed, float *feat, float *target);
 
+// 256-particle chunks per 4096-particle tile of the learned move
+constexpr int kLearnedChunks = 16;
+// A filter of at most this many 4096-particle tiles runs the learned move
+// one chunk per workgroup (else one tile per workgroup).
+constexpr long long kLearnedChunkedTiles = 2;
+__host__ __device__ inline bool learned_chunked(long long ntiles)
+{
+    return ntiles <= kLearnedChunkedTiles;
+}
+
 // Device arrays of a batch of filters (batch.hip owns them).
 struct BatchDev {
     int nfilters = 0;
@@ -118,6 +129,7 @@ struct BatchDev {
     float *state = nullptr;   // 4 x ld FP32 (x, y, vx, vy)
     double *l = nullptr;      // ld log-weights
     void *part = nullptr;     // per-tile weight partials
+    void *lsub = nullptr;     // learned move: weight partial per 256-particle chunk
     PDev *pd = nullptr;       // per filter
     ResampleDev *prm = nullptr;
     int *rep = nullptr;
@@ -140,7 +152,8 @@ struct BatchDev {
     double threshold = 0.5;
 };
 
-// Enqueues one step of every filter (four launches; see particle.hip).
+// Enqueues one step of every filter (four launches, five with the learned
+// proposal; see particle.hip).
 int batch_step(const BatchDev &b, int move_blocks, const MlpSet *net,
     hipStream_t st);
 int batch_weights(const BatchDev &b, int f, const BatchFilter &F, double *W,
