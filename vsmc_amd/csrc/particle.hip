@@ -744,6 +744,18 @@ __device__ inline void cv_learned_span(float *__restrict__ X,
     block_partials_at(acc, out);
 }
 
+// The network's fragments copied into LDS by the whole block (the forward
+// tiles then read them from LDS rather than through the vector cache).
+static_assert(sizeof(mlp::Frags) % 16 == 0, "fragments copy in 16-byte units");
+__device__ inline void frags_to_lds(const mlp::Frags *F, mlp::Frags *dst)
+{
+    const uint4 *s = reinterpret_cast<const uint4 *>(F);
+    uint4 *d = reinterpret_cast<uint4 *>(dst);
+    for (int k = threadIdx.x; k < static_cast<int>(sizeof(mlp::Frags) / 16); k += kPB)
+        d[k] = s[k];
+    __syncthreads();
+}
+
 // chunked: kLChunks blocks per tile, the partial of block b at out[b];
 // otherwise one block per tile
 __global__ void __launch_bounds__(kPB) k_cv_learned(float *__restrict__ state,
@@ -751,10 +763,12 @@ __global__ void __launch_bounds__(kPB) k_cv_learned(float *__restrict__ state,
     float oy, CvModelDev m, const mlp::Frags *__restrict__ F, const PDev *pd,
     int chunked, Part3 *out)
 {
+    __shared__ mlp::Frags sF;
+    frags_to_lds(F, &sF);
     const long long tile = chunked ? blockIdx.x / kLChunks : blockIdx.x;
     const int ch0 = chunked ? static_cast<int>(blockIdx.x % kLChunks) : 0;
     cv_learned_span(state, state + n, state + 2 * n, state + 3 * n, l, n, tile,
-        ch0, chunked ? 1 : kLChunks, seed, t, ox, oy, m, F, pd, out + blockIdx.x);
+        ch0, chunked ? 1 : kLChunks, seed, t, ox, oy, m, &sF, pd, out + blockIdx.x);
 }
 
 __global__ void __launch_bounds__(kPB) k_learned_merge(const Part3 *sub,
@@ -868,6 +882,12 @@ __global__ void __launch_bounds__(kPB) k_batch_move(float *state, double *l,
     // chunk-major (u = ch ntiles + tile), so that the tiles' first units
     // are the first blocks, spread over every XCD.
     const int per = kLearned ? kLChunks : 1;
+    const mlp::Frags *nf = net;
+    if constexpr (kLearned) {
+        __shared__ mlp::Frags sF;
+        frags_to_lds(net, &sF);
+        nf = &sF;
+    }
     for (long long u = blockIdx.x; u < ntiles * per; u += gridDim.x) {
         const long long tile = u % ntiles;
         const int ch = static_cast<int>(u / ntiles);
@@ -883,11 +903,11 @@ __global__ void __launch_bounds__(kPB) k_batch_move(float *state, double *l,
         const long long lt = tile - F.tile0;
         if (kLearned && learned_chunked(F.ntiles))
             cv_learned_span(X, X + ld, X + 2 * ld, X + 3 * ld, lf, F.n, lt,
-                ch, 1, F.seed, t, ox, oy, m, net, pd + f, sub + tile * kLChunks + ch);
+                ch, 1, F.seed, t, ox, oy, m, nf, pd + f, sub + tile * kLChunks + ch);
         else if (kLearned) {
             if (ch == 0)
                 cv_learned_span(X, X + ld, X + 2 * ld, X + 3 * ld, lf, F.n, lt,
-                    0, kLChunks, F.seed, t, ox, oy, m, net, pd + f, part + tile);
+                    0, kLChunks, F.seed, t, ox, oy, m, nf, pd + f, part + tile);
         } else if (t == 0)
             cv_tile<float, true, 1>(X, X + ld, X + 2 * ld, X + 3 * ld, lf, F.n,
                 lt, F.seed, t, ox, oy, m, pd + f, part + tile, nullptr);
