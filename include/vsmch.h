@@ -10,6 +10,12 @@
  *    (include/vsmc/vsmc.h:286-310); status codes replace its asserts.
  *  - Work is queued on the library stream (vsmch_set_stream; default: the
  *    null stream). Functions that return host values synchronise it.
+ *  - Threads: the library stream, the last error and the scratch buffers of
+ *    the free functions (resample transforms, u01 sequences, schemes) belong
+ *    to the calling host thread, so those functions may be called from
+ *    several host threads at once, as the reference's vsmc_resample_*
+ *    (lib/src/vsmc_resample.cpp) can. An object (particle set, batch,
+ *    network, genealogy) is used by one host thread at a time.
  *
  * Each entry point cites the reference interface it replaces.
  */
@@ -47,9 +53,10 @@ typedef enum {
     VSMCH_RESAMPLE_RESIDUAL_SYSTEMATIC = 5
 } vsmch_resample_scheme;
 
+/* The calling host thread's last error. */
 const char *vsmch_last_error(void);
 int vsmch_version(void);
-/* Library stream (a hipStream_t). */
+/* Library stream (a hipStream_t) of the calling host thread. */
 vsmch_status vsmch_set_stream(void *stream);
 vsmch_status vsmch_synchronize(void);
 

@@ -12,7 +12,8 @@ using namespace vsmch;
 
 namespace {
 
-hipStream_t g_stream = nullptr;
+// The library stream of the calling host thread (vsmch_set_stream).
+thread_local hipStream_t g_stream = nullptr;
 
 Workspace &ws_current()
 {
@@ -52,6 +53,8 @@ CvModelDev cv_dev(const vsmch_cv_model *m)
 }
 
 } // namespace
+
+hipStream_t vsmch::library_stream() { return g_stream; }
 
 struct vsmch_particle {
     ParticleSet *p;

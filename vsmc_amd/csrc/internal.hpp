@@ -81,6 +81,10 @@ struct Workspace {
     int *integ_dummy = nullptr;
     ResampleDev *prm = nullptr;
     long long *scal = nullptr;   // small device scalars
+    // fixed: the buffers are views into memory the workspace does not own
+    // (a caller's or a file-scope device array); ensure() then never frees
+    // or allocates, it only checks that n fits cap_n / cap_tiles
+    bool fixed = false;
     std::string err;
 
     bool ensure(long long n);
@@ -88,7 +92,18 @@ struct Workspace {
     bool all_allocated() const;
 };
 
+// The calling host thread's workspace on `device` (created on first use).
+// Each thread has its own, so concurrent calls from several host threads
+// never share buffers; a thread's workspace returns to a per-device pool
+// when the thread exits and the next new thread takes it over, after a
+// device synchronisation (work the old thread queued may still read it).
 Workspace &default_workspace(int device);
+
+// The library stream of the calling host thread (vsmch_set_stream).
+hipStream_t library_stream();
+
+// Tiles the fold passes hold a summary for, for n weights (one spare).
+inline long long workspace_tiles(long long n) { return (n + kTile - 1) / kTile + 1; }
 
 // GroupSeg: 64 consecutive tiles that share one binade and have no crossing.
 struct GroupSeg {

@@ -1547,7 +1547,13 @@ __global__ void __launch_bounds__(kBlock) k_batch_walk(const BatchFilter *bf,
         for (int q = threadIdx.x; q < nw * kWords; q += kBlock)
             to[q] = from[q];
         __syncthreads();
-        while (next < w0 + nw) {
+        for (;;) {
+            // every thread takes its copy of `next` before thread 0 may
+            // write it again, so all waves leave the loop together
+            const long long cur = next;
+            __syncthreads();
+            if (cur >= w0 + nw)
+                break;
             // thread 0 applies summaries until one does not hold
             if (threadIdx.x == 0) {
                 double x = xs;
@@ -1824,28 +1830,24 @@ __global__ void k_words_used(ResampleDev *p, const int *rep, int scheme,
 
 // ------------------------------------------------------------ host side
 
-static std::mutex g_err_mu;
-static std::string g_err;
+// The last error of the calling host thread.
+static thread_local std::string g_err;
 
-void set_error(const std::string &msg)
-{
-    std::lock_guard<std::mutex> lk(g_err_mu);
-    g_err = msg;
-}
+void set_error(const std::string &msg) { g_err = msg; }
 
-const char *get_error()
-{
-    static thread_local std::string copy;
-    std::lock_guard<std::mutex> lk(g_err_mu);
-    copy = g_err;
-    return copy.c_str();
-}
+const char *get_error() { return g_err.c_str(); }
 
 bool Workspace::ensure(long long n)
 {
-    const long long tiles = (n + kTile - 1) / kTile + 1;
+    const long long tiles = workspace_tiles(n);
     if (n <= cap_n && tiles <= cap_tiles && prm)
         return true;
+    if (fixed) {
+        set_error("size " + std::to_string(n) +
+            " exceeds the fixed workspace (at most " + std::to_string(cap_n) +
+            " weights)");
+        return false;
+    }
     release();
     const long long nn = n < 1 ? 1 : n;
     const long long ngroups = tiles / kGroup + 2;
@@ -1896,6 +1898,8 @@ bool Workspace::all_allocated() const
 
 void Workspace::release()
 {
+    if (fixed)
+        return;
     void *ps[] = {tsum, tpre, segs, xstart, xgstart, gmode, groups, tagg, toff,
         zdst, cE, csrc, v, aux, rep, hard, prm, scal};
     for (void *p : ps)
@@ -1910,18 +1914,66 @@ void Workspace::release()
     cap_n = cap_tiles = 0;
 }
 
+namespace {
+
+// Workspaces of exited host threads, per device. Never destroyed: a
+// thread-exit handler may run during process exit.
+struct WsPool {
+    std::mutex mu;
+    std::vector<std::vector<Workspace *>> spare;
+};
+
+WsPool &ws_pool()
+{
+    static WsPool *p = new WsPool();
+    return *p;
+}
+
+// A host thread's workspaces; at thread exit they go back to the pool (no
+// device call here: the runtime may already be shutting down).
+struct ThreadWs {
+    std::vector<Workspace *> ws;
+    ~ThreadWs()
+    {
+        WsPool &pool = ws_pool();
+        std::lock_guard<std::mutex> lk(pool.mu);
+        for (size_t d = 0; d < ws.size(); ++d) {
+            if (!ws[d])
+                continue;
+            if (pool.spare.size() <= d)
+                pool.spare.resize(d + 1);
+            pool.spare[d].push_back(ws[d]);
+        }
+    }
+};
+
+} // namespace
+
 Workspace &default_workspace(int device)
 {
-    static std::mutex mu;
-    static std::vector<Workspace *> ws;
-    std::lock_guard<std::mutex> lk(mu);
+    thread_local ThreadWs mine;
     if (device < 0)
         device = 0;
-    if (static_cast<int>(ws.size()) <= device)
-        ws.resize(device + 1, nullptr);
-    if (!ws[device])
-        ws[device] = new Workspace();
-    return *ws[device];
+    const size_t d = static_cast<size_t>(device);
+    if (mine.ws.size() <= d)
+        mine.ws.resize(d + 1, nullptr);
+    if (!mine.ws[d]) {
+        Workspace *w = nullptr;
+        {
+            WsPool &pool = ws_pool();
+            std::lock_guard<std::mutex> lk(pool.mu);
+            if (pool.spare.size() > d && !pool.spare[d].empty()) {
+                w = pool.spare[d].back();
+                pool.spare[d].pop_back();
+            }
+        }
+        if (w)
+            (void)hipDeviceSynchronize();   // the old owner's queued work
+        else
+            w = new Workspace();
+        mine.ws[d] = w;
+    }
+    return *mine.ws[d];
 }
 
 static inline unsigned int grid_tiles(long long n)
