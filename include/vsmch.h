@@ -90,6 +90,16 @@ vsmch_status vsmch_normal_fill(int kind, uint64_t key0, uint64_t word_offset,
 vsmch_status vsmch_resample_trans_u01_rep(int m, int n, const double *weight,
     const double *u01, int *replication);
 
+/* The counts of vsmch_resample_trans_u01_rep without allocating: no
+ * counterpart in the reference. For callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Its scratch is one file-scope device array (about
+ * 33 MiB per device, held until the process ends), so m and n are at most
+ * 4,194,304 (VSMCH_ERR_INVALID beyond). Every call on one device shares
+ * that array: calls that overlap on one device race. */
+vsmch_status vsmch_resample_trans_u01_rep_noalloc(int m, int n,
+    const double *weight, const double *u01, int *replication);
+
 /* resample_trans_u01_index (transform.hpp:75-95); replaces
  * vsmc_resample_trans_u01_index (include/vsmc/vsmc.h:288-289). */
 vsmch_status vsmch_resample_trans_u01_index(int m, int n,

@@ -48,6 +48,7 @@ SIGNATURES = {
     "vsmch_normal_fill": (_c.c_int, [_c.c_int, _c.c_uint64, _c.c_uint64, _c.c_size_t,
                                      _c.c_double, _c.c_double, _c.c_int, _P]),
     "vsmch_resample_trans_u01_rep": (_c.c_int, [_c.c_int, _c.c_int, _P, _P, _P]),
+    "vsmch_resample_trans_u01_rep_noalloc": (_c.c_int, [_c.c_int, _c.c_int, _P, _P, _P]),
     "vsmch_resample_trans_u01_index": (_c.c_int, [_c.c_int, _c.c_int, _P, _P, _P]),
     "vsmch_resample_trans_rep_index": (_c.c_int, [_c.c_int, _c.c_int, _P, _P]),
     "vsmch_resample_trans_index_rep": (_c.c_int, [_c.c_int, _c.c_int, _P, _P]),
@@ -267,6 +268,24 @@ def resample_trans_u01_rep(weight, u01, m=None, n=None):
     n = u01.numel() if n is None else n
     rep = torch.empty(max(m, 1), dtype=torch.int32, device=weight.device)
     check(lib().vsmch_resample_trans_u01_rep(m, n, _ptr(weight), _ptr(u01), _ptr(rep)))
+    _stream_sync()
+    return rep[:m]
+
+
+NOALLOC_MAX = 1 << 22
+
+
+def resample_trans_u01_rep_noalloc(weight, u01, m=None, n=None, rep=None):
+    """The counts of resample_trans_u01_rep from the library's fixed device
+    scratch (vsmch_resample_trans_u01_rep_noalloc: no device allocation in
+    the library; m, n <= NOALLOC_MAX). `rep` may be passed in."""
+    import torch
+    m = weight.numel() if m is None else m
+    n = u01.numel() if n is None else n
+    if rep is None:
+        rep = torch.empty(max(m, 1), dtype=torch.int32, device=weight.device)
+    check(lib().vsmch_resample_trans_u01_rep_noalloc(m, n, _ptr(weight), _ptr(u01),
+                                                     _ptr(rep)))
     _stream_sync()
     return rep[:m]
 
