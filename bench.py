@@ -42,6 +42,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--sections", default="all",
+                    help="comma-separated subset of step,extra,micro,learned,batched,gmm "
+                         "(profiling runs); 'all' runs every one")
     return ap.parse_args()
 
 
@@ -76,6 +79,28 @@ def time_config(va, torch, n, scheme, thr, warmup, steps, ox, oy, seed=1,
     ms = e0.elapsed_time(e1)
     nres = f.resample_count() - r0
     return f, t, ms, nres
+
+
+def time_iterate(va, torch, n, scheme, thr, warmup, steps, ox, oy, seed=1):
+    """The Sampler loop as vsmch_cv_iterate runs it (initialize + warmup
+    moves untimed, then `steps` timed moves with their events, histories and
+    the "pos" monitor), HIP events on the library stream."""
+    f = va.CvFilter(n, "float32", seed=seed)
+    h = f.history_buffers(steps)
+    f.iterate(0, ox[:warmup + 1], oy[:warmup + 1], scheme, thr, histories=False)
+    torch.cuda.synchronize()
+    r0 = f.resample_count()
+    t0 = warmup + 1
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    f.iterate(t0, ox[t0:t0 + steps], oy[t0:t0 + steps], scheme, thr, histories=h)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    nres = f.resample_count() - r0
+    assert int(h["resampled"].sum().item()) == nres
+    return f, t0 + steps, ms, nres
 
 
 def move_kernel_ms(va, torch, f, t, reps, ox, oy):
@@ -373,14 +398,14 @@ def tempered_gmm(va, torch, n=1 << 20, moves=5, cpu_budget=4.0):
                                        f"{cs:.2f}s"}}
 
 
-def measured_traffic():
-    """HBM bytes per move launch from the committed rocprofv3 counter
+def measured_traffic(name="cv_step_traffic.json", key="bytes_per_step"):
+    """HBM bytes per step (or per launch) from a committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy kernel), if any."""
-    p = os.path.join(ROOT, "profiles", "cv_move_traffic.json")
+    p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
         return None
     with open(p) as fh:
-        return json.load(fh).get("bytes_per_launch")
+        return json.load(fh).get(key)
 
 
 def cpu_baseline(budget_s):
@@ -418,9 +443,26 @@ def main():
     scheme = va.SCHEMES[a.scheme]
     T = a.warmup + a.steps + 64
     _, ox, oy = cv_simulate(T, 20161)
+    sections = ({"step", "extra", "micro", "learned", "batched", "gmm"} if a.sections == "all"
+                else set(a.sections.split(",")))
+    if a.no_extra:
+        sections &= {"step"}
+    if "step" not in sections:
+        # one workload alone (e.g. under a profiler): its figures only
+        out = {}
+        for name, fn in (("extra", None), ("micro", microbench), ("learned", learned_vs_plain),
+                         ("batched", batched_filters), ("gmm", tempered_gmm)):
+            if name in sections and fn is not None:
+                out[name] = fn(va, torch)
+        print(json.dumps(out), flush=True)
+        return
 
-    f, t, ms, nres = time_config(va, torch, a.n, scheme, a.threshold, a.warmup, a.steps,
-                                 ox, oy)
+    f, t, ms, nres = time_iterate(va, torch, a.n, scheme, a.threshold, a.warmup, a.steps,
+                                  ox, oy)
+    f.close()
+    # the step-level API (move, then an event that gathers in place)
+    f, t, ms_api, nres_api = time_config(va, torch, a.n, scheme, a.threshold, a.warmup,
+                                         a.steps, ox, oy)
     reps = min(20, T - t - 2)
     mk_ms = move_kernel_ms(va, torch, f, t, reps, ox, oy)
     f.close()
@@ -430,6 +472,7 @@ def main():
     secs = ms / 1e3
     value = a.n * a.steps / secs
     bytes_alg = a.n * (MOVE_BYTES * a.steps + RESAMPLE_BYTES * nres)
+    step_gbs = bytes_alg / secs / 1e9
     achieved = MOVE_BYTES * a.n / (mk_ms / 1e3) / 1e9
     out = {
         "metric": "particle-updates/s at N=2^24 (%HBM roofline)",
@@ -448,31 +491,48 @@ def main():
                    "weights": "fp64", "scheme": a.scheme, "threshold": a.threshold,
                    "model": "CV dt=0.1 sd 0.02, Student-t obs df=10 scale 0.1",
                    "parallelism": "single GPU"},
-        "step_hbm_frac": bytes_alg / secs / 1e9 / HBM_PEAK_GBS,
+        "step_hbm_frac": step_gbs / HBM_PEAK_GBS,
         "resampled_steps": nres,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+        # the whole step: algorithmic bytes N (48 + 48 r_t) summed over the
+        # timed steps / the timed region
+        "roofline": {"bound": "hbm", "achieved": step_gbs, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS,
                      "traffic": measured_traffic(),
-                     "kernel": "k_cv_step<float> + weight merge",
-                     "kernel_ms": mk_ms},
+                     "kernel": "PF step: move + finalize + resampling event (all kernels)",
+                     "step_ms": ms / a.steps},
+        "roofline_move": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                          "traffic": measured_traffic("cv_move_traffic.json",
+                                                      "bytes_per_launch"),
+                          "kernel": "k_cv_step<float> + weight merge",
+                          "kernel_ms": mk_ms},
+        "step_api": {"ms_per_step": ms_api / a.steps, "resampled_steps": nres_api,
+                     "updates_per_s": a.n * a.steps / (ms_api / 1e3),
+                     "step_hbm_frac": a.n * (MOVE_BYTES * a.steps + RESAMPLE_BYTES * nres_api)
+                     / (ms_api / 1e3) / 1e9 / HBM_PEAK_GBS},
         "copy_gbs": copy_gbs,
     }
-    if not a.no_extra:
+    if "extra" in sections:
         extra = {}
-        for name, sch, thr in (("systematic_always", va.SYSTEMATIC, float("inf")),
+        for name, sch, thr in (("systematic_never", va.SYSTEMATIC, 0.0),
+                               ("systematic_always", va.SYSTEMATIC, float("inf")),
                                ("stratified", va.STRATIFIED, a.threshold),
                                ("multinomial", va.MULTINOMIAL, a.threshold),
                                ("residual", va.RESIDUAL, a.threshold)):
-            g, _, ms2, nres2 = time_config(va, torch, a.n, sch, thr, 2, 20, ox, oy)
+            g, _, ms2, nres2 = time_iterate(va, torch, a.n, sch, thr, 2, 20, ox, oy)
             g.close()
             extra[name] = {"ms_per_step": ms2 / 20, "resampled_steps": nres2,
                            "updates_per_s": a.n * 20 / (ms2 / 1e3),
                            "step_hbm_frac": a.n * (MOVE_BYTES * 20 + RESAMPLE_BYTES * nres2)
                            / (ms2 / 1e3) / 1e9 / HBM_PEAK_GBS}
         out["extra"] = extra
+    if "micro" in sections:
         out["microbench"] = microbench(va, torch)
+    if "learned" in sections:
         out["learned_proposal"] = learned_vs_plain(va, torch)
+    if "batched" in sections:
         out["batched_filters"] = batched_filters(va, torch)
+    if "gmm" in sections:
         out["tempered_gmm"] = tempered_gmm(va, torch)
     out["cpu_baseline"] = None if a.no_cpu else cpu_baseline(a.cpu_seconds)
     print(json.dumps(out), flush=True)

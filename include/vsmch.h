@@ -294,6 +294,19 @@ vsmch_status vsmch_cv_run(vsmch_particle *p, const vsmch_cv_model *m,
     int scheme, double threshold, double *ess_host, int *resampled_host,
     double *pos_host, double *logz_host);
 
+/* Steps iter0 .. iter0 + k - 1 of the same loop: Sampler::initialize when
+ * iter0 == 0, then iterate (include/vsmc/core/sampler.hpp:443-476,
+ * :746-778), queued on the library stream without synchronising. Each
+ * move reads every particle through the ancestors of the previous event
+ * (two state buffers in turn), so the events gather nothing; the state is
+ * in the particle set's own buffer when the call's work completes.
+ * obs_x/obs_y: host arrays of length k. Histories (device, length k; pos
+ * 2k; each may be NULL): as vsmch_cv_run, indexed from iter0. */
+vsmch_status vsmch_cv_iterate(vsmch_particle *p, const vsmch_cv_model *m,
+    int64_t iter0, int64_t k, const double *obs_x_host,
+    const double *obs_y_host, int scheme, double threshold, double *ess,
+    int *resampled, double *pos, double *logz);
+
 /* ------------------------------------------------ learned proposal ----
  * No counterpart in the reference, which has only the bootstrap proposal
  * (the transition, user_guide/cpp/pf_seq.cpp:111-122). A 6 -> 64 -> 64 -> 8

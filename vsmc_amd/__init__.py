@@ -102,6 +102,8 @@ SIGNATURES = {
     "vsmch_cv_move": (_c.c_int, [_P, _P, _c.c_int64, _c.c_double, _c.c_double]),
     "vsmch_cv_run": (_c.c_int, [_P, _P, _c.c_int64, _P, _P, _c.c_int, _c.c_double,
                                 _P, _P, _P, _P]),
+    "vsmch_cv_iterate": (_c.c_int, [_P, _P, _c.c_int64, _c.c_int64, _P, _P, _c.c_int,
+                                    _c.c_double, _P, _P, _P, _P]),
     "vsmch_rindex_create": (_c.c_int, [_c.POINTER(_P), _c.c_int64]),
     "vsmch_rindex_destroy": (_c.c_int, [_P]),
     "vsmch_rindex_iter_size": (_c.c_int, [_P, _c.POINTER(_c.c_int64)]),

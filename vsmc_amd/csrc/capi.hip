@@ -559,6 +559,20 @@ vsmch_status vsmch_cv_run(vsmch_particle *p, const vsmch_cv_model *m,
         threshold, ess_host, resampled_host, pos_host, logz_host);
 }
 
+vsmch_status vsmch_cv_iterate(vsmch_particle *p, const vsmch_cv_model *m,
+    int64_t iter0, int64_t k, const double *obs_x_host,
+    const double *obs_y_host, int scheme, double threshold, double *ess,
+    int *resampled, double *pos, double *logz)
+{
+    if (!p || !m || iter0 < 0 || k < 0 || scheme < 0 || scheme > 5 ||
+        (k > 0 && (!obs_x_host || !obs_y_host))) {
+        set_error("vsmch_cv_iterate: invalid argument");
+        return VSMCH_ERR_INVALID;
+    }
+    return cv_iterate(p->p, cv_dev(m), iter0, k, obs_x_host, obs_y_host,
+        scheme, threshold, ess, resampled, pos, logz);
+}
+
 vsmch_status vsmch_mlp_create(vsmch_mlp **m)
 {
     if (!m)

@@ -55,6 +55,7 @@ struct EventEnd {
     long long n = 0;
     int scheme = 0;
     int words = 0;
+    int deferred = 0;   // ancestors written for the next move, state not gathered
 };
 
 // Grow-only device workspace shared by the resampling passes.
@@ -77,7 +78,7 @@ struct Workspace {
     double *v = nullptr;         // materialised uniforms (running max)
     double *aux = nullptr;       // scratch N doubles
     int *rep = nullptr;          // scratch counts
-    int *hard = nullptr;         // [0]: count, then tiles k_count could not fold
+    int *hard = nullptr;         // batched filters: [0] count, then tiles to fold in full
     int *integ_dummy = nullptr;
     ResampleDev *prm = nullptr;
     long long *scal = nullptr;   // small device scalars

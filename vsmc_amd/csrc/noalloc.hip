@@ -36,8 +36,7 @@ constexpr size_t kOffXgstart = kOffXstart + align256(sizeof(double) * (kNaTiles 
 constexpr size_t kOffGmode = kOffXgstart + align256(sizeof(double) * (kNaGroups + 1));
 constexpr size_t kOffGroups = kOffGmode + align256(sizeof(int) * (kNaGroups + 1));
 constexpr size_t kOffTagg = kOffGroups + align256(sizeof(GroupSeg) * (kNaGroups + 1));
-constexpr size_t kOffHard = kOffTagg + align256(sizeof(long long) * 3 * kNaTiles);
-constexpr size_t kOffPrm = kOffHard + align256(sizeof(int) * (kNaTiles + 1));
+constexpr size_t kOffPrm = kOffTagg + align256(sizeof(long long) * 3 * kNaTiles);
 constexpr size_t kOffV = kOffPrm + align256(sizeof(ResampleDev));
 constexpr size_t kScratchBytes = kOffV + align256(sizeof(double) * kNoallocMax);
 
@@ -96,7 +95,6 @@ Workspace scratch_workspace(unsigned char *base, hipStream_t st)
     ws.gmode = reinterpret_cast<int *>(base + kOffGmode);
     ws.groups = reinterpret_cast<GroupSeg *>(base + kOffGroups);
     ws.tagg = reinterpret_cast<long long *>(base + kOffTagg);
-    ws.hard = reinterpret_cast<int *>(base + kOffHard);
     ws.prm = reinterpret_cast<ResampleDev *>(base + kOffPrm);
     ws.v = reinterpret_cast<double *>(base + kOffV);
     return ws;

@@ -15,6 +15,8 @@
 #include "mlp.hpp"
 #include "../../include/vsmch/device_rng.hpp"
 
+#include <utility>
+
 namespace vsmch {
 
 constexpr int kPB = 256;              // threads per particle block
@@ -290,6 +292,88 @@ __global__ void __launch_bounds__(kMergeThreads) k_prepare(PDev *pd, ResampleDev
         [&](long long b, double ex) { tpre[b] = ex; }, OpAddD(), 0.0, lds);
 }
 
+// A filter step's bookkeeping after its move, one block of kMergeThreads:
+// what k_weight_finalize, k_moments_finalize, the log-evidence record and
+// k_prepare (k_decide) do in turn, in one launch. Merges the move's weight
+// partials (mode 0: set_log at the initialisation, 1: add_log), records the
+// log-evidence, the previous step's monitor from the move's partials
+// (pos_prev, when given), the ESS before resampling and the decision
+// (sampler.hpp:772-778), and for systematic / stratified events (prep) the
+// draw parameters and the approximate tile prefix. The move has read the
+// ancestors a previous event left, so anc_valid is cleared.
+__global__ void __launch_bounds__(kMergeThreads) k_step_finalize(
+    const Part3 *part, long long nb, long long n, int mode, PDev *pd,
+    ResampleDev *prm, double threshold, long long h, double *ess_hist,
+    int *rs_hist, double *logz_hist, const double *mpart, double *pos_prev,
+    int scheme, int prep, double *tpre)
+{
+    weight_merge<kMergeThreads>(part, nb, n, mode, pd);
+    __shared__ int act;
+    __shared__ int sh_equal;
+    __shared__ double sh_lmax, sh_inv;
+    __shared__ double lds[kMergeThreads / kWave];
+    if (pos_prev) {
+        for (int d = 0; d < 2; ++d) {
+            double acc = 0.0;
+            for (long long b = threadIdx.x; b < nb; b += kMergeThreads)
+                acc += mpart[b * 8 + d];
+            acc = block_reduce_n<kMergeThreads>(acc, OpAddD(), lds);
+            if (threadIdx.x == 0)
+                pos_prev[d] = acc;
+        }
+    }
+    if (threadIdx.x == 0) {
+        pd->anc_valid = 0;
+        if (logz_hist)
+            logz_hist[h] = pd->logz;
+        if (ess_hist)
+            ess_hist[h] = pd->ess;
+        const int active = decide_event(pd, prm, n, threshold, scheme);
+        if (rs_hist)
+            rs_hist[h] = active;
+        act = active;
+        sh_equal = pd->equal;
+        sh_lmax = pd->lmax;
+        sh_inv = pd->equal ? 1.0 : 1 / pd->sum;
+    }
+    __syncthreads();
+    if (!act || !prep)
+        return;
+    const int equal = sh_equal;
+    const double lmax = sh_lmax;
+    const double inv = sh_inv;
+    const double inv_n = 1 / static_cast<double>(n);
+    block_scan_chunked_n<kMergeThreads, 4>(nb,
+        [&](long long b) {
+            if (equal) {
+                const long long rem = n - b * kPTile;
+                return static_cast<double>(rem < kPTile ? rem : kPTile) * inv_n;
+            }
+            const Part3 q = part[b];
+            return q.s > 0.0 ? q.s * exp(q.m - lmax) * inv : 0.0;
+        },
+        [&](long long b, double ex) { tpre[b] = ex; }, OpAddD(), 0.0, lds);
+}
+
+// The state after the last move of cv_iterate, into the particle set's own
+// buffer home: particle i from cur at anc[i] when the last event left its
+// ancestors (pd->anc_valid), else at i. With cur == home it runs in place
+// and writes only the slots whose ancestor is another particle: such an
+// ancestor has copies, so it keeps its own slot and is never written.
+template <typename Real>
+__global__ void __launch_bounds__(kPB) k_settle(const Real *cur, Real *home,
+    const int *anc, const PDev *pd, long long n, int dim)
+{
+    const long long i = static_cast<long long>(blockIdx.x) * kPB + threadIdx.x;
+    if (i >= n)
+        return;
+    const long long a = pd->anc_valid ? static_cast<long long>(anc[i]) : i;
+    if (cur == home && a == i)
+        return;
+    for (int d = 0; d < dim; ++d)
+        home[d * n + i] = cur[d * n + a];
+}
+
 // Weighted moments r_d = sum_i W_i x_d(i) of components [c0, c0 + dim)
 // (dim <= 8); per-block partials of sum exp(l - lmax) x and the merge.
 template <typename Real>
@@ -480,16 +564,22 @@ __device__ inline Part3 part_of_array(const double (&v)[K])
     return p;
 }
 
-// Tile `tile` (kPTile particles) of a filter of n particles whose state
-// components are X, Y, VX, VY; writes the tile's weight partial to
+// Tile `tile` (kPTile particles) of a filter of n particles whose moved
+// state components go to X, Y, VX, VY; writes the tile's weight partial to
 // *part_out and, with mon_out, the monitor sums of the pre-move state.
+// The pre-move state is read from SX, SY, SVX, SVY (which may be X, Y, VX,
+// VY, for a move in place) at particle i, or at its ancestor anc[i] when the
+// last event left its ancestors there (pd->anc_valid): such a move also
+// performs the event's gather.
 template <typename Real, bool kInit, int kB>
-__device__ inline void cv_tile(Real *__restrict__ X, Real *__restrict__ Y,
-    Real *__restrict__ VX, Real *__restrict__ VY, double *__restrict__ l,
+__device__ inline void cv_tile(const Real *SX, const Real *SY, const Real *SVX,
+    const Real *SVY, const int *anc, Real *X, Real *Y, Real *VX, Real *VY,
+    double *__restrict__ l,
     long long n, long long tile, uint32_t seed, long long t, Real ox, Real oy,
     const CvModelDev &m, const PDev *pd, Part3 *part_out, double *mon_out)
 {
     const bool mon = mon_out != nullptr;
+    const int *A = (anc != nullptr && pd->anc_valid) ? anc : nullptr;
     const long long base = tile * kPTile;
     Part3 acc{-INFINITY, 0.0, 0.0};
     double mx = 0.0, my = 0.0;
@@ -511,10 +601,22 @@ __device__ inline void cv_tile(Real *__restrict__ X, Real *__restrict__ Y,
             for (int r = 0; r < kB; ++r) {
                 const long long i0 = base + ((r0 + r) * kPB + threadIdx.x) * 4;
                 if (!kInit) {
-                    xs[r] = *reinterpret_cast<const Vec4<Real> *>(X + i0);
-                    ys[r] = *reinterpret_cast<const Vec4<Real> *>(Y + i0);
-                    vxs[r] = *reinterpret_cast<const Vec4<Real> *>(VX + i0);
-                    vys[r] = *reinterpret_cast<const Vec4<Real> *>(VY + i0);
+                    if (A) {
+                        const Vec4<int> a4 = *reinterpret_cast<const Vec4<int> *>(A + i0);
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const unsigned a = static_cast<unsigned>(a4.v[q]);
+                            xs[r].v[q] = SX[a];
+                            ys[r].v[q] = SY[a];
+                            vxs[r].v[q] = SVX[a];
+                            vys[r].v[q] = SVY[a];
+                        }
+                    } else {
+                        xs[r] = *reinterpret_cast<const Vec4<Real> *>(SX + i0);
+                        ys[r] = *reinterpret_cast<const Vec4<Real> *>(SY + i0);
+                        vxs[r] = *reinterpret_cast<const Vec4<Real> *>(SVX + i0);
+                        vys[r] = *reinterpret_cast<const Vec4<Real> *>(SVY + i0);
+                    }
                     if (!equal) {
                         const Vec4<double> lq = *reinterpret_cast<const Vec4<double> *>(l + i0);
 #pragma unroll
@@ -578,10 +680,11 @@ __device__ inline void cv_tile(Real *__restrict__ X, Real *__restrict__ Y,
             Real x{}, y{}, vx{}, vy{};
             double lold = 0.0;
             if (!kInit) {
-                x = X[i];
-                y = Y[i];
-                vx = VX[i];
-                vy = VY[i];
+                const long long a = A ? static_cast<long long>(A[i]) : i;
+                x = SX[a];
+                y = SY[a];
+                vx = SVX[a];
+                vy = SVY[a];
                 lold = equal ? lw_equal : l[i] - c;
                 if (mon) {
                     const double w = exp(lold);
@@ -614,14 +717,18 @@ __device__ inline void cv_tile(Real *__restrict__ X, Real *__restrict__ Y,
     }
 }
 
+// One move over all tiles: from src (through the ancestors anc when the
+// last event left them, see cv_tile) into state; src == state moves in place.
 template <typename Real, bool kInit, int kB>
-__global__ void __launch_bounds__(kPB) k_cv_step(Real *__restrict__ state,
-    double *__restrict__ l, long long n, uint32_t seed, long long t, Real ox,
-    Real oy, CvModelDev m, const PDev *pd, Part3 *part, double *mon)
+__global__ void __launch_bounds__(kPB) k_cv_step(Real *state, const Real *src,
+    const int *anc, double *__restrict__ l, long long n, uint32_t seed,
+    long long t, Real ox, Real oy, CvModelDev m, const PDev *pd, Part3 *part,
+    double *mon)
 {
-    cv_tile<Real, kInit, kB>(state, state + n, state + 2 * n, state + 3 * n, l,
-        n, blockIdx.x, seed, t, ox, oy, m, pd, part + blockIdx.x,
-        mon ? mon + 8 * static_cast<long long>(blockIdx.x) : nullptr);
+    cv_tile<Real, kInit, kB>(src, src + n, src + 2 * n, src + 3 * n, anc,
+        state, state + n, state + 2 * n, state + 3 * n, l, n, blockIdx.x, seed,
+        t, ox, oy, m, pd,
+        part + blockIdx.x, mon ? mon + 8 * static_cast<long long>(blockIdx.x) : nullptr);
 }
 
 // ------------------------------------------------- learned proposal
@@ -909,10 +1016,12 @@ __global__ void __launch_bounds__(kPB) k_batch_move(float *state, double *l,
                 cv_learned_span(X, X + ld, X + 2 * ld, X + 3 * ld, lf, F.n, lt,
                     0, kLChunks, F.seed, t, ox, oy, m, nf, pd + f, part + tile);
         } else if (t == 0)
-            cv_tile<float, true, 1>(X, X + ld, X + 2 * ld, X + 3 * ld, lf, F.n,
+            cv_tile<float, true, 1>(X, X + ld, X + 2 * ld, X + 3 * ld, nullptr,
+                X, X + ld, X + 2 * ld, X + 3 * ld, lf, F.n,
                 lt, F.seed, t, ox, oy, m, pd + f, part + tile, nullptr);
         else
-            cv_tile<float, false, 1>(X, X + ld, X + 2 * ld, X + 3 * ld, lf, F.n,
+            cv_tile<float, false, 1>(X, X + ld, X + 2 * ld, X + 3 * ld, nullptr,
+                X, X + ld, X + 2 * ld, X + 3 * ld, lf, F.n,
                 lt, F.seed, t, ox, oy, m, pd + f, part + tile, nullptr);
     }
 }
@@ -1098,7 +1207,8 @@ void particle_destroy(ParticleSet *p)
     if (!p)
         return;
     void *ps[] = {p->state, p->logw, p->scratch, p->rep, p->part, p->mpart,
-        p->mout, p->pd, p->rng_words, p->cess_buf, p->lsub};
+        p->mout, p->pd, p->rng_words, p->cess_buf, p->lsub, p->state_alt,
+        p->anc};
     for (void *q : ps)
         if (q)
             (void)hipFree(q);
@@ -1137,6 +1247,9 @@ int weight_read(ParticleSet *p, double *W)
     return kOk;
 }
 
+static int resample_event(ParticleSet *p, int scheme, bool prep, int *rep_out,
+    int *idx_out, bool defer);
+
 // One resampling event. The decision is made on the device; with
 // host_decides the host reads it right away (one synchronisation) and a
 // step that does not resample launches nothing more. Otherwise the whole
@@ -1167,6 +1280,17 @@ int resample_step(ParticleSet *p, int scheme, double threshold, long long t,
         if (!*p->host_active)
             return kOk;
     }
+    return resample_event(p, scheme, prep, rep_out, idx_out, false);
+}
+
+// The event after its decision (made on the device by k_prepare, k_decide
+// or k_step_finalize; its kernels return early when it is inactive):
+// counts, ancestors, and the in-place state gather, or with `defer` the
+// ancestors of every particle in p->anc for the next move, state untouched.
+// prep: the decision kernel prepared the draw parameters and tile prefix.
+static int resample_event(ParticleSet *p, int scheme, bool prep, int *rep_out,
+    int *idx_out, bool defer)
+{
     int *rep = rep_out ? rep_out : p->rep;
     int rc = resample_scheme(p->ws, scheme, p->n, p->n, 0, 0, p->logw,
         kWeightLog, rep, true, prep);
@@ -1181,8 +1305,10 @@ int resample_step(ParticleSet *p, int scheme, double threshold, long long t,
     end.n = p->n;
     end.scheme = scheme;
     end.words = prep ? 1 : 0;
-    return trans_rep_index(p->ws, p->n, p->n, rep, idx_out, p->state, p->dim,
-        p->n, p->bytes, true, false, p->n >= 2, end);
+    end.deferred = defer ? 1 : 0;
+    return trans_rep_index(p->ws, p->n, p->n, rep, defer ? p->anc : idx_out,
+        defer ? nullptr : p->state, p->dim, p->n, p->bytes, true, false,
+        p->n >= 2, end);
 }
 
 // r_d = sum_i W_i buf[i * dim + d0 + d] for 8 components at a time.
@@ -1269,18 +1395,19 @@ static int move_batch(int bytes)
 
 template <typename Real, bool kInit>
 static void launch_cv(int batch, long long nb, hipStream_t st, Real *s,
-    double *l, long long n, uint32_t seed, long long t, Real ox, Real oy,
-    const CvModelDev &m, const PDev *pd, Part3 *part, double *mon)
+    const Real *src, const int *anc, double *l, long long n, uint32_t seed,
+    long long t, Real ox, Real oy, const CvModelDev &m, const PDev *pd,
+    Part3 *part, double *mon)
 {
     if (batch == 4)
         hipLaunchKernelGGL((k_cv_step<Real, kInit, 4>), dim3(nb), dim3(kPB),
-            0, st, s, l, n, seed, t, ox, oy, m, pd, part, mon);
+            0, st, s, src, anc, l, n, seed, t, ox, oy, m, pd, part, mon);
     else if (batch == 2)
         hipLaunchKernelGGL((k_cv_step<Real, kInit, 2>), dim3(nb), dim3(kPB),
-            0, st, s, l, n, seed, t, ox, oy, m, pd, part, mon);
+            0, st, s, src, anc, l, n, seed, t, ox, oy, m, pd, part, mon);
     else
         hipLaunchKernelGGL((k_cv_step<Real, kInit, 1>), dim3(nb), dim3(kPB),
-            0, st, s, l, n, seed, t, ox, oy, m, pd, part, mon);
+            0, st, s, src, anc, l, n, seed, t, ox, oy, m, pd, part, mon);
 }
 
 int cv_step(ParticleSet *p, const CvModelDev &m, long long t, double ox,
@@ -1299,19 +1426,19 @@ int cv_step(ParticleSet *p, const CvModelDev &m, long long t, double ox,
         float *s = static_cast<float *>(p->state);
         const float fx = static_cast<float>(ox), fy = static_cast<float>(oy);
         if (t == 0)
-            launch_cv<float, true>(batch, nb, st, s, p->logw, p->n, p->seed,
-                t, fx, fy, m, p->pd, part, nullptr);
+            launch_cv<float, true>(batch, nb, st, s, s, nullptr, p->logw, p->n,
+                p->seed, t, fx, fy, m, p->pd, part, nullptr);
         else
-            launch_cv<float, false>(batch, nb, st, s, p->logw, p->n, p->seed,
-                t, fx, fy, m, p->pd, part, mon);
+            launch_cv<float, false>(batch, nb, st, s, s, nullptr, p->logw, p->n,
+                p->seed, t, fx, fy, m, p->pd, part, mon);
     } else {
         double *s = static_cast<double *>(p->state);
         if (t == 0)
-            launch_cv<double, true>(batch, nb, st, s, p->logw, p->n, p->seed,
-                t, ox, oy, m, p->pd, part, nullptr);
+            launch_cv<double, true>(batch, nb, st, s, s, nullptr, p->logw, p->n,
+                p->seed, t, ox, oy, m, p->pd, part, nullptr);
         else
-            launch_cv<double, false>(batch, nb, st, s, p->logw, p->n, p->seed,
-                t, ox, oy, m, p->pd, part, mon);
+            launch_cv<double, false>(batch, nb, st, s, s, nullptr, p->logw, p->n,
+                p->seed, t, ox, oy, m, p->pd, part, mon);
     }
     hipLaunchKernelGGL(k_weight_finalize, dim3(1), dim3(kMergeThreads), 0, st, static_cast<const Part3 *>(p->part),
         nb, p->n, t == 0 ? 0 : 1, p->pd);
@@ -1396,11 +1523,85 @@ int cv_transitions(ParticleSet *p, const CvModelDev &m, long long T,
     return kOk;
 }
 
+template <typename Real>
+static int cv_iterate_impl(ParticleSet *p, const CvModelDev &m, long long t0,
+    long long k, const double *ox, const double *oy, int scheme,
+    double threshold, double *ess_d, int *rs_d, double *pos_d, double *logz_d)
+{
+    hipStream_t st = p->ws.stream;
+    const long long nb = pblocks(p->n);
+    if (!p->state_alt)
+        VSMCH_HIP(hipMalloc(&p->state_alt,
+            static_cast<size_t>(p->n) * p->dim * p->bytes));
+    if (!p->anc)
+        VSMCH_HIP(hipMalloc(&p->anc, sizeof(int) * p->n));
+    Real *const home = static_cast<Real *>(p->state);
+    Real *cur = home;
+    Real *nxt = static_cast<Real *>(p->state_alt);
+    const bool prep = p->n >= 2 && (scheme == kSystematic || scheme == kStratified);
+    const int batch = move_batch(p->bytes);
+    Part3 *part = static_cast<Part3 *>(p->part);
+    for (long long s = 0; s < k; ++s) {
+        const long long t = t0 + s;
+        const Real fx = static_cast<Real>(ox[s]), fy = static_cast<Real>(oy[s]);
+        // the move of step s also gives the monitor record of step s - 1
+        double *mon = (pos_d && s > 0) ? p->mpart : nullptr;
+        if (t == 0)
+            launch_cv<Real, true>(batch, nb, st, nxt, cur, nullptr, p->logw,
+                p->n, p->seed, t, fx, fy, m, p->pd, part, nullptr);
+        else
+            launch_cv<Real, false>(batch, nb, st, nxt, cur,
+                s > 0 ? p->anc : nullptr, p->logw, p->n, p->seed, t, fx, fy, m,
+                p->pd, part, mon);
+        std::swap(cur, nxt);
+        hipLaunchKernelGGL(k_step_finalize, dim3(1), dim3(kMergeThreads), 0, st,
+            static_cast<const Part3 *>(part), nb, p->n, t == 0 ? 0 : 1, p->pd,
+            p->ws.prm, threshold, s, ess_d, rs_d, logz_d,
+            static_cast<const double *>(p->mpart),
+            mon ? pos_d + 2 * (s - 1) : nullptr, scheme, prep ? 1 : 0,
+            p->ws.tpre);
+        VSMCH_HIP(hipGetLastError());
+        const int rc = resample_event(p, scheme, prep, nullptr, nullptr, true);
+        if (rc != kOk)
+            return rc;
+    }
+    hipLaunchKernelGGL(k_settle<Real>, dim3((p->n + kPB - 1) / kPB), dim3(kPB),
+        0, st, static_cast<const Real *>(cur), home,
+        static_cast<const int *>(p->anc), static_cast<const PDev *>(p->pd),
+        p->n, p->dim);
+    VSMCH_HIP(hipGetLastError());
+    if (pos_d && k > 0)
+        return moments(p, 0, 2, pos_d + 2 * (k - 1));
+    return kOk;
+}
+
+int cv_iterate(ParticleSet *p, const CvModelDev &m, long long t0, long long k,
+    const double *ox, const double *oy, int scheme, double threshold,
+    double *ess_d, int *rs_d, double *pos_d, double *logz_d)
+{
+    if (p->dim != 4) {
+        set_error("cv model needs dim == 4");
+        return kErrInvalid;
+    }
+    if (t0 < 0 || k < 0) {
+        set_error("cv_iterate: t0 and k must be non-negative");
+        return kErrInvalid;
+    }
+    if (p->bytes == 4)
+        return cv_iterate_impl<float>(p, m, t0, k, ox, oy, scheme, threshold,
+            ess_d, rs_d, pos_d, logz_d);
+    return cv_iterate_impl<double>(p, m, t0, k, ox, oy, scheme, threshold,
+        ess_d, rs_d, pos_d, logz_d);
+}
+
 int cv_run(ParticleSet *p, const CvModelDev &m, long long T,
     const double *ox, const double *oy, int scheme, double threshold,
     double *ess_d, int *rs_d, double *pos_d, double *logz_d,
     const MlpSet *net)
 {
+    if (!net)
+        return cv_iterate(p, m, 0, T, ox, oy, scheme, threshold, ess_d, rs_d,
+            pos_d, logz_d);
     hipStream_t st = p->ws.stream;
     const long long nb = pblocks(p->n);
     int rc;

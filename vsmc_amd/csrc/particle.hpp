@@ -19,6 +19,9 @@ struct PDev {
     long long nresampled;      // resampling events so far
     int equal;                 // weights are 1/N (l ignored)
     int resampled;             // last decision
+    // the last event left its ancestors in ParticleSet::anc instead of
+    // gathering the state: the next move reads particle i from anc[i]
+    int anc_valid;
 };
 
 // After a resampling event: equal weights, advance the resampling stream.
@@ -26,10 +29,11 @@ struct PDev {
 // constructor word; stratified: up to and including the first uniform above
 // s_{M-2}, transform.hpp:61-69).
 __device__ inline void after_event(PDev *pd, ResampleDev *prm, long long n,
-    const int *rep, int scheme, int words)
+    const int *rep, int scheme, int words, int deferred = 0)
 {
     if (!prm->active)
         return;
+    pd->anc_valid = deferred;
     if (words) {
         unsigned long long used = 1;
         if (scheme == kStratified) {
@@ -76,6 +80,11 @@ struct ParticleSet {
     int *host_active = nullptr;  // pinned: the decision read by the host
     void *cess_buf = nullptr;    // conditional-ESS search (temper.hip)
     void *lsub = nullptr;        // learned move: weight partial per chunk
+    // cv_iterate: the second state buffer (moves alternate between the two)
+    // and the ancestors an event leaves for the next move; both allocated on
+    // first use
+    void *state_alt = nullptr;
+    int *anc = nullptr;
     Workspace ws;
 };
 
@@ -93,6 +102,15 @@ int resample_step(ParticleSet *p, int scheme, double threshold, long long t,
 int moments(ParticleSet *p, int c0, int dim, double *out_dev);
 int cv_step(ParticleSet *p, const CvModelDev &m, long long t, double ox,
     double oy, bool with_monitor);
+// Steps t0 .. t0 + k - 1 of the bootstrap filter (Sampler::initialize when
+// t0 == 0, then iterate): moves alternate between the particle set's two
+// state buffers and read each particle through the ancestors the previous
+// event left, so the events gather nothing; at the end the state is
+// settled into the set's own buffer. Observations and histories are
+// indexed 0 .. k - 1 (device histories, each may be null).
+int cv_iterate(ParticleSet *p, const CvModelDev &m, long long t0, long long k,
+    const double *ox, const double *oy, int scheme, double threshold,
+    double *ess_d, int *rs_d, double *pos_d, double *logz_d);
 struct MlpSet;
 // net: the learned proposal for the moves t >= 1 (null: bootstrap)
 int cv_run(ParticleSet *p, const CvModelDev &m, long long T,

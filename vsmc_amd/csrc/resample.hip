@@ -828,13 +828,10 @@ __device__ __forceinline__ void resolve_walk(ResolveShared &sh,
 template <class WSrc>
 __global__ void __launch_bounds__(kBlock) k_resolve(WSrc src, long long nh,
     const long long *ndev, const ResampleDev *prm, const TileSeg *segs,
-    const GroupSeg *groups, double *xstart, double *xgstart, int *gmode,
-    int *hard)
+    const GroupSeg *groups, double *xstart, double *xgstart, int *gmode)
 {
     if (!prm->active)
         return;
-    if (threadIdx.x == 0)
-        hard[0] = 0;   // the list k_count fills after this pass
     const long long n = eff_n(ndev, nh);
     const long long ntiles = ceil_div(n, kTile);
     const long long ngroups = ceil_div(ntiles, kGroup);
@@ -916,37 +913,20 @@ __device__ __forceinline__ double count_tile(WSrc src, USrc us, long long M,
     long long tidx, double x0, int *rep, long long *tagg, int *hard,
     int hard_id = -1);
 
-// Replication counts (and residual integer parts) for one tile whose sums
-// stay in one binade; any other tile goes to the list `hard`.
+// Every tile with the full exact fold (the one-binade fast path first), a
+// tile whose sums cross a binade in the same block.
 template <class WSrc, class USrc>
-__global__ void __launch_bounds__(kBlock) k_count(WSrc src, USrc us,
+__global__ void __launch_bounds__(kBlock) k_count_all(WSrc src, USrc us,
     long long nh, const long long *ndev, const ResampleDev *prm, FoldBufs fb,
-    int *rep, long long *tagg, int *hard)
+    int *rep, long long *tagg)
 {
     if (!prm->active)
         return;
     const long long M = eff_n(ndev, nh);
     if (static_cast<long long>(blockIdx.x) * kTile >= M)
         return;
-    count_tile<false>(src, us, M, blockIdx.x, fb.xstart[blockIdx.x], rep, tagg,
-        hard);
-}
-
-// The tiles of a list (hard[1 .. hard[0]]), with the full exact fold.
-template <class WSrc, class USrc>
-__global__ void __launch_bounds__(kBlock) k_count_hard(WSrc src, USrc us,
-    long long nh, const long long *ndev, const ResampleDev *prm, FoldBufs fb,
-    int *rep, long long *tagg, const int *hard)
-{
-    if (!prm->active)
-        return;
-    const long long M = eff_n(ndev, nh);
-    const int nhard = hard[0];
-    for (int q = blockIdx.x; q < nhard; q += gridDim.x) {
-        const long long t = hard[1 + q];
-        count_tile<true>(src, us, M, t, fb.xstart[t], rep, tagg,
-            static_cast<int *>(nullptr));
-    }
+    count_tile<true>(src, us, M, blockIdx.x, fb.xstart[blockIdx.x], rep, tagg,
+        static_cast<int *>(nullptr));
 }
 
 // Replication counts (and residual integer parts) of tile `tidx` of M
@@ -1287,7 +1267,7 @@ __device__ __forceinline__ void scatter_tile(const int *rep, long long M, long l
     L3 tot;
     const L3 ex = block_exclusive_scan(loc, OpAddL3(), L3{0, 0, 0}, lds, &tot);
     // each list is staged in LDS in tile order and written out coalesced
-    if (!kExpand) {
+    if (!kExpand && zdst) {
         int zr = static_cast<int>(ex.a);
 #pragma unroll
         for (int j = 0; j < kItems; ++j)
@@ -1344,6 +1324,28 @@ __device__ inline long long upper_bound_i(const int *a, long long n, long long x
     return lo;
 }
 
+// The same search by one whole wave, 64 probes a round: about log64(n)
+// dependent loads instead of log2(n) (every lane returns the result; the
+// loop bounds are wave-uniform).
+__device__ inline long long wave_upper_bound_i(const int *a, long long n, long long x)
+{
+    const long long lane = threadIdx.x & (kWave - 1);
+    long long lo = 0, hi = n;   // the first index with a[i] > x lies in [lo, hi]
+    while (lo < hi) {
+        const long long step = (hi - lo + kWave - 1) / kWave;
+        const long long p = lo + lane * step;
+        const bool le = p < hi && a[p] <= x;
+        // a is non-decreasing, so the probes with a[p] <= x are a prefix
+        const long long c = __popcll(__ballot(le));
+        if (c == 0)
+            break;                                  // a[lo] > x
+        const long long nhi = lo + c * step;
+        lo = lo + (c - 1) * step + 1;               // past the last such probe
+        hi = nhi < hi ? nhi : hi;                   // at the next probe, if any
+    }
+    return lo;
+}
+
 // Source of each rank k (the k-th childless slot, or the k-th output in
 // expand mode): the source whose copies cover k. Optionally gathers an SoA
 // state in place: x[d][dst] = x[d][src] (sources are never destinations).
@@ -1363,7 +1365,8 @@ __global__ void __launch_bounds__(kBlock) k_expand(const ResampleDev *prm,
     if (prm && !prm->active)
         return;
     if (end.pd && blockIdx.x == 0 && threadIdx.x == 0)
-        after_event(end.pd, eprm, end.n, end.rep, end.scheme, end.words);
+        after_event(end.pd, eprm, end.n, end.rep, end.scheme, end.words,
+            end.deferred);
     const long long K = kExpand ? toff[3 * ntiles + 1] : toff[3 * ntiles + 0];
     const long long C = toff[3 * ntiles + 2];
     if (static_cast<long long>(blockIdx.x) * kTile >= K || C <= 0)
@@ -1384,11 +1387,16 @@ __device__ __forceinline__ void expand_chunk(long long chunk, long long K, long 
     __shared__ int mark[kTileLds];
     __shared__ int mred[kWaves];
     __shared__ long long clo_s, chi_s;
-    if (threadIdx.x == 0) {
-        const long long lo = upper_bound_i(cE, C, k0) - 1;
-        const long long hi = upper_bound_i(cE, C, k0 + nk - 1) - 1;
-        clo_s = lo < 0 ? 0 : lo;
-        chi_s = hi < 0 ? 0 : hi;
+    // the first and last source of the chunk, one wave each
+    const int wave = static_cast<int>(threadIdx.x) / kWave;
+    if (wave < 2) {
+        const long long b = wave_upper_bound_i(cE, C, wave == 0 ? k0 : k0 + nk - 1) - 1;
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+            if (wave == 0)
+                clo_s = b < 0 ? 0 : b;
+            else
+                chi_s = b < 0 ? 0 : b;
+        }
     }
 #pragma unroll
     for (int j = 0; j < kItems; ++j)
@@ -2006,7 +2014,7 @@ static int run_fold(Workspace &ws, WSrc src, long long nmax,
         ws.prm, ws.segs, ws.groups);
     hipLaunchKernelGGL(k_resolve<WSrc>, dim3(1), dim3(kBlock), 0, st, src,
         nmax, ndev, ws.prm, ws.segs, ws.groups, ws.xstart, ws.xgstart,
-        ws.gmode, ws.hard);
+        ws.gmode);
     hipLaunchKernelGGL(k_starts, dim3(gg), dim3(kWave), 0, st, nmax, ndev,
         ws.prm, ws.segs, ws.xgstart, ws.gmode, ws.xstart);
     VSMCH_HIP(hipGetLastError());
@@ -2025,14 +2033,12 @@ static int run_count(Workspace &ws, WSrc src, USrc us, long long M, int *rep,
     int rc = run_fold(ws, src, M, nullptr, tpre_ready);
     if (rc != kOk)
         return rc;
+    // one block per tile; a tile whose sums cross a binade is folded in
+    // full by its own block while the others take the one-binade fast path
     const unsigned int gt = grid_tiles(M);
-    hipLaunchKernelGGL((k_count<WSrc, USrc>), dim3(gt), dim3(kBlock), 0,
+    hipLaunchKernelGGL((k_count_all<WSrc, USrc>), dim3(gt), dim3(kBlock), 0,
         ws.stream, src, us, M, static_cast<const long long *>(nullptr), ws.prm,
-        fold_bufs(ws), rep, ws.tagg, ws.hard);
-    hipLaunchKernelGGL((k_count_hard<WSrc, USrc>), dim3(gt < 256 ? gt : 256),
-        dim3(kBlock), 0, ws.stream, src, us, M,
-        static_cast<const long long *>(nullptr), ws.prm, fold_bufs(ws), rep,
-        ws.tagg, static_cast<const int *>(ws.hard));
+        fold_bufs(ws), rep, ws.tagg);
     VSMCH_HIP(hipGetLastError());
     return kOk;
 }

@@ -180,6 +180,40 @@ class CvFilter(Particle):
                                  scheme, threshold, P(ess), P(rs), P(pos), P(logz)))
         return {"ess": ess, "resampled": rs, "pos": pos.reshape(T, 2), "logz": logz}
 
+    @staticmethod
+    def history_buffers(k):
+        import torch
+        return {"ess": torch.empty(k, dtype=torch.float64, device="cuda"),
+                "resampled": torch.empty(k, dtype=torch.int32, device="cuda"),
+                "pos": torch.empty((k, 2), dtype=torch.float64, device="cuda"),
+                "logz": torch.empty(k, dtype=torch.float64, device="cuda")}
+
+    def iterate(self, t0, obs_x, obs_y, scheme, threshold, histories=True):
+        """Steps t0 .. t0 + len(obs_x) - 1 (Sampler::initialize when t0 == 0,
+        then iterate), queued without synchronising: each move gathers the
+        previous event's ancestors itself. histories: True (new device
+        buffers), False (none) or a dict from history_buffers(). Returns the
+        device histories (ess, resampled, pos, logz) or None."""
+        import numpy as np
+        import torch
+        k = len(obs_x)
+        ox = np.ascontiguousarray(obs_x, dtype=np.float64)
+        oy = np.ascontiguousarray(obs_y, dtype=np.float64)
+        h = None
+        ptr = [None] * 4
+        if isinstance(histories, dict):
+            h = histories   # preallocated device tensors of at least k records
+        elif histories:
+            h = self.history_buffers(k)
+            torch.cuda.synchronize()
+        if h is not None:
+            ptr = [ctypes.c_void_p(h[n].data_ptr()) for n in ("ess", "resampled", "pos", "logz")]
+        P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        check(lib().vsmch_cv_iterate(self._h, ctypes.byref(self.model), t0, k, P(ox), P(oy),
+                                     scheme, threshold, *ptr))
+        self._keep_obs = (ox, oy)
+        return h
+
     def move_learned(self, net, t, ox, oy):
         check(lib().vsmch_cv_move_learned(self._h, ctypes.byref(self.model), net._h, t,
                                           ox, oy))
