@@ -20,9 +20,13 @@ def c0_obs():
             np.array([float.fromhex(x) for x in g["obs_y"]]))
 
 
-NS = [2, 3, 1000, 4096, 5003, 12288, 40001]
-TS = [6, 1, 30, 17, 40, 25, 33]
-SEEDS = [7, 8, 9, 10, 11, 12, 13]
+# 2^17 and 2^18 - 5 particles: 32 and 64 tiles, so the walk over a filter's
+# tile summaries (16 held at a time) carries its cursor and exact start
+# across several windows, with the binade crossings of the running sum
+# (at 1/4, 1/2 of the total) in later windows
+NS = [2, 3, 1000, 4096, 5003, 12288, 40001, 1 << 17, (1 << 18) - 5]
+TS = [6, 1, 30, 17, 40, 25, 33, 14, 11]
+SEEDS = [7, 8, 9, 10, 11, 12, 13, 14, 15]
 
 
 def _tracks():

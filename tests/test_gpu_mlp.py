@@ -120,10 +120,26 @@ def test_train_lg(dev):
     net.close()
 
 
-def test_learned_move_vs_oracle(dev):
+# Student-t observations (df 10, scale 0.1) with 5% outliers at 10x scale,
+# normalising constants kept (obs_model 1)
+ST = dict(dt=0.1, sd_pos=0.02, sd_vel=0.02, sd_pos0=0.02, sd_vel0=0.02, obs_model=1,
+          obs_scale=0.1, obs_df=10.0)
+
+
+def _st_track(T, seed):
+    from vsmc_amd.models import cv_simulate
+    _, ox, oy = cv_simulate(T, seed, dt=0.1, sd_pos=0.02, sd_vel=0.02, sd_pos0=0.02,
+                            sd_vel0=0.02, obs="student", obs_scale=0.1, obs_df=10.0,
+                            outlier_frac=0.05, outlier_mult=10.0)
+    return ox, oy
+
+
+@pytest.mark.parametrize("model", ["lg", "student"])
+def test_learned_move_vs_oracle(dev, model):
     """One learned move: new states and log-weights against the oracle's
     per-particle formula with the device network's outputs and the
-    particles' own normals (Philox block t + 1)."""
+    particles' own normals (Philox block t + 1), for the linear-Gaussian and
+    the Student-t (outlier) observation models."""
     import torch
     import vsmc_amd as va
     rs = np.random.default_rng(3)
@@ -131,8 +147,9 @@ def test_learned_move_vs_oracle(dev):
     net = va.ProposalNet()
     net.set_params(p, shift, sc)
     n = 5000
-    ox, oy = _lg_track(4, 2)
-    f = va.CvFilter(n, "float32", seed=21, **LG)
+    cfg = LG if model == "lg" else ST
+    ox, oy = _lg_track(4, 2) if model == "lg" else _st_track(4, 2)
+    f = va.CvFilter(n, "float32", seed=21, **cfg)
     k0 = f.keys()[0]
     keys = (k0 - 1 + np.arange(n, dtype=np.int64)) % 0xFFFFFFFF + 1
     f.init(float(ox[0]), float(oy[0]))
@@ -162,7 +179,8 @@ def test_learned_move_vs_oracle(dev):
     inc = np.zeros(n)
     for i in range(n):
         es1[:, i], inc[i] = ol.cv_learned_particle(s0[:, i], float(o32), float(p32), out[i],
-                                                   z[:, i], 0.1, 0.02, 0.02, 2, 0.01, 10.0)
+                                                   z[:, i], 0.1, 0.02, 0.02, cfg["obs_model"],
+                                                   cfg["obs_scale"], cfg["obs_df"])
     np.testing.assert_allclose(s1, es1, rtol=1e-5, atol=2e-6)
     lw = np.log(W0) + inc
     lw -= lw.max()

@@ -207,6 +207,53 @@ def test_full_size_counts(dev, scheme, s):
     assert len(bad) == 0, (len(bad), bad[:5])
 
 
+@pytest.mark.parametrize("s", [0.5, 2.0, 8.0])
+def test_full_size_multinomial_residual(dev, s):
+    """N = 2^24 log-weights ~ N(0, s^2), the schemes whose uniforms need libm
+    log / exp (multinomial: U01SequenceSorted, u01_sequence.hpp:130-164;
+    residual: the residuals resampled with it, residual.hpp:43-58). Counts
+    bit-exact with the oracle's merge given the device's own uniforms; against
+    the oracle's libm uniforms a handful of counts may move. The residual
+    stratified / systematic schemes are bit-exact with the oracle."""
+    import vsmc_amd as va
+    n = 1 << 24
+    rs = np.random.default_rng(int(s * 100))
+    lw = rs.normal(0, s, n)
+    W = np.exp(lw - lw.max())
+    W /= W.sum()
+    Wd = _t(W)
+    key, off = 123, 7
+    print(f"multinomial s={s} n={n}", flush=True)
+    rep, off2 = va.resample(ol.MULTINOMIAL, Wd, n, key, off)
+    rep = _np(rep)
+    u = _np(va.sorted_uniforms(n, key, off))
+    assert (np.diff(u) >= 0).all()
+    assert (rep == ol.trans_u01_rep(W, u)).all()
+    erep, _ = ol.resample(ol.MULTINOMIAL, W, n, key, skip=off)
+    mism = int((rep != erep).sum())
+    print("multinomial counts moved by libm:", mism, flush=True)
+    assert mism <= 64
+    assert rep.sum() == n
+    print(f"residual s={s}", flush=True)
+    rep, _ = va.resample(ol.RESIDUAL, Wd, n, key, off)
+    rep = _np(rep)
+    resid, integ, R = va.resample_trans_residual(Wd, n)
+    eresid, einteg, eR = ol.trans_residual(W, n)
+    assert R == eR and (_np(integ) == einteg).all()
+    assert (_np(resid) == eresid).all()
+    ur = _np(va.sorted_uniforms(R, key, off))
+    assert (rep == einteg + ol.trans_u01_rep(eresid, ur)).all()
+    assert rep.sum() == n
+    for scheme in (ol.RESIDUAL_STRATIFIED, ol.RESIDUAL_SYSTEMATIC):
+        print(f"scheme={scheme} s={s}", flush=True)
+        rep, off3 = va.resample(scheme, Wd, n, key, off)
+        erep, used = ol.resample(scheme, W, n, key, skip=off)
+        rep = _np(rep)
+        bad = np.nonzero(rep != erep)[0]
+        assert len(bad) == 0, (scheme, len(bad), bad[:5])
+        assert off3 - off == used
+
+
 def test_growing_weights_counts(dev):
     """Weights growing by a factor 2 every 2^18 particles (N = 2^24): the
     running sum crosses a binade in every group of 64 tiles, more crossing
