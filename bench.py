@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--sections", default="all",
-                    help="comma-separated subset of step,extra,micro,learned,batched,gmm "
+                    help="comma-separated subset of step,extra,dropin,micro,learned,batched,gmm "
                          "(profiling runs); 'all' runs every one")
     return ap.parse_args()
 
@@ -408,6 +408,40 @@ def measured_traffic(name="cv_step_traffic.json", key="bytes_per_step"):
         return json.load(fh).get(key)
 
 
+def dropin_path(n=1 << 24, T=24):
+    """The drop-in C++ layer at N particles: the reference's PF example
+    written with device functors (tests/cpp/pf_hip.cpp: InitializeHIP /
+    MoveHIP over a __device__ eval_sp, Weight::add_log from the functor's
+    device array, Particle::resample through the library's device event, the
+    "pos" monitor), FP64 state, stratified when ESS < N/2; T - 4 timed steps
+    after initialize + 3."""
+    import subprocess
+    import tempfile
+    from vsmc_amd.models import cv_simulate
+    exe = os.path.join(ROOT, "tests", "cpp", "pf_hip")
+    if not os.path.exists(exe):
+        return {"error": "tests/cpp/pf_hip not built"}
+    _, ox, oy = cv_simulate(T, 20161)
+    with tempfile.NamedTemporaryFile("w", suffix=".data", delete=False) as fh:
+        for a, b in zip(ox, oy):
+            fh.write(f"{float(a)!r} {float(b)!r}\n")
+        path = fh.name
+    try:
+        r = subprocess.run([exe, str(n), str(T), "1", "0.5", "1", path, "time"],
+                           capture_output=True, text=True, timeout=600)
+    finally:
+        os.unlink(path)
+    if r.returncode != 0:
+        return {"error": r.stderr[-500:]}
+    _, steps, ms, nres = r.stdout.strip().split("\n")[-1].split("\t")
+    steps, ms, nres = int(steps), float(ms), int(nres)
+    return {"n": n, "steps": steps, "ms_per_step": ms / steps,
+            "updates_per_s": n * steps / (ms / 1e3), "resampled_steps": nres,
+            "state": "fp64 x4 (StateHIP)", "scheme": "stratified", "threshold": 0.5,
+            "path": "C++ Sampler: InitializeHIP / MoveHIP device functors, "
+                    "Weight::add_log, Particle::resample on the device, monitor"}
+
+
 def cpu_baseline(budget_s):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
@@ -443,14 +477,16 @@ def main():
     scheme = va.SCHEMES[a.scheme]
     T = a.warmup + a.steps + 64
     _, ox, oy = cv_simulate(T, 20161)
-    sections = ({"step", "extra", "micro", "learned", "batched", "gmm"} if a.sections == "all"
+    sections = ({"step", "extra", "dropin", "micro", "learned", "batched", "gmm"}
+                if a.sections == "all"
                 else set(a.sections.split(",")))
     if a.no_extra:
         sections &= {"step"}
     if "step" not in sections:
         # one workload alone (e.g. under a profiler): its figures only
         out = {}
-        for name, fn in (("extra", None), ("micro", microbench), ("learned", learned_vs_plain),
+        for name, fn in (("extra", None), ("dropin", lambda va_, t_: dropin_path()),
+                         ("micro", microbench), ("learned", learned_vs_plain),
                          ("batched", batched_filters), ("gmm", tempered_gmm)):
             if name in sections and fn is not None:
                 out[name] = fn(va, torch)
@@ -526,6 +562,8 @@ def main():
                            "step_hbm_frac": a.n * (MOVE_BYTES * 20 + RESAMPLE_BYTES * nres2)
                            / (ms2 / 1e3) / 1e9 / HBM_PEAK_GBS}
         out["extra"] = extra
+    if "dropin" in sections:
+        out["dropin_functor_path"] = dropin_path()
     if "micro" in sections:
         out["microbench"] = microbench(va, torch)
     if "learned" in sections:

@@ -204,6 +204,16 @@ vsmch_status vsmch_weight_set_equal(vsmch_particle *p);
 vsmch_status vsmch_weight_set_log(vsmch_particle *p, const double *logw);
 /* Weight::add_log (weight.hpp:168-173) from a device array. */
 vsmch_status vsmch_weight_add_log(vsmch_particle *p, const double *inc);
+/* Weight::set, mul, set_log and add_log with a stride (weight.hpp:105-171)
+ * from a device array: src[i * stride] for particle i (stride >= 1). The
+ * weights are kept as logarithms, so set / mul take non-negative linear
+ * weights (log 0 = -inf gives weight 0). */
+#define VSMCH_WEIGHT_SET_LOG 0
+#define VSMCH_WEIGHT_ADD_LOG 1
+#define VSMCH_WEIGHT_SET 3
+#define VSMCH_WEIGHT_MUL 4
+vsmch_status vsmch_weight_update(vsmch_particle *p, int op, const double *src,
+    int64_t stride);
 /* Weight::ess (weight.hpp:76); also the log-evidence accumulated by
  * set_log/add_log (log sum_i W_prev,i exp(inc_i),
  * user_guide/tex/smc.tex:169-182). Synchronises. */
@@ -211,6 +221,14 @@ vsmch_status vsmch_weight_ess(vsmch_particle *p, double *ess_host);
 vsmch_status vsmch_weight_log_zconst(vsmch_particle *p, double *logz_host);
 /* Normalised linear weights W (Weight::data, weight.hpp:81). */
 vsmch_status vsmch_weight_read(vsmch_particle *p, double *W);
+
+/* StateMatrix::copy(N, index) (include/vsmc/core/state_matrix.hpp:
+ * 246-253) from a device array of n == N ancestor indices: particle i takes
+ * the state particle index[i] had before the call (equal to the reference's
+ * in-order loop for the indices resample_trans_rep_index produces, where
+ * every source keeps its own slot). An index outside [0, N) is rejected
+ * (VSMCH_ERR_INVALID, state unchanged). Synchronises the library stream. */
+vsmch_status vsmch_particle_copy(vsmch_particle *p, int64_t n, const int *index);
 
 /* Particle::resample(op, threshold) (include/vsmc/core/particle.hpp:
  * 170-197): if ESS < threshold * N, draws counts with `scheme` from the

@@ -19,7 +19,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdint>
 #include <cstring>
 #include <functional>
@@ -27,6 +29,7 @@
 #include <iterator>
 #include <limits>
 #include <map>
+#include <memory>
 #include <ostream>
 #include <stdexcept>
 #include <string>
@@ -132,6 +135,41 @@ class StateHIP {
 
     explicit StateHIP(size_type N) : n_(N) {}
     static constexpr std::size_t dim() { return Dim; }
+
+    // StateMatrix::copy (state_matrix.hpp:246-253): particle dst takes the
+    // state particle index[dst] had before the call; index is a host array
+    // (any integer type, as Particle::resample passes it).
+    template <typename IntType>
+    void copy(size_type N, const IntType *index)
+    {
+        if (index == nullptr) {
+            resample_event(N);
+            return;
+        }
+        if (N != n_)
+            throw std::runtime_error("StateHIP::copy: N differs from the size");
+        std::vector<int> h(N);
+        for (size_type i = 0; i != N; ++i)
+            h[i] = static_cast<int>(index[i]);
+        int *d = nullptr;
+        check_hip(hipMalloc(&d, sizeof(int) * (N ? N : 1)), "StateHIP::copy");
+        check_hip(hipMemcpy(d, h.data(), sizeof(int) * N, hipMemcpyHostToDevice),
+            "StateHIP::copy");
+        const vsmch_status rc = vsmch_particle_copy(h_, static_cast<int64_t>(N), d);
+        (void)hipFree(d);
+        check(rc, "StateHIP::copy");
+    }
+    // copy(N, nullptr): what Particle::resample calls when the weight class
+    // has no host weights (resample_data() == nullptr, particle.hpp:175-192)
+    // once it has decided to resample: the whole event on the device --
+    // counts with the state's scheme from the particle set's resampling
+    // engine, ancestors, in-place gather and equal weights.
+    void copy(size_type N, std::nullptr_t) { resample_event(N); }
+
+    // The scheme copy(N, nullptr) resamples with (Sampler sets it; the
+    // reference Sampler's default is Stratified, sampler.hpp:83-114).
+    void resample_scheme(int scheme) { scheme_ = scheme; }
+    int resample_scheme() const { return scheme_; }
     size_type size() const { return n_; }
     view_type view() const { return view_type{data_, n_}; }
     RealType *data() const { return data_; }
@@ -165,11 +203,28 @@ class StateHIP {
         return os;
     }
 
-    void bind_(RealType *p) { data_ = p; }
+    void bind_(RealType *p, vsmch_particle *h = nullptr)
+    {
+        data_ = p;
+        h_ = h;
+    }
 
   private:
     size_type n_;
     RealType *data_ = nullptr;
+    vsmch_particle *h_ = nullptr;
+    int scheme_ = VSMCH_RESAMPLE_STRATIFIED;
+
+    void resample_event(size_type N)
+    {
+        if (N != n_)
+            throw std::runtime_error("StateHIP::copy: N differs from the size");
+        int r = 0;
+        // the caller has decided: always resample
+        check(vsmch_particle_resample(h_, scheme_,
+                  std::numeric_limits<double>::infinity(), &r, nullptr, nullptr),
+            "StateHIP::copy");
+    }
 };
 
 template <typename CharT, typename Traits, std::size_t Dim, typename RealType>
@@ -194,14 +249,68 @@ struct SingleParticleHIP {
     __device__ ParticleStream &rng() const { return *rng_; }
 };
 
-// Weight (include/vsmc/core/weight.hpp:65-235) kept on the device.
+namespace detail {
+
+// True for memory the device owns (hipMalloc); host arrays, pinned or not,
+// are staged to the device first.
+inline bool is_device_ptr(const void *p)
+{
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+// Device scratch of n doubles, grown on demand, freed with its owner.
+class DeviceDoubles {
+  public:
+    DeviceDoubles() = default;
+    DeviceDoubles(const DeviceDoubles &) = delete;
+    DeviceDoubles &operator=(const DeviceDoubles &) = delete;
+    ~DeviceDoubles()
+    {
+        if (p_)
+            (void)hipFree(p_);
+    }
+    double *get(std::size_t n)
+    {
+        if (n > n_) {
+            if (p_)
+                check_hip(hipFree(p_), "scratch");
+            p_ = nullptr;
+            check_hip(hipMalloc(&p_, sizeof(double) * (n ? n : 1)), "scratch");
+            n_ = n;
+        }
+        return p_;
+    }
+
+  private:
+    double *p_ = nullptr;
+    std::size_t n_ = 0;
+};
+
+} // namespace detail
+
+// Weight (include/vsmc/core/weight.hpp:65-235) kept on the device, with the
+// reference's members: set / mul / set_log / add_log (plain and with a
+// stride), set_equal, ess, data, read_weight, draw. A pointer argument may
+// be device or host memory (host arrays are copied to the device first);
+// any other iterator is read on the host. The weights are kept as their
+// logarithms: set and mul take non-negative linear weights.
 // resample_data() is null: Particle::resample then hands the whole event to
-// the state (particle.hpp:175-192), here the library's device resampler.
+// the state (particle.hpp:175-192), here StateHIP::copy(N, nullptr), which
+// runs the library's device resampler.
 class WeightHIP {
   public:
     using size_type = std::size_t;
     WeightHIP() = default;
-    WeightHIP(vsmch_particle *h, size_type n) : h_(h), n_(n) {}
+    WeightHIP(vsmch_particle *h, size_type n)
+        : h_(h), n_(n), buf_(std::make_shared<detail::DeviceDoubles>()),
+          cum_(std::make_shared<detail::DeviceDoubles>())
+    {
+    }
     size_type size() const { return n_; }
     size_type resample_size() const { return n_; }
     double ess() const
@@ -211,20 +320,86 @@ class WeightHIP {
         return e;
     }
     void set_equal() { check(vsmch_weight_set_equal(h_), "Weight::set_equal"); }
-    void set_log(const double *dev) { check(vsmch_weight_set_log(h_, dev), "Weight::set_log"); }
-    void add_log(const double *dev) { check(vsmch_weight_add_log(h_, dev), "Weight::add_log"); }
-    const double *resample_data() const { return nullptr; }
-    // normalised weights to a host array (Weight::read_weight)
-    void read_weight(double *host) const
+
+    template <typename It>
+    void set(It first) { update(VSMCH_WEIGHT_SET, first, 1, "Weight::set"); }
+    template <typename It>
+    void set(It first, int stride) { update(VSMCH_WEIGHT_SET, first, stride, "Weight::set"); }
+    template <typename It>
+    void mul(It first) { update(VSMCH_WEIGHT_MUL, first, 1, "Weight::mul"); }
+    template <typename It>
+    void mul(It first, int stride) { update(VSMCH_WEIGHT_MUL, first, stride, "Weight::mul"); }
+    template <typename It>
+    void set_log(It first) { update(VSMCH_WEIGHT_SET_LOG, first, 1, "Weight::set_log"); }
+    template <typename It>
+    void set_log(It first, int stride)
     {
-        double *d = nullptr;
-        check_hip(hipMalloc(&d, sizeof(double) * n_), "read_weight");
-        check(vsmch_weight_read(h_, d), "Weight::read_weight");
-        check(vsmch_synchronize(), "Weight::read_weight");
-        check_hip(hipMemcpy(host, d, sizeof(double) * n_, hipMemcpyDeviceToHost),
-            "read_weight");
-        check_hip(hipFree(d), "read_weight");
+        update(VSMCH_WEIGHT_SET_LOG, first, stride, "Weight::set_log");
     }
+    template <typename It>
+    void add_log(It first) { update(VSMCH_WEIGHT_ADD_LOG, first, 1, "Weight::add_log"); }
+    template <typename It>
+    void add_log(It first, int stride)
+    {
+        update(VSMCH_WEIGHT_ADD_LOG, first, stride, "Weight::add_log");
+    }
+
+    const double *resample_data() const { return nullptr; }
+
+    // Weight::data (weight.hpp:81): the normalised weights as a device
+    // array, valid until the weights change
+    const double *data() const
+    {
+        double *d = buf_->get(n_);
+        check(vsmch_weight_read(h_, d), "Weight::data");
+        return d;
+    }
+
+    // normalised weights to a host array or output iterator
+    // (Weight::read_weight, weight.hpp:83-95)
+    template <typename OutputIter>
+    void read_weight(OutputIter first) const
+    {
+        const std::vector<double> w = host_weights();
+        std::copy(w.begin(), w.end(), first);
+    }
+    template <typename RandomIter>
+    void read_weight(RandomIter first, int stride) const
+    {
+        const std::vector<double> w = host_weights();
+        for (size_type i = 0; i != n_; ++i, first += stride)
+            *first = w[i];
+    }
+    void read_resample_weight(double *first) const { read_weight(first); }
+
+    // Weight::draw (weight.hpp:175-179, DiscreteDistribution with
+    // normalised weights, discrete_distribution.hpp:250-285): one U01 from
+    // eng (U01Distribution<double>: u * 2^-32 from a 32-bit engine,
+    // (u >> 11) * 2^-53 from a 64-bit one), then the first index whose
+    // sequential cumulative weight is >= u (the last index if none). The
+    // cumulative sums are the exact sequential ones, computed on the device.
+    template <typename URNG>
+    size_type draw(URNG &eng) const
+    {
+        static_assert(URNG::min() == 0, "draw: engine must start at 0");
+        double u;
+        if (static_cast<uint64_t>(URNG::max()) == 0xFFFFFFFFull)
+            u = static_cast<double>(static_cast<uint32_t>(eng())) * 0x1p-32;
+        else if (static_cast<uint64_t>(URNG::max()) == ~0ull)
+            u = static_cast<double>(static_cast<uint64_t>(eng()) >> 11) * 0x1p-53;
+        else
+            throw std::runtime_error("Weight::draw: engine must give 32 or 64 bits");
+        double *w = const_cast<double *>(data());
+        double *c = cum_->get(n_);
+        check(vsmch_exact_cumsum(n_, w, c), "Weight::draw");
+        std::vector<double> h(n_);
+        check(vsmch_synchronize(), "Weight::draw");
+        check_hip(hipMemcpy(h.data(), c, sizeof(double) * n_, hipMemcpyDeviceToHost),
+            "Weight::draw");
+        const auto it = std::lower_bound(h.begin(), h.end(), u);
+        return it == h.end() ? n_ - 1 : static_cast<size_type>(it - h.begin());
+    }
+
     // log of the normalising constant accumulated by set_log / add_log
     // (user_guide/tex/smc.tex:169-182; the reference discards it)
     double log_zconst() const
@@ -237,6 +412,55 @@ class WeightHIP {
   private:
     vsmch_particle *h_ = nullptr;
     size_type n_ = 0;
+    std::shared_ptr<detail::DeviceDoubles> buf_;   // data(), staged inputs
+    std::shared_ptr<detail::DeviceDoubles> cum_;   // draw()
+
+    std::vector<double> host_weights() const
+    {
+        std::vector<double> h(n_);
+        const double *d = data();
+        check(vsmch_synchronize(), "Weight::read_weight");
+        check_hip(hipMemcpy(h.data(), d, sizeof(double) * n_, hipMemcpyDeviceToHost),
+            "Weight::read_weight");
+        return h;
+    }
+
+    void update(int op, const double *first, int stride, const char *what)
+    {
+        if (stride < 1)
+            throw std::runtime_error(std::string(what) + ": stride must be >= 1");
+        if (detail::is_device_ptr(first)) {
+            check(vsmch_weight_update(h_, op, first, stride), what);
+            return;
+        }
+        std::vector<double> h(n_);
+        for (size_type i = 0; i != n_; ++i)
+            h[i] = first[i * static_cast<size_type>(stride)];
+        stage(op, h, what);
+    }
+    void update(int op, double *first, int stride, const char *what)
+    {
+        update(op, const_cast<const double *>(first), stride, what);
+    }
+    template <typename It>
+    void update(int op, It first, int stride, const char *what)
+    {
+        std::vector<double> h(n_);
+        for (size_type i = 0; i != n_; ++i) {
+            h[i] = static_cast<double>(*first);
+            if (i + 1 != n_)
+                std::advance(first, stride);
+        }
+        stage(op, h, what);
+    }
+    void stage(int op, const std::vector<double> &h, const char *what)
+    {
+        double *d = buf_->get(n_);
+        check_hip(hipMemcpy(d, h.data(), sizeof(double) * n_, hipMemcpyHostToDevice), what);
+        check(vsmch_weight_update(h_, op, d, 1), what);
+        // the update reads the staging buffer on the library stream
+        check(vsmch_synchronize(), what);
+    }
 };
 
 // Particle<T> (include/vsmc/core/particle.hpp:47-210) over a device
@@ -260,7 +484,7 @@ class Particle {
         check(vsmch_particle_create(&h_, &cfg), "Particle");
         void *st = nullptr;
         check(vsmch_particle_state(h_, &st), "Particle");
-        value_.bind_(static_cast<typename T::state_type *>(st));
+        value_.bind_(static_cast<typename T::state_type *>(st), h_);
         weight_ = WeightHIP(h_, N);
         uint64_t *w = nullptr;
         check(vsmch_particle_rng_words(h_, &w), "Particle");

@@ -14,6 +14,8 @@
 // lines from the state's operator<<).
 #include <vsmch/sampler.hpp>
 
+#include <chrono>
+
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
@@ -183,6 +185,25 @@ int main(int argc, char **argv)
         vsmch::Sampler<PFState> sampler(N,
             static_cast<vsmch::ResampleScheme>(scheme), threshold);
         sampler.init(PFInit()).move(PFMove(), false).monitor("pos", 2, PFMEval());
+        if (argc > 7 && std::string(argv[7]) == "time") {
+            // initialize + 3 untimed steps, then T - 4 timed steps (host
+            // clock around the steps, the queue drained on both sides)
+            sampler.initialize(argv[6]).iterate(3);
+            vsmch::check(vsmch_synchronize(), "time");
+            vsmch::check_hip(hipDeviceSynchronize(), "time");
+            const auto t0 = std::chrono::steady_clock::now();
+            sampler.iterate(T - 4);
+            vsmch::check(vsmch_synchronize(), "time");
+            vsmch::check_hip(hipDeviceSynchronize(), "time");
+            const auto t1 = std::chrono::steady_clock::now();
+            std::size_t nres = 0;
+            for (std::size_t t = 4; t != T; ++t)
+                nres += sampler.resampled_history(t) ? 1 : 0;
+            std::cout << "TIME\t" << (T - 4) << '\t'
+                      << std::chrono::duration<double, std::milli>(t1 - t0).count()
+                      << '\t' << nres << '\n';
+            return 0;
+        }
         sampler.initialize(argv[6]).iterate(T - 1);
         std::cout.precision(17);
         std::cout << sampler;

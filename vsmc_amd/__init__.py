@@ -12,6 +12,8 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libvsmch.so")
 
+WEIGHT_SET_LOG, WEIGHT_ADD_LOG, WEIGHT_SET, WEIGHT_MUL = 0, 1, 3, 4
+
 PHILOX4X32 = 0
 THREEFRY4X64 = 1
 
@@ -69,6 +71,8 @@ SIGNATURES = {
     "vsmch_weight_set_equal": (_c.c_int, [_P]),
     "vsmch_weight_set_log": (_c.c_int, [_P, _P]),
     "vsmch_weight_add_log": (_c.c_int, [_P, _P]),
+    "vsmch_weight_update": (_c.c_int, [_P, _c.c_int, _P, _c.c_int64]),
+    "vsmch_particle_copy": (_c.c_int, [_P, _c.c_int64, _P]),
     "vsmch_weight_ess": (_c.c_int, [_P, _c.POINTER(_c.c_double)]),
     "vsmch_weight_log_zconst": (_c.c_int, [_P, _c.POINTER(_c.c_double)]),
     "vsmch_weight_read": (_c.c_int, [_P, _P]),

@@ -60,7 +60,8 @@ def build(verbose=True):
 ROOT = os.path.dirname(HERE)
 CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", "pf_hip.cpp"),
              os.path.join(ROOT, "tests", "cpp", "rindex_hip.cpp"),
-             os.path.join(ROOT, "tests", "cpp", "rw_hip.cpp")]
+             os.path.join(ROOT, "tests", "cpp", "rw_hip.cpp"),
+             os.path.join(ROOT, "tests", "cpp", "plug_hip.cpp")]
 
 
 def build_cpp_tests(verbose=True):

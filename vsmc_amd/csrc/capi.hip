@@ -407,6 +407,27 @@ vsmch_status vsmch_weight_add_log(vsmch_particle *p, const double *inc)
     return weight_update(p->p, inc, 1);
 }
 
+vsmch_status vsmch_weight_update(vsmch_particle *p, int op, const double *src,
+    int64_t stride)
+{
+    if (!p || !src || stride < 1 ||
+        (op != VSMCH_WEIGHT_SET_LOG && op != VSMCH_WEIGHT_ADD_LOG &&
+         op != VSMCH_WEIGHT_SET && op != VSMCH_WEIGHT_MUL)) {
+        set_error("vsmch_weight_update: invalid argument");
+        return VSMCH_ERR_INVALID;
+    }
+    return weight_update(p->p, src, op, stride);
+}
+
+vsmch_status vsmch_particle_copy(vsmch_particle *p, int64_t n, const int *index)
+{
+    if (!p || !index || n != p->p->n) {
+        set_error("vsmch_particle_copy: invalid argument (n must equal N)");
+        return VSMCH_ERR_INVALID;
+    }
+    return state_copy(p->p, index);
+}
+
 vsmch_status vsmch_weight_ess(vsmch_particle *p, double *ess_host)
 {
     if (!p || !ess_host)

@@ -83,16 +83,21 @@ __device__ inline double logw_of(const double *l, long long i, const PDev *pd)
     return pd->equal ? pd->log_equal : l[i] - pd->c;
 }
 
-// mode 0: set_log (l = src), mode 1: add_log (l = log W + src)
+// Weight updates from src[i * stride] (weight.hpp:105-171): mode 0 set_log
+// (l = src), 1 add_log (l = log W + src), 3 set (l = log src), 4 mul
+// (l = log W + log src). Linear weights are kept as their logarithms.
 __global__ void __launch_bounds__(kPB) k_weight_pass(double *l,
-    const double *src, long long n, int mode, const PDev *pd, Part3 *part)
+    const double *src, long long n, int mode, long long stride, const PDev *pd,
+    Part3 *part)
 {
     const long long base = static_cast<long long>(blockIdx.x) * kPTile;
     Part3 acc{-INFINITY, 0.0, 0.0};
     for (int j = 0; j < kPItems; ++j) {
         const long long i = base + j * kPB + threadIdx.x;
         if (i < n) {
-            const double v = mode == 0 ? src[i] : logw_of(l, i, pd) + src[i];
+            const double a = src[i * stride];
+            const double in = mode >= 3 ? log(a) : a;
+            const double v = (mode == 0 || mode == 3) ? in : logw_of(l, i, pd) + in;
             l[i] = v;
             acc = part_merge(acc, part_of(v));
         }
@@ -1222,16 +1227,69 @@ void particle_destroy(ParticleSet *p)
     delete p;
 }
 
-int weight_update(ParticleSet *p, const double *src, int mode)
+int weight_update(ParticleSet *p, const double *src, int mode, long long stride)
 {
     hipStream_t st = p->ws.stream;
     const long long nb = pblocks(p->n);
     if (mode != 2)
         hipLaunchKernelGGL(k_weight_pass, dim3(nb), dim3(kPB), 0, st, p->logw,
-            src, p->n, mode, p->pd, static_cast<Part3 *>(p->part));
+            src, p->n, mode, stride, p->pd, static_cast<Part3 *>(p->part));
+    // set merges as set_log (the previous weights do not enter the evidence
+    // increment), mul as add_log
+    const int merge = mode == 3 ? 0 : (mode == 4 ? 1 : mode);
     hipLaunchKernelGGL(k_weight_finalize, dim3(1), dim3(kMergeThreads), 0, st, static_cast<const Part3 *>(p->part),
-        nb, p->n, mode, p->pd);
+        nb, p->n, merge, p->pd);
     VSMCH_HIP(hipGetLastError());
+    return kOk;
+}
+
+// StateMatrix::copy (state_matrix.hpp:246-253) on the SoA state: particle
+// dst takes the pre-copy state of particle idx[dst]. Reads go to the old
+// state (gathered into `to`, then copied back), which equals the
+// reference's in-order loop whenever every source keeps its own slot
+// (idx[src] == src), as in every index of resample_trans_rep_index.
+template <typename E>
+__global__ void __launch_bounds__(kPB) k_copy_gather(const E *from, E *to,
+    const int *idx, long long n, int dim, int *bad)
+{
+    const long long i = static_cast<long long>(blockIdx.x) * kPB + threadIdx.x;
+    if (i >= n)
+        return;
+    const long long a = idx[i];
+    if (a < 0 || a >= n) {
+        *bad = 1;   // rejected below: nothing is copied back
+        return;
+    }
+    for (int d = 0; d < dim; ++d)
+        to[d * n + i] = from[d * n + a];
+}
+
+int state_copy(ParticleSet *p, const int *idx)
+{
+    hipStream_t st = p->ws.stream;
+    const size_t bytes = static_cast<size_t>(p->n) * p->dim * p->bytes;
+    if (!p->state_alt)
+        VSMCH_HIP(hipMalloc(&p->state_alt, bytes));
+    int *bad = reinterpret_cast<int *>(p->mout);
+    VSMCH_HIP(hipMemsetAsync(bad, 0, sizeof(int), st));
+    const dim3 g(static_cast<unsigned int>((p->n + kPB - 1) / kPB));
+    if (p->bytes == 4)
+        hipLaunchKernelGGL(k_copy_gather<float>, g, dim3(kPB), 0, st,
+            static_cast<const float *>(p->state), static_cast<float *>(p->state_alt),
+            idx, p->n, p->dim, bad);
+    else
+        hipLaunchKernelGGL(k_copy_gather<double>, g, dim3(kPB), 0, st,
+            static_cast<const double *>(p->state), static_cast<double *>(p->state_alt),
+            idx, p->n, p->dim, bad);
+    VSMCH_HIP(hipGetLastError());
+    int hbad = 0;
+    VSMCH_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
+    VSMCH_HIP(hipStreamSynchronize(st));
+    if (hbad) {
+        set_error("copy: an index lies outside [0, N)");
+        return kErrInvalid;
+    }
+    VSMCH_HIP(hipMemcpyAsync(p->state, p->state_alt, bytes, hipMemcpyDeviceToDevice, st));
     return kOk;
 }
 

@@ -93,7 +93,12 @@ int weighted_sum(ParticleSet *p, int dim, const double *buf, double *out_dev);
 int particle_create(ParticleSet **out, long long n, int dim, int bytes,
     uint32_t seed, hipStream_t st);
 void particle_destroy(ParticleSet *p);
-int weight_update(ParticleSet *p, const double *src, int mode);
+// mode 0 set_log, 1 add_log, 2 set_equal, 3 set (linear), 4 mul (linear);
+// src[i * stride]
+int weight_update(ParticleSet *p, const double *src, int mode,
+    long long stride = 1);
+// StateMatrix::copy with a device index array of n entries
+int state_copy(ParticleSet *p, const int *idx);
 int read_pdev(ParticleSet *p, PDev *h);
 int weight_read(ParticleSet *p, double *W);
 int resample_step(ParticleSet *p, int scheme, double threshold, long long t,

@@ -80,6 +80,20 @@ class Particle:
         check(lib().vsmch_weight_add_log(self._h, ctypes.c_void_p(inc.data_ptr())))
         self._keep = inc
 
+    def weight_update(self, op, src, stride=1):
+        """Weight::set / mul / set_log / add_log with a stride (op one of
+        WEIGHT_SET, WEIGHT_MUL, WEIGHT_SET_LOG, WEIGHT_ADD_LOG): particle i
+        takes src[i * stride] of the flat device array src."""
+        src = src.to(device="cuda", dtype=_f64()).contiguous()
+        check(lib().vsmch_weight_update(self._h, op, ctypes.c_void_p(src.data_ptr()), stride))
+        self._keep = src
+
+    def copy(self, idx):
+        """StateMatrix::copy(N, idx): particle i takes the state of idx[i]."""
+        import torch
+        idx = idx.to(device="cuda", dtype=torch.int32).contiguous()
+        check(lib().vsmch_particle_copy(self._h, idx.numel(), ctypes.c_void_p(idx.data_ptr())))
+
     def ess(self):
         v = ctypes.c_double()
         check(lib().vsmch_weight_ess(self._h, ctypes.byref(v)))
