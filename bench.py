@@ -219,30 +219,37 @@ def _train_net(va, torch, model, track, n=1 << 14, steps=600):
     net.init_random(1)
     loss, s_train = _timed(torch, lambda: net.train(feat, target, batch=4096, steps=steps,
                                                     lr=2e-3))
-    return net, {"samples": int(feat.shape[0]), "steps": steps, "batch": 4096,
-                 "data_gpu_s": s_data, "train_gpu_s": s_train,
-                 "loss_first": float(loss[0]), "loss_last20": float(loss[-20:].mean())}
+    return net, loss, {"samples": int(feat.shape[0]), "steps": steps, "batch": 4096,
+                       "data_gpu_s": s_data, "train_gpu_s": s_train,
+                       "loss_first": float(loss[0]), "loss_last20": float(loss[-20:].mean()),
+                       "gain_nats": net.gain(loss)}
 
 
 def learned_vs_plain(va, torch, tracks=64, T=50, sizes=(1 << 10, 1 << 12, 1 << 14, 1 << 16,
-                                                           1 << 18)):
+                                                           1 << 18), t_train=64):
     """log-evidence RMSE against the FP64 Kalman filter vs GPU-seconds per
     filter run, learned proposal vs bootstrap, on the linear-Gaussian CV
     model (obs sd 0.01), T = 50, systematic resampling at ESS < N/2; plus
     the Student-t model with 5% outliers at 10x scale against a 2^22-particle
     bootstrap reference. Each (method, N) runs the tracks as one batch of
     filters (one graph-replayed step for all of them): GPU-seconds per filter
-    = the batch's GPU time / tracks."""
+    = the batch's GPU time / tracks. The network trains on the bootstrap
+    filter's transitions over a t_train-step track (2^14 particles, (t_train
+    - 1) 2^14 >= 10^6 transitions); when it gains less than 0.1 nats per
+    transition over the bootstrap proposal the learned run falls back to the
+    bootstrap proposal (ProposalNet.select). The bootstrap curve extends to
+    16 times the largest learned N, so every learned RMSE has a bootstrap
+    point at least as accurate."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
     import oracle_lib as ol
     from vsmc_amd.models import cv_simulate, cv_simulate_many
 
-    def sim(kind, F, seed):
+    def sim(kind, F, seed, steps=T):
         if kind == "lg":
-            return cv_simulate_many(F, T, seed, sd_pos0=0.02, sd_vel0=0.02, obs="gauss",
+            return cv_simulate_many(F, steps, seed, sd_pos0=0.02, sd_vel0=0.02, obs="gauss",
                                     obs_scale=0.01)
-        return cv_simulate_many(F, T, seed, sd_pos0=0.02, sd_vel0=0.02, obs="student",
+        return cv_simulate_many(F, steps, seed, sd_pos0=0.02, sd_vel0=0.02, obs="student",
                                 obs_scale=0.1, obs_df=10.0, outlier_frac=0.05,
                                 outlier_mult=10.0)
 
@@ -259,8 +266,11 @@ def learned_vs_plain(va, torch, tracks=64, T=50, sizes=(1 << 10, 1 << 12, 1 << 1
     out = {}
     for kind, model in (("linear_gaussian", LG_MODEL), ("student_t_outliers", ST_MODEL)):
         k = "lg" if kind == "linear_gaussian" else "st"
-        tx, ty = sim(k, 1, 11)
-        net, train = _train_net(va, torch, model, (tx[0], ty[0]))
+        tx, ty = sim(k, 1, 11, t_train)
+        net, loss, train = _train_net(va, torch, model, (tx[0], ty[0]))
+        use = net.select(loss)
+        train["proposal"] = "learned" if use is not None else \
+            "bootstrap (fallback: gain below 0.1 nats per transition)"
         ox, oy = sim(k, tracks, 1000)
         if k == "lg":
             ref = np.array([ol.kalman_cv(ox[s], oy[s], 0.1, 0.02, 0.02, 0.02, 0.02, 0.01)[0]
@@ -270,9 +280,9 @@ def learned_vs_plain(va, torch, tracks=64, T=50, sizes=(1 << 10, 1 << 12, 1 << 1
             ref, _ = run_batch(model, 1 << 22, None, ox, oy)
             ref_desc = "bootstrap filter at N = 2^22"
         rows = []
-        for n in sizes + (sizes[-1] * 4,):
-            for name, nn in (("bootstrap", None), ("learned", net)):
-                if nn is not None and n > sizes[-1]:
+        for n in sizes + (sizes[-1] * 4, sizes[-1] * 16):
+            for name, nn in (("bootstrap", None), ("learned", use)):
+                if name == "learned" and n > sizes[-1]:
                     continue   # the bootstrap curve extends past the learned sizes
                 z, sec = run_batch(model, n, nn, ox, oy)
                 errs = z - ref

@@ -185,7 +185,10 @@ double vo_mlp_loss_grad(const float *params, const double *shift,
  * s1[4] and the log-weight increment
  *   log g(y | x_t) + log f(x_t | x_{t-1}) - log q(x_t | x_{t-1}, y)
  * = ll(x_t) + sum_d [ls_d + z_d^2 / 2 - u_d^2 / 2],
- * with ls_d clamped to [-5, 3] as on the device. obs_model / scale / df as
+ * with ls_d clamped to [-5, 3] as on the device, the lower end raised to
+ * -0.3465 (just above -ln(2) / 2, so e^(2 ls) > 1/2) for the Student-t
+ * likelihood with constants (obs_model 1): below it p g / q has an infinite
+ * second moment under polynomial tails. obs_model / scale / df as
  * vo_pf_config (0: Student-t without constants, 1: with, 2: Gaussian). */
 double vo_cv_learned_particle(const double s0[4], double ox, double oy,
     const double o[8], const double z[4], double dt, double sd_pos,
@@ -195,9 +198,10 @@ double vo_cv_learned_particle(const double s0[4], double ox, double oy,
     const double m[4] = {s0[0] + dt * s0[2], s0[1] + dt * s0[3], s0[2], s0[3]};
     const double sc[4] = {sd_pos, sd_pos, sd_vel, sd_vel};
     double inc = 0;
+    const double ls_min = obs_model == 1 ? (double)-0.3465f : -5;
     for (int d = 0; d < 4; ++d) {
         double ls = o[4 + d];
-        ls = ls < -5 ? -5 : (ls > 3 ? 3 : ls);
+        ls = ls < ls_min ? ls_min : (ls > 3 ? 3 : ls);
         const double u = o[d] + exp(ls) * z[d];
         s1[d] = m[d] + sc[d] * u;
         inc += ls + 0.5 * z[d] * z[d] - 0.5 * u * u;

@@ -239,3 +239,49 @@ def test_learned_pf_large_n(dev):
         assert np.isfinite(r["pos"]).all()
         assert abs(r["logz"][-1] - kz) < 0.5
     net.close()
+
+
+def test_collapse_falls_back_to_bootstrap(dev):
+    """A network whose proposal means sit 50 standard deviations off makes a
+    learned step's ESS collapse (below N / 1000). From then on the filter's
+    learned moves take the bootstrap proposal: from the same state, a
+    learned move equals the bootstrap move (to FP32 rounding of the two
+    orders of operations), while before any collapse it does not."""
+    import vsmc_amd as va
+    n = 1 << 14
+    p = np.zeros(ol.MLP_NPARAMS, np.float32)
+    p[ol.MLP_NPARAMS - 8:ol.MLP_NPARAMS - 4] = 50.0   # b3: the mean offsets
+    net = va.ProposalNet()
+    net.set_params(p, np.zeros(6, np.float32), np.ones(6, np.float32))
+    ox, oy = _lg_track(4, 7)
+
+    def start(collapse):
+        f = va.CvFilter(n, "float32", seed=3, **LG)
+        f.init(float(ox[0]), float(oy[0]))
+        f.resample(va.SYSTEMATIC, 0.5)
+        if collapse:
+            f.move_learned(net, 1, float(ox[1]), float(oy[1]))
+            assert f.ess() < n / 1000
+            f.resample(va.SYSTEMATIC, 0.5)
+        return f
+
+    a, b = start(True), start(True)
+    a.move(2, float(ox[2]), float(oy[2]))                 # bootstrap kernel
+    b.move_learned(net, 2, float(ox[2]), float(oy[2]))    # after the collapse
+    sa, sb = a.state().cpu().numpy(), b.state().cpu().numpy()
+    wa, wb = a.weights().cpu().numpy(), b.weights().cpu().numpy()
+    np.testing.assert_allclose(sb, sa, rtol=1e-5, atol=1e-6)
+    keep = wa > 1e-200
+    # log-likelihoods near -90 (1.0 from an observation with sd 0.01): the
+    # FP32 state's rounding moves them by up to about 1e-4 relative
+    np.testing.assert_allclose(np.log(wb[keep]), np.log(wa[keep]), rtol=1e-4, atol=1e-3)
+    a.close()
+    b.close()
+    # without a collapse the learned move is the network's
+    c, d = start(False), start(False)
+    c.move(1, float(ox[1]), float(oy[1]))
+    d.move_learned(net, 1, float(ox[1]), float(oy[1]))
+    assert np.abs(d.state().cpu().numpy() - c.state().cpu().numpy()).max() > 0.5
+    c.close()
+    d.close()
+    net.close()

@@ -120,7 +120,8 @@ def test_learned_particle_weight(obs_model):
                                          0.1, 10.0)
         m = np.array([s0[0] + dt * s0[2], s0[1] + dt * s0[3], s0[2], s0[3]])
         sc = np.array([sp, sp, sv, sv])
-        ls = np.clip(o[4:], -5, 3)
+        floor = float(np.float32(-0.3465)) if obs_model == 1 else -5.0
+        ls = np.clip(o[4:], floor, 3)
         mu_q = m + sc * o[:4]
         sd_q = sc * np.exp(ls)
         np.testing.assert_allclose(s1, mu_q + sd_q * z, rtol=1e-14, atol=1e-14)
@@ -135,6 +136,22 @@ def test_learned_particle_weight(obs_model):
                 logg -= 2 * (math.lgamma(5.5) - math.lgamma(5.0)
                              - 0.5 * math.log(10 * math.pi) - math.log(0.1))
         assert abs(inc - (logg + logf - logq)) < 1e-9, (inc, logg + logf - logq)
+
+
+def test_heavy_tail_scale_floor():
+    """Under the Student-t likelihood (obs_model 1) the proposal's log-scale
+    is floored just above -ln(2)/2 (e^(2 ls) > 1/2: p g / q then has a finite
+    second moment); the Gaussian likelihood keeps the -5 floor."""
+    floor = float(np.float32(-0.3465))
+    assert 0.5 < math.exp(2 * floor) < 0.5002
+    s0 = np.array([0.3, -0.2, 1.0, 0.5])
+    z = np.array([0.1, -1.2, 0.7, 2.0])
+    o = np.concatenate([np.zeros(4), np.full(4, -3.0)])
+    m = np.array([s0[0] + 0.1 * s0[2], s0[1] + 0.1 * s0[3], s0[2], s0[3]])
+    sc = np.array([0.02, 0.02, 0.02, 0.02])
+    for model, ls in ((1, floor), (2, -3.0), (0, -3.0)):
+        s1, _ = ol.cv_learned_particle(s0, 0.4, -0.1, o, z, 0.1, 0.02, 0.02, model, 0.1, 10.0)
+        np.testing.assert_allclose(s1, m + sc * np.exp(ls) * z, rtol=1e-14, atol=1e-15)
 
 
 def test_bootstrap_proposal_weight():

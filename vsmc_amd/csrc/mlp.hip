@@ -47,27 +47,36 @@ __global__ void __launch_bounds__(kMB) k_pack(const float *__restrict__ p,
     const int kq = 4 * (l >> 4);
     int e = id >> 6;     // entry index over [f1 4 | f2 16 | f3 4 | g3 4 | g2 16]
     bf16x4 v{0, 0, 0, 0};
+    const int g = l >> 4;
     if (e < 4) {
         const int rt = e;
 #pragma unroll
         for (int q = 0; q < 4; ++q)
             if (kq + q < kIn)
                 v[q] = mlp::f2bf(p[mlp::oW1 + (rt * 16 + li) * kIn + kq + q]);
-        F->f1[rt][l] = v;
+        F->fwd.f1[rt][l] = v;
     } else if ((e -= 4) < 16) {
-        const int rt = e >> 2, kt = e & 3;
+        // K = 32 fragments: 8 rt x kt pairs, the other 8 entries idle
+        if (e < 8) {
+            const int rt = e >> 1, kt = e & 1;
+            mlp::bf16x8 w;
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            v[q] = mlp::f2bf(p[mlp::oW2 + (rt * 16 + li) * kH + kt * 16 + kq + q]);
-        F->f2[rt][kt][l] = v;
-    } else if ((e -= 16) < 4) {
-        const int kt = e;
-        if (li < kOut) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                v[q] = mlp::f2bf(p[mlp::oW3 + li * kH + kt * 16 + kq + q]);
+            for (int q = 0; q < 8; ++q)
+                w[q] = __builtin_bit_cast(__bf16,
+                    mlp::f2bf(p[mlp::oW2 + (rt * 16 + li) * kH + mlp::kslot(kt, g, q)]));
+            F->fwd.f2[rt][kt][l] = w;
         }
-        F->f3[kt][l] = v;
+    } else if ((e -= 16) < 4) {
+        if (e < 2) {
+            const int kt = e;
+            mlp::bf16x8 w;
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                w[q] = __builtin_bit_cast(__bf16, li < kOut ?
+                    mlp::f2bf(p[mlp::oW3 + li * kH + mlp::kslot(kt, g, q)]) :
+                    static_cast<uint16_t>(0));
+            F->fwd.f3[kt][l] = w;
+        }
     } else if ((e -= 4) < 4) {
         const int rt = e;
 #pragma unroll
@@ -83,13 +92,13 @@ __global__ void __launch_bounds__(kMB) k_pack(const float *__restrict__ p,
         F->g2[rt][kt][l] = v;
     } else if ((e -= 16) == 0) {
         // biases and input normalisation (64 threads)
-        F->b1[l] = p[mlp::oB1 + l];
-        F->b2[l] = p[mlp::oB2 + l];
+        F->fwd.b1[l] = p[mlp::oB1 + l];
+        F->fwd.b2[l] = p[mlp::oB2 + l];
         if (l < 16)
-            F->b3[l] = l < kOut ? p[mlp::oB3 + l] : 0.0f;
+            F->fwd.b3[l] = l < kOut ? p[mlp::oB3 + l] : 0.0f;
         if (l < 8) {
-            F->shift[l] = l < kIn ? shift[l] : 0.0f;
-            F->scale[l] = l < kIn ? scale[l] : 0.0f;
+            F->fwd.shift[l] = l < kIn ? shift[l] : 0.0f;
+            F->fwd.scale[l] = l < kIn ? scale[l] : 0.0f;
         }
     }
 }
@@ -120,7 +129,7 @@ __global__ void __launch_bounds__(kMB) k_forward(const Frags *__restrict__ F,
         float f[6];
         load_feat(feat, s, valid, f);
         mlp::Fwd r;
-        mlp::forward_tile(F, mlp::input_frag(F, f), r);
+        mlp::forward_tile(&F->fwd, mlp::input_frag(&F->fwd, f), r);
         if (valid && fo < kOut) {
 #pragma unroll
             for (int v = 0; v < 4; ++v)
@@ -195,9 +204,9 @@ __global__ void __launch_bounds__(kMB) k_grad(const Frags *__restrict__ F,
         const long long s = off + j * stride;
         float f[6];
         load_feat(feat, s, valid, f);
-        const bf16x4 xin = mlp::input_frag(F, f);
+        const bf16x4 xin = mlp::input_frag(&F->fwd, f);
         mlp::Fwd r;
-        mlp::forward_tile(F, xin, r);
+        mlp::forward_tile(&F->fwd, xin, r);
         // output gradient (lanes 0-15: means, 16-31: log-scales)
         f32x4 other;
 #pragma unroll

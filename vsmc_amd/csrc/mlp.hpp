@@ -5,18 +5,24 @@
 // learned proposal; the model it corrects is the PF of
 // user_guide/cpp/pf_seq.cpp:70-81, :111-122).
 //
-// Layout. Every layer is evaluated transposed, Z^T = W H^T, with
-// v_mfma_f32_16x16x16_bf16: A = a 16x16 weight tile (output features x
-// input features), B = a 16x16 activation tile (input features x 16
-// samples), accumulator C/D = output features x samples. In that
-// instruction lane l holds A[i][k] for i = l % 16, k = 4 (l / 16) + v;
-// B[k][j] for j = l % 16, k = 4 (l / 16) + v; and D[i][j] for j = l % 16,
-// i = 4 (l / 16) + v (v = 0..3). The D tile of one layer is therefore
-// already the B tile of the next: activations never leave registers in the
-// forward pass. Weight tiles are pre-swizzled into per-lane 8-byte
-// fragments (Frags) by a pack kernel after every optimiser step.
-// Accumulation is FP32; inputs, weights and activations are bfloat16
-// (round to nearest even).
+// Layout. Every layer is evaluated transposed, Z^T = W H^T: A = a weight
+// tile (output features x input features), B = an activation tile (input
+// features x 16 samples), accumulator C/D = output features x samples. The
+// first layer (6 inputs) uses v_mfma_f32_16x16x16_bf16, where lane l holds
+// A[i][k] for i = l % 16, k = 4 (l / 16) + v and B[k][j] for j = l % 16,
+// k = 4 (l / 16) + v (v = 0..3); the 64-wide layers use gfx950's
+// v_mfma_f32_16x16x32_bf16, where lane l holds A[i][k] and B[k][j] for
+// k = 8 (l / 16) + v (v = 0..7). Both write D[i][j] for j = l % 16,
+// i = 4 (l / 16) + v (v = 0..3). A contraction index may be numbered in any
+// order shared by A and B, so the K = 32 slot 8 g + v of half kt of a
+// 64-wide input is hidden unit kslot(kt, g, v): rows 4 g + v of its first
+// 16-row tile for v < 4, of its second tile for v >= 4 -- exactly the rows
+// lane group g holds of the previous layer's two D tiles. Activations thus
+// never leave registers in the forward pass. Weight tiles are pre-swizzled
+// into per-lane fragments (Frags) by a pack kernel after every optimiser
+// step. Accumulation is FP32; inputs, weights and activations are bfloat16
+// (round to nearest even, the hardware conversion); tanh comes from the
+// hardware exp2 and reciprocal (tanh_fast).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -38,19 +44,32 @@ constexpr int kParams = oB3 + kOut;   // 5128
 constexpr int kGradLen = kParams + 1; // + loss
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4h __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-struct Frags {
+// Hidden unit of K = 32 slot (g, v) of half kt of a 64-wide input.
+__host__ __device__ constexpr int kslot(int kt, int g, int v)
+{
+    return 32 * kt + (v < 4 ? 4 * g + v : 16 + 4 * g + (v - 4));
+}
+
+// What the forward pass reads (the learned move stages this in LDS).
+struct FwdFrags {
     bf16x4 f1[4][64];      // W1 as A: row tile rt (K = inputs, padded to 16)
-    bf16x4 f2[4][4][64];   // W2 as A: [rt][kt]
-    bf16x4 f3[4][64];      // W3 as A: [kt] (outputs padded to 16 rows)
-    bf16x4 g3[4][64];      // W3^T as A: [rt] (K = outputs padded to 16)
-    bf16x4 g2[4][4][64];   // W2^T as A: [rt][kt]
+    bf16x8 f2[4][2][64];   // W2 as A: [rt][kt], K slots by kslot
+    bf16x8 f3[2][64];      // W3 as A: [kt] (outputs padded to 16 rows)
     float b1[kH];
     float b2[kH];
     float b3[16];          // padded with zeros
     float shift[8];        // input normalisation (f - shift) * scale
     float scale[8];
+};
+
+struct Frags {
+    FwdFrags fwd;
+    bf16x4 g3[4][64];      // W3^T as A: [rt] (K = outputs padded to 16)
+    bf16x4 g2[4][4][64];   // W2^T as A: [rt][kt]
 };
 
 __host__ __device__ inline uint16_t f2bf(float f)
@@ -67,14 +86,21 @@ __host__ __device__ inline float bf2f(uint16_t b)
     return __builtin_bit_cast(float, static_cast<uint32_t>(b) << 16);
 }
 
+// Round to nearest even, by the hardware conversion (the same results as
+// f2bf for every non-NaN input).
 __device__ inline bf16x4 pack4(f32x4 v)
 {
-    bf16x4 r;
-    r[0] = static_cast<short>(f2bf(v[0]));
-    r[1] = static_cast<short>(f2bf(v[1]));
-    r[2] = static_cast<short>(f2bf(v[2]));
-    r[3] = static_cast<short>(f2bf(v[3]));
-    return r;
+    const bf16x4h h{static_cast<__bf16>(v[0]), static_cast<__bf16>(v[1]),
+        static_cast<__bf16>(v[2]), static_cast<__bf16>(v[3])};
+    return __builtin_bit_cast(bf16x4, h);
+}
+
+// The K = 32 operand of two 4-row fragments (slots v < 4, then v >= 4).
+__device__ inline bf16x8 cat8(bf16x4 lo, bf16x4 hi)
+{
+    const bf16x4h a = __builtin_bit_cast(bf16x4h, lo);
+    const bf16x4h b = __builtin_bit_cast(bf16x4h, hi);
+    return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7);
 }
 
 __device__ inline f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c)
@@ -82,14 +108,31 @@ __device__ inline f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c)
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+__device__ inline f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
 __device__ inline f32x4 load4(const float *p)
 {
     return f32x4{p[0], p[1], p[2], p[3]};
 }
 
+// tanh x = 1 - 2 / (2^(2 x log2 e) + 1) with v_exp_f32 and v_rcp_f32. With
+// both within about one ulp, the absolute error stays below 1e-6 for every
+// x (at most about 0.5 of exp's relative error, plus 2 ulp of the
+// reciprocal, plus the final rounding); large |x| gives exactly +-1, NaN
+// stays NaN. The activations are rounded to bfloat16 (relative 2^-9) right
+// after, so this error is below their resolution wherever |tanh| > 2e-4.
+__device__ inline float tanh_fast(float x)
+{
+    const float e = __builtin_amdgcn_exp2f(x * 2.88539008177792681f);
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
+
 __device__ inline f32x4 tanh4(f32x4 v)
 {
-    return f32x4{tanhf(v[0]), tanhf(v[1]), tanhf(v[2]), tanhf(v[3])};
+    return f32x4{tanh_fast(v[0]), tanh_fast(v[1]), tanh_fast(v[2]), tanh_fast(v[3])};
 }
 
 // Forward pass of one 16-sample column tile. x: this lane's input
@@ -102,7 +145,7 @@ struct Fwd {
     f32x4 o;
 };
 
-__device__ inline void forward_tile(const Frags *__restrict__ F, bf16x4 x,
+__device__ inline void forward_tile(const FwdFrags *__restrict__ F, bf16x4 x,
     Fwd &r)
 {
     const int l = threadIdx.x & 63;
@@ -115,26 +158,26 @@ __device__ inline void forward_tile(const Frags *__restrict__ F, bf16x4 x,
         r.h1[rt] = tanh4(acc);
         h1b[rt] = pack4(r.h1[rt]);
     }
+    const bf16x8 h1k[2] = {cat8(h1b[0], h1b[1]), cat8(h1b[2], h1b[3])};
     bf16x4 h2b[4];
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt) {
         f32x4 acc = load4(&F->b2[rt * 16 + fo]);
 #pragma unroll
-        for (int kt = 0; kt < 4; ++kt)
-            acc = mfma16(F->f2[rt][kt][l], h1b[kt], acc);
+        for (int kt = 0; kt < 2; ++kt)
+            acc = mfma32(F->f2[rt][kt][l], h1k[kt], acc);
         r.h2[rt] = tanh4(acc);
         h2b[rt] = pack4(r.h2[rt]);
     }
     f32x4 acc = load4(&F->b3[fo]);
-#pragma unroll
-    for (int kt = 0; kt < 4; ++kt)
-        acc = mfma16(F->f3[kt][l], h2b[kt], acc);
+    acc = mfma32(F->f3[0][l], cat8(h2b[0], h2b[1]), acc);
+    acc = mfma32(F->f3[1][l], cat8(h2b[2], h2b[3]), acc);
     r.o = acc;
 }
 
 // Normalised, rounded input fragment of a lane from the sample's raw
 // features f[6] (lanes 32-63 carry the zero padding rows).
-__device__ inline bf16x4 input_frag(const Frags *__restrict__ F,
+__device__ inline bf16x4 input_frag(const FwdFrags *__restrict__ F,
     const float (&f)[6])
 {
     const int l = threadIdx.x & 63;

@@ -109,9 +109,10 @@ __global__ void __launch_bounds__(kPB) k_weight_pass(double *l,
 // Also prepares the resampling parameters of this step (weights from l)
 // and records the history entry `t` when hist is non-null. kThreads: the
 // block size it runs with.
+// guard: the step was a learned move; a collapsed ESS sets pd->fallback.
 template <int kThreads = kPB>
 __device__ inline void weight_merge(const Part3 *part, long long nblocks,
-    long long n, int mode, PDev *pd);
+    long long n, int mode, PDev *pd, int guard = 0);
 
 // One block of kMergeThreads: four times the threads of a particle block,
 // so each thread merges a quarter of the partials (the merge is a chain of
@@ -119,9 +120,9 @@ __device__ inline void weight_merge(const Part3 *part, long long nblocks,
 constexpr int kMergeThreads = 1024;
 
 __global__ void __launch_bounds__(kMergeThreads) k_weight_finalize(const Part3 *part,
-    long long nblocks, long long n, int mode, PDev *pd)
+    long long nblocks, long long n, int mode, PDev *pd, int guard = 0)
 {
-    weight_merge<kMergeThreads>(part, nblocks, n, mode, pd);
+    weight_merge<kMergeThreads>(part, nblocks, n, mode, pd, guard);
 }
 
 // Block reduction over kThreads threads, the result broadcast (lds: one
@@ -147,7 +148,7 @@ __device__ inline T block_reduce_n(T v, Op op, T *lds)
 // The body of k_weight_finalize (one block of kThreads).
 template <int kThreads>
 __device__ inline void weight_merge(const Part3 *part, long long nblocks,
-    long long n, int mode, PDev *pd)
+    long long n, int mode, PDev *pd, int guard)
 {
     __shared__ double dlds[kThreads / kWave];
     Part3 acc{-INFINITY, 0.0, 0.0};
@@ -197,6 +198,10 @@ __device__ inline void weight_merge(const Part3 *part, long long nblocks,
     const double inc = mode == 0 ? pd->c - log(static_cast<double>(n)) : pd->c;
     pd->logz_inc = inc;
     pd->logz += inc;
+    if (mode == 0)
+        pd->fallback = 0;
+    if (guard && !(pd->ess >= kLearnedCollapse * static_cast<double>(n)))
+        pd->fallback = 1;
 }
 
 // Resampling decision and event parameters (Particle::resample,
@@ -760,6 +765,8 @@ __global__ void __launch_bounds__(kPB) k_cv_step(Real *state, const Real *src,
 // chunk order into the tile partial. The choice depends on the filter's
 // size alone, so a filter gives the same results alone and in a batch.
 constexpr int kLChunks = kLearnedChunks;
+// -ln(2) / 2 rounded up: e^(2 ls) = 0.50007 at the floor
+constexpr float kLsFloorHeavy = -0.3465f;
 static_assert(kLChunks * kPB == kPTile, "a learned chunk is one block of particles");
 
 __device__ inline Part3 merge_chunks(const Part3 *c)
@@ -774,7 +781,7 @@ __device__ inline void cv_learned_span(float *__restrict__ X,
     float *__restrict__ Y, float *__restrict__ VX, float *__restrict__ VY,
     double *__restrict__ l, long long n, long long tile, int ch0, int nch,
     uint32_t seed, long long t, float ox, float oy, const CvModelDev &m,
-    const mlp::Frags *__restrict__ F, const PDev *pd, Part3 *out)
+    const mlp::FwdFrags *__restrict__ F, const PDev *pd, Part3 *out)
 {
     __shared__ float fl[kWaves][64][8];
     __shared__ float ol[kWaves][64][8];
@@ -789,6 +796,12 @@ __device__ inline void cv_learned_span(float *__restrict__ X,
     const float sp = static_cast<float>(m.sd_pos);
     const float sv = static_cast<float>(m.sd_vel);
     const unsigned long long blk = static_cast<unsigned long long>(t) + 1;
+    const bool boot = pd->fallback != 0;
+    // Under a Student-t likelihood (polynomial tails) the weight p g / q has
+    // a finite second moment only if the proposal's variance exceeds half the
+    // transition's: e^(2 ls) > 1/2, so ls stays above -ln(2) / 2 there. A
+    // Gaussian likelihood keeps it finite for any scale.
+    const float ls_min = m.obs_model == 1 ? kLsFloorHeavy : -5.0f;
     Part3 acc{-INFINITY, 0.0, 0.0};
     for (int ch = ch0; ch < ch0 + nch; ++ch) {
         const long long i = base + ch * kPB + wave * 64 + lane;
@@ -810,8 +823,10 @@ __device__ inline void cv_learned_span(float *__restrict__ X,
         fl[wave][lane][4] = ox - x;
         fl[wave][lane][5] = oy - y;
         __syncthreads();
+        // after a collapse the filter moves with the bootstrap proposal:
+        // zero network outputs (u = z, increment 0)
 #pragma unroll 1
-        for (int ct = 0; ct < 4; ++ct) {
+        for (int ct = 0; ct < (boot ? 0 : 4); ++ct) {
             const int pp = ct * 16 + (lane & 15);
             float g[6];
 #pragma unroll
@@ -824,6 +839,11 @@ __device__ inline void cv_learned_span(float *__restrict__ X,
                 for (int v = 0; v < 4; ++v)
                     ol[wave][pp][fo + v] = r.o[v];
             }
+        }
+        if (boot) {
+#pragma unroll
+            for (int v = 0; v < 8; ++v)
+                ol[wave][lane][v] = 0.0f;
         }
         __syncthreads();
         if (valid) {
@@ -839,7 +859,7 @@ __device__ inline void cv_learned_span(float *__restrict__ X,
 #pragma unroll
             for (int d = 0; d < 4; ++d) {
                 float ls = ol[wave][lane][4 + d];
-                ls = ls < -5.0f ? -5.0f : (ls > 3.0f ? 3.0f : ls);
+                ls = ls < ls_min ? ls_min : (ls > 3.0f ? 3.0f : ls);
                 const float u = ol[wave][lane][d] + expf(ls) * z[d];
                 s1[d] = mp[d] + sc[d] * u;
                 inc += ls + 0.5f * z[d] * z[d] - 0.5f * u * u;
@@ -860,14 +880,16 @@ __device__ inline void cv_learned_span(float *__restrict__ X,
     block_partials_at(acc, out);
 }
 
-// The network's fragments copied into LDS by the whole block (the forward
-// tiles then read them from LDS rather than through the vector cache).
-static_assert(sizeof(mlp::Frags) % 16 == 0, "fragments copy in 16-byte units");
-__device__ inline void frags_to_lds(const mlp::Frags *F, mlp::Frags *dst)
+// The forward pass's fragments copied into LDS by the whole block (the
+// forward tiles then read them from LDS rather than through the vector
+// cache); the backward-only fragments stay in global memory.
+static_assert(sizeof(mlp::FwdFrags) % 16 == 0, "fragments copy in 16-byte units");
+static_assert(alignof(mlp::FwdFrags) >= 16, "fragments copy in 16-byte units");
+__device__ inline void frags_to_lds(const mlp::FwdFrags *F, mlp::FwdFrags *dst)
 {
     const uint4 *s = reinterpret_cast<const uint4 *>(F);
     uint4 *d = reinterpret_cast<uint4 *>(dst);
-    for (int k = threadIdx.x; k < static_cast<int>(sizeof(mlp::Frags) / 16); k += kPB)
+    for (int k = threadIdx.x; k < static_cast<int>(sizeof(mlp::FwdFrags) / 16); k += kPB)
         d[k] = s[k];
     __syncthreads();
 }
@@ -879,8 +901,8 @@ __global__ void __launch_bounds__(kPB) k_cv_learned(float *__restrict__ state,
     float oy, CvModelDev m, const mlp::Frags *__restrict__ F, const PDev *pd,
     int chunked, Part3 *out)
 {
-    __shared__ mlp::Frags sF;
-    frags_to_lds(F, &sF);
+    __shared__ mlp::FwdFrags sF;
+    frags_to_lds(&F->fwd, &sF);
     const long long tile = chunked ? blockIdx.x / kLChunks : blockIdx.x;
     const int ch0 = chunked ? static_cast<int>(blockIdx.x % kLChunks) : 0;
     cv_learned_span(state, state + n, state + 2 * n, state + 3 * n, l, n, tile,
@@ -998,10 +1020,10 @@ __global__ void __launch_bounds__(kPB) k_batch_move(float *state, double *l,
     // chunk-major (u = ch ntiles + tile), so that the tiles' first units
     // are the first blocks, spread over every XCD.
     const int per = kLearned ? kLChunks : 1;
-    const mlp::Frags *nf = net;
+    const mlp::FwdFrags *nf = nullptr;
     if constexpr (kLearned) {
-        __shared__ mlp::Frags sF;
-        frags_to_lds(net, &sF);
+        __shared__ mlp::FwdFrags sF;
+        frags_to_lds(&net->fwd, &sF);
         nf = &sF;
     }
     for (long long u = blockIdx.x; u < ntiles * per; u += gridDim.x) {
@@ -1057,7 +1079,9 @@ __global__ void __launch_bounds__(kPB) k_batch_finalize(const BatchFilter *bf,
             part[F.tile0 + k] = merge_chunks(sub + (F.tile0 + k) * kLChunks);
         __syncthreads();
     }
-    weight_merge(part + F.tile0, F.ntiles, F.n, t == 0 ? 0 : 1, pd + f);
+    // a learned batch's moves (t >= 1) are guarded against collapse
+    weight_merge(part + F.tile0, F.ntiles, F.n, t == 0 ? 0 : 1, pd + f,
+        sub != nullptr && t > 0 ? 1 : 0);
     __shared__ int act;
     if (threadIdx.x == 0) {
         PDev *p = pd + f;
@@ -1552,7 +1576,7 @@ int cv_step_learned(ParticleSet *p, const CvModelDev &m, const MlpSet *net,
             0, st, static_cast<const Part3 *>(p->lsub), nb,
             static_cast<Part3 *>(p->part));
     hipLaunchKernelGGL(k_weight_finalize, dim3(1), dim3(kMergeThreads), 0, st,
-        static_cast<const Part3 *>(p->part), nb, p->n, 1, p->pd);
+        static_cast<const Part3 *>(p->part), nb, p->n, 1, p->pd, 1);
     VSMCH_HIP(hipGetLastError());
     return kOk;
 }

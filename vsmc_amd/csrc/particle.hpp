@@ -22,7 +22,14 @@ struct PDev {
     // the last event left its ancestors in ParticleSet::anc instead of
     // gathering the state: the next move reads particle i from anc[i]
     int anc_valid;
+    // learned proposal: a learned step's ESS collapsed below
+    // kLearnedCollapse N, and the filter's remaining moves take the bootstrap
+    // proposal (cleared by set_log / the initialisation)
+    int fallback;
 };
+
+// ESS fraction below which a learned step counts as collapsed.
+constexpr double kLearnedCollapse = 1e-3;
 
 // After a resampling event: equal weights, advance the resampling stream.
 // With `words` the event's stream use is computed here (systematic: the one

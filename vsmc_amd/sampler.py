@@ -357,6 +357,24 @@ class ProposalNet:
                                     loss.ctypes.data_as(ctypes.c_void_p)))
         return loss[:steps]
 
+    # Loss of the bootstrap proposal (mean 0, log-scale 0) on standardised
+    # transitions: E[sum_d z_d^2 / 2] over 4 components.
+    BOOTSTRAP_LOSS = 2.0
+
+    @classmethod
+    def gain(cls, losses, last=20):
+        """Nats per transition the trained network gains over the bootstrap
+        proposal (mean of the last training losses)."""
+        import numpy as np
+        return float(cls.BOOTSTRAP_LOSS - np.mean(np.asarray(losses)[-last:]))
+
+    def select(self, losses, min_gain=0.1):
+        """The proposal a filter should run with: this network, or None (the
+        bootstrap proposal) when its gain over the bootstrap is below
+        min_gain nats per transition -- too little to pay for the network
+        (for a gain g the ESS improves by at most about e^g)."""
+        return self if self.gain(losses) >= min_gain else None
+
 
 class GmmSampler:
     """Tempered SMC for a c-component Gaussian mixture on the device (the
