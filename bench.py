@@ -452,23 +452,49 @@ def dropin_path(n=1 << 24, T=24):
                     "Weight::add_log, Particle::resample on the device, monitor"}
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(budget_s):
+    """The oracle's OpenMP restatement of the reference's Sampler loop on the
+    host: the eval_sp loop in parallel, the weight normalisation, resampling,
+    rep -> index and copy sequential, as in the reference's OMP backend. The
+    threads are this process's CPU share (16 on a one-GPU box, whatever the
+    host's core count), with the per-phase split of the time."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
     from vsmc_amd.models import cv_simulate
-    threads = min(os.cpu_count() or 1, 16)
+    ncpu = os.cpu_count() or 1
+    try:
+        share = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        share = ncpu
+    threads = min(share, 16)
     n, T = 1 << 18, 10
     _, ox, oy = cv_simulate(200, 20161)
     # grow the sample until it takes a meaningful share of the budget
-    secs = ol.pf_run(n, T, 1, ol.SYSTEMATIC, 0.5, ox, oy, threads=threads)["seconds"]
-    while secs < budget_s / 4 and n < (1 << 24):
+    r = ol.pf_run(n, T, 1, ol.SYSTEMATIC, 0.5, ox, oy, threads=threads)
+    while r["seconds"] < budget_s / 4 and n < (1 << 24):
         n *= 2
-        secs = ol.pf_run(n, T, 1, ol.SYSTEMATIC, 0.5, ox, oy, threads=threads)["seconds"]
+        r = ol.pf_run(n, T, 1, ol.SYSTEMATIC, 0.5, ox, oy, threads=threads)
+    secs = r["seconds"]
     return {"value": n * T / secs, "unit": "particle-updates/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "host_cpus": ncpu, "affinity_cpus": share,
+            "cpu_model": _cpu_model(),
+            "phase_seconds": r["phase_seconds"],
+            "resampled_steps": int(r["resampled"].sum()),
             "sample": f"oracle Sampler restatement, N={n}, T={T}, FP64 state, "
-                      f"systematic, ESS<N/2; OpenMP eval_sp loop, sequential "
-                      f"weights/resampling as in the reference; {secs:.2f}s"}
+                      f"systematic, ESS<N/2; OpenMP eval_sp loop on {threads} threads, "
+                      f"sequential weights/resampling/copy as in the reference; "
+                      f"{secs:.2f}s"}
 
 
 def main():

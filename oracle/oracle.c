@@ -1000,6 +1000,18 @@ static double now_seconds(void)
     return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
+/* Seconds of the last vo_pf_run per phase: 0 the init / move functor with
+ * the log-likelihood (the parallel eval_sp loop), 1 the weight update
+ * (set_log / add_log: normalisation and ESS), 2 the resampling counts and
+ * rep -> index, 3 the state copy, 4 the monitor. */
+static double g_pf_phase[5];
+
+void vo_pf_phase_seconds(double out[5])
+{
+    for (int k = 0; k < 5; ++k)
+        out[k] = g_pf_phase[k];
+}
+
 /* Sampler<PFState>(N, scheme, threshold) with init PFInit and move PFMove
  * (user_guide/cpp/pf_seq.cpp:70-122, vSMC NormalDistribution), monitor
  * "pos" (dim 2, MonitorMCMC), driven as initialize(obs) + iterate(T - 1)
@@ -1030,6 +1042,15 @@ double vo_pf_run(const vo_pf_config *cfg, const double *obs_x,
 #ifdef _OPENMP
     const int nthreads = cfg->threads > 0 ? cfg->threads : omp_get_max_threads();
 #endif
+    for (int k = 0; k < 5; ++k)
+        g_pf_phase[k] = 0;
+    double tp = now_seconds();
+#define VO_PHASE(k)                            \
+    do {                                       \
+        const double tn_ = now_seconds();      \
+        g_pf_phase[k] += tn_ - tp;             \
+        tp = tn_;                              \
+    } while (0)
     const double t0 = now_seconds();
     /* do_reset: weight().set_equal() */
     double ess = vo_weight_set_equal(N, W);
@@ -1053,9 +1074,11 @@ double vo_pf_run(const vo_pf_config *cfg, const double *obs_x,
                 xi[3] = vo_normal_philox(&nv, &eng[i]);
                 w[i] = pf_loglik(cfg, xi[0], xi[1], obs_x[0], obs_y[0]);
             }
+            VO_PHASE(0);
             double dz;
             ess = vo_weight_set_log(N, W, w, &dz);
             logz = dz;
+            VO_PHASE(1);
         } else {
             /* PFMove::eval_sp: x += n + dt * vx; y += n + dt * vy;
              * vx += n; vy += n (velocity read before its update) */
@@ -1074,9 +1097,11 @@ double vo_pf_run(const vo_pf_config *cfg, const double *obs_x,
                 xi[3] += vo_normal_philox(&nv, &eng[i]);
                 w[i] = pf_loglik(cfg, xi[0], xi[1], obs_x[t], obs_y[t]);
             }
+            VO_PHASE(0);
             double dz;
             ess = vo_weight_add_log(N, W, w, &dz);
             logz += dz;
+            VO_PHASE(1);
         }
         /* do_resample: record ESS before resampling, then
          * Particle::resample (particle.hpp:170-197) */
@@ -1086,8 +1111,11 @@ double vo_pf_run(const vo_pf_config *cfg, const double *obs_x,
         if (resampled) {
             vo_resample(cfg->scheme, N, N, &rng, W, rep);
             vo_trans_rep_index(N, N, rep, idx);
+            VO_PHASE(2);
             vo_copy_rowmajor(N, 4, x, idx);
+            VO_PHASE(3);
             ess = vo_weight_set_equal(N, W);
+            VO_PHASE(1);
         }
         if (resampled_out)
             resampled_out[t] = resampled;
@@ -1098,10 +1126,12 @@ double vo_pf_run(const vo_pf_config *cfg, const double *obs_x,
                 buf[2 * i + 1] = x[4 * i + 1];
             }
             vo_monitor_moments(N, 2, buf, W, pos_out + 2 * t);
+            VO_PHASE(4);
         }
         if (logz_out)
             logz_out[t] = logz;
     }
+#undef VO_PHASE
     const double elapsed = now_seconds() - t0;
     if (state_out)
         memcpy(state_out, x, sizeof(double) * 4 * N);

@@ -214,6 +214,9 @@ typedef struct {
  * (monitor "pos" at MonitorMCMC), logz[T] (cumulative log-evidence),
  * state_out[4N] (AoS x,y,vx,vy), W_out[N]. Returns the seconds spent in
  * the timed region. */
+/* Per-phase seconds of the last vo_pf_run: move, weights, resampling
+ * (counts + rep -> index), copy, monitor. */
+void vo_pf_phase_seconds(double out[5]);
 double vo_pf_run(const vo_pf_config *cfg, const double *obs_x,
     const double *obs_y, double *ess, int64_t *resampled, double *pos,
     double *logz, double *state_out, double *W_out);

@@ -310,8 +310,15 @@ def pf_run(N, T, seed, scheme, threshold, obs_x, obs_y, obs_model=0, sd_pos=0.02
     secs = oracle().vo_pf_run(c.byref(cfg), np.ascontiguousarray(obs_x, np.float64),
                               np.ascontiguousarray(obs_y, np.float64), vp(ess), vp(rs),
                               vp(pos), vp(logz), vp(state), vp(W))
+    ph = np.zeros(5)
+    L = oracle()
+    L.vo_pf_phase_seconds.restype = None
+    L.vo_pf_phase_seconds.argtypes = [P(np.float64)]
+    L.vo_pf_phase_seconds(ph)
     out = {"ess": ess, "resampled": rs, "pos": pos.reshape(T, 2), "logz": logz,
-           "seconds": secs}
+           "seconds": secs,
+           "phase_seconds": dict(zip(("move", "weights", "resample", "copy", "monitor"),
+                                     ph.tolist()))}
     if want_state:
         out["state"] = state.reshape(N, 4)
         out["W"] = W
