@@ -89,12 +89,14 @@ def test_move_vs_oracle(dev, kind):
     assert agree >= n - 3, agree
 
 
-def test_first_increment(dev):
+@pytest.mark.parametrize("n", [1 << 14, (1 << 20) + 77])
+def test_first_increment(dev, n):
     """From equal weights at alpha = 0, the device's increment equals the
-    oracle's conditional-ESS solution on the same log-likelihoods."""
+    oracle's conditional-ESS solution on the same log-likelihoods (at 2^20 + 77
+    the search's 512 blocks loop over several particles per thread, with a
+    ragged end)."""
     import vsmc_amd as va
     x = _data()
-    n = 1 << 14
     s = va.GmmSampler(n, x, ncomp=4, seed=9, cess=0.9)
     s.run(max_steps=1)
     ll = s.state().cpu().numpy()[13].astype(np.float64)

@@ -166,6 +166,41 @@ __device__ inline T wave_reduce(T v, Op op)
     return shfl_any(v, 0);
 }
 
+// Wave sums of V per-lane values (V a power of two, at most kWave) with
+// V shuffles instead of V log2(kWave): each xor step over lane bit m halves
+// the values a lane carries (the lower half of the lanes keeps the first
+// half of its values, the upper half the second, each adding its partner's
+// copy), then the remaining lane bits are summed plainly. Afterwards lane l
+// holds the wave's total of value (l / (kWave / V)). The summation order is
+// fixed by the lane layout, so results are deterministic.
+template <int H, int V>
+__device__ inline void wave_transpose_step(double (&v)[V], int lane)
+{
+    if constexpr (H >= 1) {
+        constexpr int m = kWave / V * H;   // lane bit of this step
+        const bool upper = (lane & m) != 0;
+#pragma unroll
+        for (int j = 0; j < H; ++j) {
+            const double send = upper ? v[j] : v[j + H];
+            const double keep = upper ? v[j + H] : v[j];
+            v[j] = keep + __shfl_xor(send, m, kWave);
+        }
+        wave_transpose_step<H / 2>(v, lane);
+    }
+}
+
+template <int V>
+__device__ inline double wave_transpose_sum(double (&v)[V])
+{
+    static_assert(V >= 1 && V <= kWave && (V & (V - 1)) == 0, "V");
+    wave_transpose_step<V / 2>(v, static_cast<int>(threadIdx.x % kWave));
+    double r = v[0];
+#pragma unroll
+    for (int m = kWave / V / 2; m >= 1; m >>= 1)
+        r += __shfl_xor(r, m, kWave);
+    return r;
+}
+
 // Block-wide reduction, result broadcast to all threads.
 template <typename T, typename Op>
 __device__ inline T block_reduce(T v, Op op, T *lds)

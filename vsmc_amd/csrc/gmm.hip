@@ -21,6 +21,7 @@
 #include "../../include/vsmch/device_rng.hpp"
 
 #include <math.h>
+#include <type_traits>
 #include <vector>
 
 namespace vsmch {
@@ -74,37 +75,50 @@ __device__ inline float gmm_log_prior(int c, const float *mu, const float *lam,
 }
 
 // sum_k log(sum_i w_i exp(log(lambda_i) / 2 - lambda_i (x_k - mu_i)^2 / 2)
-// + 1e-13) - n log(2 pi) / 2, the data in LDS.
-__device__ inline float gmm_log_lik(int c, const float *mu, const float *lam,
+// + 1e-13) - n log(2 pi) / 2, the data in LDS. The component count is a
+// compile-time constant so that exactly C exponentials are issued per point.
+// Each exponent, in log2 units, is a_i - r^2 with r = s_i x + m_i,
+// s_i = sqrt(lambda_i log2(e) / 2), m_i = -mu_i s_i: two FMAs, then the
+// hardware exp2. The logarithm is taken of the product of two consecutive
+// points' mixture densities (each at least 1e-13 and at most
+// max_i sqrt(lambda_i), so the product stays in the normal FP32 range) and
+// accumulated in log2.
+template <int C>
+__device__ inline float gmm_mix(float x, const float *a, const float *sc,
+    const float *m)
+{
+    float s = 1e-13f;
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+        const float r = fmaf(x, sc[i], m[i]);
+        s += __builtin_amdgcn_exp2f(fmaf(-r, r, a[i]));
+    }
+    return s;
+}
+
+template <int C>
+__device__ inline float gmm_log_lik(const float *mu, const float *lam,
     const float *w, const float *xs, int n)
 {
-    float a[kGmmMaxC], h[kGmmMaxC];
-    // exp(v) = exp2(v log2 e): fold log2 e into the coefficients
+    float a[C], sc[C], m[C];
     const float l2e = 1.4426950408889634f;
 #pragma unroll
-    for (int i = 0; i < kGmmMaxC; ++i) {
-        if (i < c) {
-            a[i] = (__logf(w[i]) + 0.5f * __logf(lam[i])) * l2e;
-            h[i] = -0.5f * lam[i] * l2e;
-        } else {
-            a[i] = -INFINITY;
-            h[i] = 0.0f;
-        }
+    for (int i = 0; i < C; ++i) {
+        a[i] = (__logf(w[i]) + 0.5f * __logf(lam[i])) * l2e;
+        sc[i] = sqrtf(0.5f * l2e * lam[i]);
+        m[i] = -mu[i] * sc[i];
     }
-    float ll = 0.0f;
-    for (int k = 0; k < n; ++k) {
-        const float x = xs[k];
-        float s = 0.0f;
-#pragma unroll
-        for (int i = 0; i < kGmmMaxC; ++i) {
-            if (i < c) {
-                const float r = x - mu[i];
-                s += exp2f(a[i] + h[i] * r * r);
-            }
-        }
-        ll += __logf(s + 1e-13f);
+    float ll2 = 0.0f;
+    int k = 0;
+    for (; k + 1 < n; k += 2) {
+        const float s0 = gmm_mix<C>(xs[k], a, sc, m);
+        const float s1 = gmm_mix<C>(xs[k + 1], a, sc, m);
+        ll2 += __builtin_amdgcn_logf(s0 * s1);
     }
-    return ll - 0.91893853320467274178f * static_cast<float>(n);
+    if (k < n)
+        ll2 += __builtin_amdgcn_logf(gmm_mix<C>(xs[k], a, sc, m));
+    return ll2 * 0.69314718055994531f -
+        0.91893853320467274178f * static_cast<float>(n);
 }
 
 __device__ inline float stream_normal(ParticleStream &s, float &cache,
@@ -122,8 +136,11 @@ __device__ inline float stream_normal(ParticleStream &s, float &cache,
     return p.z0;
 }
 
-// Gamma(shape >= 1, scale) by Marsaglia and Tsang (2000); at most 32
-// attempts (acceptance above 95% each), then the mean.
+// Gamma(shape >= 1, scale) by Marsaglia and Tsang (2000). Each attempt is
+// accepted with probability at least 0.95 for shape >= 1 (0.98 at the prior's
+// shape 2), so 32 straight rejections have probability below 0.05^32 = 2e-42
+// per draw; the bound is there so that a NaN or infinite shape cannot spin,
+// and its fallback (the mean) is never reached by a finite shape in practice.
 __device__ inline float stream_gamma(ParticleStream &s, float shape,
     float scale, float &cache, bool &have)
 {
@@ -152,9 +169,11 @@ __device__ inline void load_data(float *xs, const float *data, int n)
 // Initialisation (gmm_init::eval_sp, gmm.hpp:364-387): mu ~ N(mu0, sd0),
 // lambda ~ Gamma(shape0, scale0), weights Gamma(1, 1) normalised; words
 // [0, 64 (3c + 1)) of particle i's stream.
+template <int C>
 __global__ void __launch_bounds__(kGB) k_gmm_init(float *state, long long n,
-    int c, uint32_t seed, const GmmDev *gp, const float *data, int ndata)
+    uint32_t seed, const GmmDev *gp, const float *data, int ndata)
 {
+    constexpr int c = C;
     extern __shared__ float xs[];
     load_data(xs, data, ndata);
     const long long i = static_cast<long long>(blockIdx.x) * kGB + threadIdx.x;
@@ -164,7 +183,7 @@ __global__ void __launch_bounds__(kGB) k_gmm_init(float *state, long long n,
     ParticleStream s(seed_key(seed, static_cast<unsigned long long>(i)), 0);
     float cache = 0.0f;
     bool have = false;
-    float mu[kGmmMaxC], lam[kGmmMaxC], w[kGmmMaxC];
+    float mu[C], lam[C], w[C];
     float sum = 0.0f;
     for (int k = 0; k < c; ++k) {
         mu[k] = static_cast<float>(g.mu0) + static_cast<float>(g.sd0) *
@@ -182,7 +201,7 @@ __global__ void __launch_bounds__(kGB) k_gmm_init(float *state, long long n,
         state[(2 * c + k) * n + i] = w[k];
     }
     state[3 * c * n + i] = gmm_log_prior(c, mu, lam, g);
-    state[(3 * c + 1) * n + i] = gmm_log_lik(c, mu, lam, w, xs, ndata);
+    state[(3 * c + 1) * n + i] = gmm_log_lik<C>(mu, lam, w, xs, ndata);
 }
 
 __device__ inline float logit_side(int c, const float *w)
@@ -199,10 +218,12 @@ __device__ inline float logit_side(int c, const float *w)
 // One Metropolis-Hastings move of `kind` (0 mu, 1 lambda, 2 weight) at
 // temperature alpha (gmm.hpp:447-543). Draws: c normals then one uniform
 // from word 64 (32 t + m + 1) of particle i's stream.
+template <int C>
 __global__ void __launch_bounds__(kGB) k_gmm_move(float *state, long long n,
-    int c, int kind, uint32_t seed, long long t, int m, GmmDev *gp,
+    int kind, uint32_t seed, long long t, int m, GmmDev *gp,
     const float *data, int ndata)
 {
+    constexpr int c = C;
     extern __shared__ float xs[];
     load_data(xs, data, ndata);
     __shared__ long long acc_lds[kGB / 64];
@@ -216,7 +237,7 @@ __global__ void __launch_bounds__(kGB) k_gmm_move(float *state, long long n,
         const float sd_mu = static_cast<float>(0.15 / a);
         const float sd_lam = static_cast<float>((1 + sqrt(1 / a)) * 0.15);
         const float sd_w = static_cast<float>((1 + sqrt(1 / a)) * 0.2);
-        float mu[kGmmMaxC], lam[kGmmMaxC], w[kGmmMaxC];
+        float mu[C], lam[C], w[C];
         for (int k = 0; k < c; ++k) {
             mu[k] = state[k * n + i];
             lam[k] = state[(c + k) * n + i];
@@ -228,12 +249,12 @@ __global__ void __launch_bounds__(kGB) k_gmm_move(float *state, long long n,
             64ULL * static_cast<unsigned long long>(32 * t + m + 1));
         float cache = 0.0f;
         bool have = false;
-        float z[kGmmMaxC];
+        float z[C];
         for (int k = 0; k < c; ++k)
             z[k] = stream_normal(s, cache, have);
         const float log_u = logf(u01_u32f(s.next()));
         float jac = 0.0f;
-        float w2[kGmmMaxC], mu2[kGmmMaxC], lam2[kGmmMaxC];
+        float w2[C], mu2[C], lam2[C];
         for (int k = 0; k < c; ++k) {
             mu2[k] = mu[k];
             lam2[k] = lam[k];
@@ -259,7 +280,7 @@ __global__ void __launch_bounds__(kGB) k_gmm_move(float *state, long long n,
             jac = logit_side(c, w2) - logit_side(c, w);
         }
         const float lp1 = gmm_log_prior(c, mu2, lam2, g);
-        const float ll1 = gmm_log_lik(c, mu2, lam2, w2, xs, ndata);
+        const float ll1 = gmm_log_lik<C>(mu2, lam2, w2, xs, ndata);
         const float pr = jac + lp1 + alpha * ll1 - (lp0 + alpha * ll0);
         // The reference rejects when pr < log u (mh_reject_common). A float
         // uniform of exactly 0 (probability 2^-24) makes an infinite normal
@@ -322,18 +343,22 @@ __device__ inline double cand(const GmmDev &g, int k)
     return g.lo + (g.hi - g.lo) * static_cast<double>(k + 1) / kCand;
 }
 
-// Per-block sums S1_k = sum W_i e^{d_k (l_i - lmax)}, S2_k = sum W_i
-// e^{2 d_k (l_i - lmax)} for the kCand candidates d_k of the bracket.
+// Per-block sums S1_k = sum W_i e^{d_k x_i}, S2_k = sum W_i e^{2 d_k x_i},
+// x_i = l_i - lmax <= 0, for the kCand equally spaced candidates
+// d_k = lo + (k + 1) h of the bracket, h = (hi - lo) / kCand. Two FP64
+// exponentials per particle: e^{d_0 x}, then k steps of the ratio e^{h x}
+// (relative error of a term about 1e-13 at worst, far below the bracket's
+// resolution). Written as part[block][0 .. kCand) = S1, [kCand, 2 kCand) = S2.
 __global__ void __launch_bounds__(kGB) k_gmm_search_pass(const float *ll,
     const double *logw, const PDev *pd, long long n, const GmmDev *gp,
     double *part)
 {
-    __shared__ double lds[kGB / 64];
+    __shared__ double lds[kGB / kWave][2 * kCand];
     const GmmDev g = *gp;
-    double d[kCand];
-    for (int k = 0; k < kCand; ++k)
-        d[k] = cand(g, k);
+    const double h = (g.hi - g.lo) / kCand;
+    const double d0 = cand(g, 0);
     double s1[kCand], s2[kCand];
+#pragma unroll
     for (int k = 0; k < kCand; ++k)
         s1[k] = s2[k] = 0.0;
     const int equal = pd->equal;
@@ -344,49 +369,83 @@ __global__ void __launch_bounds__(kGB) k_gmm_search_pass(const float *ll,
         // normalised weights: the conditional ESS is a fraction of 1
         const double W = equal ? w_equal : exp(logw[i] - c);
         const double x = static_cast<double>(ll[i]) - g.llmax;
+        const double r = exp(h * x);
+        double e = exp(d0 * x);
+#pragma unroll
         for (int k = 0; k < kCand; ++k) {
-            const double e = exp(d[k] * x);
-            s1[k] += W * e;
-            s2[k] += W * e * e;
+            const double we = W * e;
+            s1[k] += we;
+            s2[k] = fma(we, e, s2[k]);
+            e *= r;
         }
     }
+    // s1 and s2 as one array of 2 kCand values; lane l then holds the
+    // wave's total of value l / 2
+    double v[2 * kCand];
+#pragma unroll
     for (int k = 0; k < kCand; ++k) {
-        const double a = block_reduce(s1[k], OpAddD(), lds);
-        const double b = block_reduce(s2[k], OpAddD(), lds);
-        if (threadIdx.x == 0) {
-            part[(2 * blockIdx.x) * kCand + k] = a;
-            part[(2 * blockIdx.x + 1) * kCand + k] = b;
-        }
+        v[k] = s1[k];
+        v[kCand + k] = s2[k];
+    }
+    const double tot = wave_transpose_sum<2 * kCand>(v);
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    static_assert(kWave == 4 * kCand, "two lanes per value");
+    if ((lane & 1) == 0)
+        lds[wave][lane >> 1] = tot;
+    __syncthreads();
+    if (threadIdx.x < 2 * kCand) {
+        double t = lds[0][threadIdx.x];
+        for (int w = 1; w < kGB / kWave; ++w)
+            t += lds[w][threadIdx.x];
+        part[blockIdx.x * (2 * kCand) + threadIdx.x] = t;
     }
 }
 
-// Narrows the bracket to [d_j, d_{j+1}] around the last candidate j whose
-// conditional ESS reaches the target (d_{-1} = lo); final: dalpha = lo,
-// or hi when no candidate reached it (a step smaller than the resolution).
+// Sums the blocks' partials (in a fixed order: deterministic), then narrows
+// the bracket to [d_j, d_{j+1}] around the last candidate j whose
+// conditional ESS S1^2 / S2 reaches the target (d_{-1} = lo); final:
+// dalpha = lo, or hi when no candidate reached it (a step smaller than the
+// resolution).
 __global__ void __launch_bounds__(kGB) k_gmm_search_merge(const double *part,
     long long nb, GmmDev *g, int final_round)
 {
-    __shared__ double lds[kGB / 64];
+    constexpr int kV = 2 * kCand;          // values per block
+    constexpr int kSlices = kGB / kV;      // threads per value
+    __shared__ double lds[kSlices][kV];
     __shared__ double cs[kCand];
-    for (int k = 0; k < kCand; ++k) {
-        double a = 0.0, b = 0.0;
-        for (long long j = threadIdx.x; j < nb; j += kGB) {
-            a += part[(2 * j) * kCand + k];
-            b += part[(2 * j + 1) * kCand + k];
-        }
-        a = block_reduce(a, OpAddD(), lds);
-        b = block_reduce(b, OpAddD(), lds);
-        if (threadIdx.x == 0)
-            cs[k] = a * a / b;
+    const int v = threadIdx.x % kV;
+    const int sl = threadIdx.x / kV;
+    // four independent chains so that the loads overlap
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    long long j = sl;
+    for (; j + 3 * kSlices < nb; j += 4 * kSlices) {
+        a0 += part[j * kV + v];
+        a1 += part[(j + kSlices) * kV + v];
+        a2 += part[(j + 2 * kSlices) * kV + v];
+        a3 += part[(j + 3 * kSlices) * kV + v];
     }
+    for (; j < nb; j += kSlices)
+        a0 += part[j * kV + v];
+    lds[sl][v] = (a0 + a1) + (a2 + a3);
+    __syncthreads();
+    if (threadIdx.x < kCand) {
+        double s1 = 0.0, s2 = 0.0;
+        for (int q = 0; q < kSlices; ++q) {
+            s1 += lds[q][threadIdx.x];
+            s2 += lds[q][kCand + threadIdx.x];
+        }
+        cs[threadIdx.x] = s1 * s1 / s2;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        const GmmDev h = *g;
+        const GmmDev hd = *g;
         int j = -1;
         for (int k = 0; k < kCand; ++k)
-            if (cs[k] >= h.target)
+            if (cs[k] >= hd.target)
                 j = k;
-        const double lo = j < 0 ? h.lo : cand(h, j);
-        const double hi = j + 1 < kCand ? cand(h, j + 1) : cand(h, kCand - 1);
+        const double lo = j < 0 ? hd.lo : cand(hd, j);
+        const double hi = j + 1 < kCand ? cand(hd, j + 1) : cand(hd, kCand - 1);
         g->lo = lo;
         g->hi = j == kCand - 1 ? lo : hi;
         if (final_round)
@@ -467,9 +526,10 @@ int gmm_create(GmmSet **out, long long n, int c, uint32_t seed, double cess,
         delete s;
         return rc;
     }
+    // search blocks: two per CU, each thread looping over particles
     s->nb = (n + kGB - 1) / kGB;
-    if (s->nb > 4096)
-        s->nb = 4096;
+    if (s->nb > 512)
+        s->nb = 512;
     std::vector<float> xf(ndata);
     for (long long k = 0; k < ndata; ++k)
         xf[k] = static_cast<float>(data[k]);
@@ -502,16 +562,38 @@ int gmm_create(GmmSet **out, long long n, int c, uint32_t seed, double cess,
 
 ParticleSet *gmm_particles(GmmSet *s) { return s->p; }
 
+// Calls f(std::integral_constant<int, c>) for the component counts the
+// kernels are instantiated for (2 .. kGmmMaxC, checked by gmm_create).
+template <typename F>
+int with_components(int c, F &&f)
+{
+    switch (c) {
+    case 2: return f(std::integral_constant<int, 2>());
+    case 3: return f(std::integral_constant<int, 3>());
+    case 4: return f(std::integral_constant<int, 4>());
+    case 5: return f(std::integral_constant<int, 5>());
+    case 6: return f(std::integral_constant<int, 6>());
+    case 7: return f(std::integral_constant<int, 7>());
+    case 8: return f(std::integral_constant<int, 8>());
+    default:
+        set_error("gmm: component count outside 2 .. 8");
+        return kErrInvalid;
+    }
+}
+
 // One move of `kind` over all particles at the current alpha (tests).
 int gmm_move(GmmSet *s, int kind, long long t, int m)
 {
     ParticleSet *p = s->p;
     const unsigned g = static_cast<unsigned>((p->n + kGB - 1) / kGB);
-    hipLaunchKernelGGL(k_gmm_move, dim3(g), dim3(kGB), data_bytes(s),
-        p->ws.stream, static_cast<float *>(p->state), p->n, s->c, kind,
-        p->seed, t, m, s->g, static_cast<const float *>(s->data), s->ndata);
-    VSMCH_HIP(hipGetLastError());
-    return kOk;
+    return with_components(s->c, [&](auto cc) {
+        constexpr int C = decltype(cc)::value;
+        hipLaunchKernelGGL(k_gmm_move<C>, dim3(g), dim3(kGB), data_bytes(s),
+            p->ws.stream, static_cast<float *>(p->state), p->n, kind, p->seed,
+            t, m, s->g, static_cast<const float *>(s->data), s->ndata);
+        VSMCH_HIP(hipGetLastError());
+        return kOk;
+    });
 }
 
 int gmm_set_alpha(GmmSet *s, double alpha)
@@ -543,11 +625,17 @@ int gmm_run(GmmSet *s, long long max_steps, double *alpha_d, double *ess_d,
     g0.alpha = 0;
     g0.dalpha = 0;
     VSMCH_HIP(hipMemcpy(s->g, &g0, sizeof(g0), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_gmm_init, dim3(gp), dim3(kGB), data_bytes(s), st,
-        state, n, c, p->seed, static_cast<const GmmDev *>(s->g),
-        static_cast<const float *>(s->data), s->ndata);
-    VSMCH_HIP(hipGetLastError());
-    int rc = weight_update(p, nullptr, 2);   // set_equal
+    int rc = with_components(c, [&](auto cc) {
+        constexpr int C = decltype(cc)::value;
+        hipLaunchKernelGGL(k_gmm_init<C>, dim3(gp), dim3(kGB), data_bytes(s),
+            st, state, n, p->seed, static_cast<const GmmDev *>(s->g),
+            static_cast<const float *>(s->data), s->ndata);
+        VSMCH_HIP(hipGetLastError());
+        return kOk;
+    });
+    if (rc != kOk)
+        return rc;
+    rc = weight_update(p, nullptr, 2);   // set_equal
     if (rc != kOk)
         return rc;
     // step 0 records
