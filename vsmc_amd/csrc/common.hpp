@@ -201,6 +201,17 @@ __device__ inline double wave_transpose_sum(double (&v)[V])
     return r;
 }
 
+// Pins a loaded value in registers at this point, so that the compiler
+// cannot move its load into the branch that uses it: a load under a branch
+// is waited for before the branch ends, which serialises a thread's loads.
+// Issue a batch of loads, then pin each one. (An empty asm statement: no
+// instruction is emitted.)
+template <typename T>
+__device__ __forceinline__ void keep_loaded(T &v)
+{
+    asm volatile("" : "+v"(v));
+}
+
 // Block-wide reduction, result broadcast to all threads.
 template <typename T, typename Op>
 __device__ inline T block_reduce(T v, Op op, T *lds)

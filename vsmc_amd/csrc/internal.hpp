@@ -75,6 +75,7 @@ struct Workspace {
     int *zdst = nullptr;         // childless slots in dst order
     int *cE = nullptr;           // extra-copy prefix of sources with extras
     int *csrc = nullptr;         // those sources
+    int *tq = nullptr;           // per tile: slice [tq[2t], tq[2t+1]) of csrc its childless slots take from
     double *v = nullptr;         // materialised uniforms (running max)
     double *aux = nullptr;       // scratch N doubles
     int *rep = nullptr;          // scratch counts

@@ -16,6 +16,7 @@
 #include "particle.hpp"
 #include "../../include/vsmch/device_rng.hpp"
 
+#include <climits>
 #include <mutex>
 #include <type_traits>
 #include <vector>
@@ -979,14 +980,23 @@ __device__ __forceinline__ double count_tile(WSrc src, USrc us, long long M,
     __syncthreads();
     const auto sv = src.loaded();
     long long z = 0, e = 0, c = 0;
-#pragma unroll 4
+    // the integer parts (residual schemes; zero otherwise) from clamped,
+    // unconditional loads, all in flight before their first use
+    long long ig[kItems];
+#pragma unroll
     for (int j = 0; j < kItems; ++j) {
         const int k = j * kBlock + static_cast<int>(threadIdx.x);
+        ig[j] = sv.integ(base + (k < nvalid ? k : nvalid - 1));
+    }
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const int k = j * kBlock + static_cast<int>(threadIdx.x);
+        keep_loaded(ig[j]);
         if (k < nvalid) {
             const long long g = __builtin_bit_cast(long long, tile[lds_pad(k)]);
             const long long gp = (k == 0) ? gfirst :
                 __builtin_bit_cast(long long, tile[lds_pad(k - 1)]);
-            const long long r = g - gp + sv.integ(base + k);
+            const long long r = g - gp + ig[j];
             rep[base + k] = static_cast<int>(r);
             z += r == 0;
             e += r > 1 ? r - 1 : 0;
@@ -1137,11 +1147,20 @@ __global__ void __launch_bounds__(kBlock) k_ridx_agg(const int *rep,
     const long long base = static_cast<long long>(blockIdx.x) * kTile;
     __shared__ long long lds[kWaves];
     long long z = 0, e = 0, c = 0;
+    // all loads first (clamped), then the counts: a load under a branch
+    // would wait for itself
+    int rv[kItems];
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
         const long long g = base + j * kBlock + threadIdx.x;
+        rv[j] = rep[g < M ? g : M - 1];
+    }
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        keep_loaded(rv[j]);
+        const long long g = base + j * kBlock + threadIdx.x;
         if (g < M) {
-            const long long r = rep[g];
+            const long long r = rv[j];
             if (kExpand) {
                 e += r;
                 c += r > 0;
@@ -1164,6 +1183,15 @@ __global__ void __launch_bounds__(kBlock) k_ridx_agg(const int *rep,
 
 struct L3 {
     long long a, b, c;
+};
+struct I3 {
+    int a, b, c;
+};
+struct OpAddI3 {
+    __device__ I3 operator()(I3 x, I3 y) const
+    {
+        return I3{x.a + y.a, x.b + y.b, x.c + y.c};
+    }
 };
 struct OpAddL3 {
     __device__ L3 operator()(L3 x, L3 y) const
@@ -1237,10 +1265,19 @@ __device__ __forceinline__ void scatter_tile(const int *rep, long long M, long l
     __shared__ int tile[kTileLds];   // rep, element k at lds_pad(k)
     __shared__ int stage[kTile];     // one output list at a time
     const int nvalid = M - base < kTile ? static_cast<int>(M - base) : kTile;
+    // all loads first (clamped), then their uses: a load under a branch
+    // would wait for itself
+    int rl[kItems];
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
         const int k = j * kBlock + static_cast<int>(threadIdx.x);
-        const int rv = k < nvalid ? rep[base + k] : 0;
+        rl[j] = rep[base + (k < nvalid ? k : nvalid - 1)];
+    }
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const int k = j * kBlock + static_cast<int>(threadIdx.x);
+        keep_loaded(rl[j]);
+        const int rv = k < nvalid ? rl[j] : 0;
         tile[lds_pad(k)] = rv;
         if (!kExpand && idx && k < nvalid && rv > 0)
             idx[base + k] = static_cast<int>(base + k);
@@ -1436,24 +1473,32 @@ __device__ __forceinline__ void expand_chunk(long long chunk, long long K, long 
     }
     // all index loads of this thread's ranks first, then all state loads,
     // then the stores: the memory latencies overlap instead of adding up
+    // (loads from clamped ranks, unconditional: a guarded load would wait
+    // for itself)
+    // src[j] is a valid source for every j (an idle rank takes the last
+    // rank's); bit j of live marks the ranks below nk
     int src[kItems], dst[kItems];
+    unsigned live = 0;
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
-        const int kk = j * kBlock + static_cast<int>(threadIdx.x);
-        src[j] = -1;
-        dst[j] = 0;
-        if (kk < nk) {
-            const long long k = k0 + kk;
-            long long c = marked ? clo + mark[lds_pad(kk)] : upper_bound_i(cE, C, k) - 1;
-            c = c < 0 ? 0 : (c >= C ? C - 1 : c);
-            src[j] = csrc[c];
-            dst[j] = kExpand ? static_cast<int>(k) : zdst[k];
-        }
+        const int kr = j * kBlock + static_cast<int>(threadIdx.x);
+        const int kk = kr < nk ? kr : nk - 1;
+        const long long k = k0 + kk;
+        long long c = marked ? clo + mark[lds_pad(kk)] : upper_bound_i(cE, C, k) - 1;
+        c = c < 0 ? 0 : (c >= C ? C - 1 : c);
+        src[j] = csrc[c];
+        dst[j] = kExpand ? static_cast<int>(k) : zdst[k];
+        live |= kr < nk ? 1u << j : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        keep_loaded(src[j]);
+        keep_loaded(dst[j]);
     }
     if (idx) {
 #pragma unroll
         for (int j = 0; j < kItems; ++j)
-            if (src[j] >= 0)
+            if ((live >> j) & 1u)
                 idx[dst[j]] = src[j];
     }
     if (state) {
@@ -1462,17 +1507,439 @@ __device__ __forceinline__ void expand_chunk(long long chunk, long long K, long 
         for (int d = 0; d < dim; ++d) {
             E *const col = state + d * ld;
             E v[kItems];
+            // unconditional loads (every src[j] is valid)
 #pragma unroll
             for (int j = 0; j < kItems; ++j)
-                if (src[j] >= 0)
-                    v[j] = col[static_cast<unsigned>(src[j])];
+                v[j] = col[static_cast<unsigned>(src[j])];
 #pragma unroll
             for (int j = 0; j < kItems; ++j)
-                if (src[j] >= 0)
+                keep_loaded(v[j]);
+#pragma unroll
+            for (int j = 0; j < kItems; ++j)
+                if ((live >> j) & 1u)
                     col[static_cast<unsigned>(dst[j])] = v[j];
         }
     }
     __syncthreads();
+}
+
+// Tiles whose childless, extras and sources prefixes k_ridx_prefix holds in
+// LDS (three int arrays, 60 KB) to find every tile's slice of sources there.
+constexpr int kPrefixLdsTiles = 5120;
+
+// Last index i in [0, n) with a[i] <= x for a non-decreasing a in LDS, or
+// -1 (one thread).
+__device__ inline int lds_last_le(const int *a, int n, long long x)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] <= x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo - 1;
+}
+
+// The M == N rep -> index prefixes, one block: the exclusive per-tile
+// offsets of (childless, extras, sources) as k_ridx_scan (toff, totals at
+// [3 ntiles ..]) and, when the tiles fit kPrefixLdsTiles, for every tile t
+// with childless slots the slice [tq[2t], tq[2t + 1]) of the compacted
+// sources they take from: from the first source of the tile whose extras
+// hold rank Z_t to the last source of the tile whose extras hold rank
+// Z_{t+1} - 1, searched in LDS. Per-tile counts and their prefixes stay
+// below 2^31 (M < 2^31).
+__global__ void __launch_bounds__(kScanThreads) k_ridx_prefix(const long long *tagg,
+    long long ntiles, const ResampleDev *prm, long long *toff, int *tq)
+{
+    if (prm && !prm->active)
+        return;
+    constexpr int kPer = 4;   // tiles per thread and chunk
+    __shared__ I3 red[kScanThreads / kWave];
+    __shared__ int zp[kPrefixLdsTiles + 1];   // childless prefix
+    __shared__ int ep[kPrefixLdsTiles + 1];   // extras prefix
+    __shared__ int cp[kPrefixLdsTiles + 1];   // sources prefix
+    const bool in_lds = ntiles <= kPrefixLdsTiles;
+    I3 carry{0, 0, 0};
+    for (long long c0 = 0; c0 < ntiles; c0 += static_cast<long long>(kScanThreads) * kPer) {
+        const long long b0 = c0 + static_cast<long long>(threadIdx.x) * kPer;
+        // clamped, unconditional loads, all in flight before their first use
+        long long raw[3 * kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const long long t = b0 + k < ntiles ? b0 + k : ntiles - 1;
+            raw[3 * k + 0] = tagg[3 * t + 0];
+            raw[3 * k + 1] = tagg[3 * t + 1];
+            raw[3 * k + 2] = tagg[3 * t + 2];
+        }
+        I3 v[kPer];
+        I3 loc{0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            keep_loaded(raw[3 * k + 0]);
+            keep_loaded(raw[3 * k + 1]);
+            keep_loaded(raw[3 * k + 2]);
+            const bool ok = b0 + k < ntiles;
+            v[k] = ok ? I3{static_cast<int>(raw[3 * k]), static_cast<int>(raw[3 * k + 1]),
+                static_cast<int>(raw[3 * k + 2])} : I3{0, 0, 0};
+            loc = OpAddI3()(loc, v[k]);
+        }
+        I3 tot;
+        I3 run = OpAddI3()(carry,
+            block_exclusive_scan_n<kScanThreads>(loc, OpAddI3(), I3{0, 0, 0}, red, &tot));
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const long long t = b0 + k;
+            if (t < ntiles) {
+                toff[3 * t + 0] = run.a;
+                toff[3 * t + 1] = run.b;
+                toff[3 * t + 2] = run.c;
+                if (in_lds) {
+                    zp[t] = run.a;
+                    ep[t] = run.b;
+                    cp[t] = run.c;
+                }
+            }
+            run = OpAddI3()(run, v[k]);
+        }
+        carry = OpAddI3()(carry, tot);
+    }
+    if (threadIdx.x == 0) {
+        toff[3 * ntiles + 0] = carry.a;
+        toff[3 * ntiles + 1] = carry.b;
+        toff[3 * ntiles + 2] = carry.c;
+        if (in_lds) {
+            zp[ntiles] = carry.a;
+            ep[ntiles] = carry.b;
+            cp[ntiles] = carry.c;
+        }
+    }
+    if (!in_lds)
+        return;
+    __syncthreads();   // zp, ep and cp complete for the whole block
+    const int nt = static_cast<int>(ntiles);
+    for (int t = threadIdx.x; t < nt; t += kScanThreads) {
+        const long long Z = zp[t];
+        const long long Z1 = zp[t + 1];
+        const bool anyz = Z1 > Z;
+        int slo = lds_last_le(ep, nt, Z);
+        int shi = lds_last_le(ep, nt, anyz ? Z1 - 1 : Z);
+        slo = slo < 0 ? 0 : slo;
+        shi = shi < 0 ? 0 : shi;
+        tq[2 * t] = anyz ? cp[slo] : 0;
+        tq[2 * t + 1] = anyz ? cp[shi + 1] : 0;
+    }
+}
+
+// The sources with extra copies of tile tidx, in order (csrc), and the
+// exclusive prefix of their extra copies (cE), at the tile's offsets in
+// toff (M == N). The offsets load with the tile's rep; one block scan of
+// the packed per-thread (extras << 16 | sources).
+__global__ void __launch_bounds__(kBlock) k_ridx_sources(const int *rep,
+    long long M, const ResampleDev *prm, const long long *toff, int *cE,
+    int *csrc)
+{
+    if (prm && !prm->active)
+        return;
+    const long long tidx = blockIdx.x;
+    const long long base = tidx * kTile;
+    const int nvalid = M - base < kTile ? static_cast<int>(M - base) : kTile;
+    __shared__ int lds[kTileLds];
+    __shared__ long long red[kWaves];
+    const long long e0 = toff[3 * tidx + 1];
+    const long long c0 = toff[3 * tidx + 2];
+    const int *const rept = rep + base;
+    {
+        // clamped, unconditional loads, all in flight before their first use
+        int v[kItems];
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const int k = j * kBlock + static_cast<int>(threadIdx.x);
+            v[j] = rept[static_cast<unsigned>(k < nvalid ? k : nvalid - 1)];
+        }
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const int k = j * kBlock + static_cast<int>(threadIdx.x);
+            keep_loaded(v[j]);
+            lds[lds_pad(k)] = k < nvalid ? v[j] : 0;
+        }
+    }
+    __syncthreads();
+    const int kb = static_cast<int>(threadIdx.x) * kItems;
+    int r[kItems];
+    long long ec = 0;   // extras << 16 | sources (at most kItems sources)
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        r[j] = lds[lds_pad(kb + j)];
+        ec += r[j] > 1 ? (static_cast<long long>(r[j] - 1) << 16) | 1 : 0;
+    }
+    long long tot;
+    const long long ex = block_exclusive_scan(ec, OpAddL(), 0LL, red, &tot);
+    // (the scan ends with a barrier: every rep has been read)
+    const int nsrc = static_cast<int>(tot & 0xffff);   // sources of the tile
+    // cE, staged in tile order, then written out coalesced
+    {
+        int cr = static_cast<int>(ex & 0xffff);
+        long long er = e0 + (ex >> 16);
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            if (r[j] > 1) {
+                lds[cr++] = static_cast<int>(er);
+                er += r[j] - 1;
+            }
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < nsrc; q += kBlock)
+            cE[c0 + q] = lds[q];
+        __syncthreads();
+    }
+    {
+        int cr = static_cast<int>(ex & 0xffff);
+#pragma unroll
+        for (int j = 0; j < kItems; ++j)
+            if (r[j] > 1)
+                lds[cr++] = static_cast<int>(base + kb + j);
+        __syncthreads();
+        for (int q = threadIdx.x; q < nsrc; q += kBlock)
+            csrc[c0 + q] = lds[q];
+    }
+}
+
+// Last index i in [0, n) with at(i) <= x for a non-decreasing at(), or -1:
+// one whole wave, 64 probes a round (every lane returns the result; the loop
+// bounds are wave-uniform).
+template <typename At>
+__device__ inline long long wave_last_le(At at, long long n, long long x)
+{
+    const long long lane = threadIdx.x & (kWave - 1);
+    long long lo = 0, hi = n;   // the first index with at(i) > x lies in [lo, hi]
+    while (lo < hi) {
+        const long long step = (hi - lo + kWave - 1) / kWave;
+        const long long p = lo + lane * step;
+        const bool le = p < hi && at(p) <= x;
+        const long long c = __popcll(__ballot(le));
+        if (c == 0)
+            break;
+        const long long nhi = lo + c * step;
+        lo = lo + (c - 1) * step + 1;
+        hi = nhi < hi ? nhi : hi;
+    }
+    return lo - 1;
+}
+
+// Ancestor indices of tile tidx when M == N (the rep -> index layout of
+// transform.hpp:116-133): a particle with copies keeps its own index; the
+// tile's childless slots, ranks [Z, Z + z) among all childless slots in
+// order, take the sources whose extra copies cover those ranks (csrc, with
+// cE the exclusive prefix of their extra copies). Those sources lie in the
+// compacted slices of the source tiles whose extras cover the ranks: the
+// slice comes from tq (k_ridx_prefix) or, without it, from the per-tile
+// prefixes (toff) in two rounds of a wave search. The block scans that
+// slice coalesced, and the tile's indices are written in one coalesced pass.
+// With `state`, the childless slots' rows are gathered in place (sources
+// always have copies, so they are never written). Block 0 ends the event
+// when end.pd is set; nothing in this kernel reads what that writes.
+template <typename E>
+__global__ void __launch_bounds__(kBlock) k_ridx_fill(const ResampleDev *prm,
+    const int *rep, long long M, const long long *toff, const int *tq,
+    long long ntiles, const int *cE, const int *csrc, int *idx, E *state,
+    int dim, long long ld, ResampleDev *eprm, EventEnd end)
+{
+    if (prm && !prm->active)
+        return;
+    if (end.pd && blockIdx.x == 0 && threadIdx.x == 0)
+        after_event(end.pd, eprm, end.n, end.rep, end.scheme, end.words,
+            end.deferred);
+    const long long tidx = blockIdx.x;
+    const long long base = tidx * kTile;
+    const int nvalid = M - base < kTile ? static_cast<int>(M - base) : kTile;
+    // one array, reused: rep, then per childless rank the slot of its source
+    // in csrc, then the output index
+    __shared__ int lds[kTileLds];
+    __shared__ int ired[kWaves];
+    __shared__ long long qrange[2];
+    __shared__ int src0s;   // the source of the tile's first childless rank
+    // the tile's childless ranks [Z, Z + zt) and the sources' count C come
+    // from the per-tile prefixes, so the searches for the source tiles start
+    // at once, with the tile's rep loads in flight
+    const long long Z = toff[3 * tidx + 0];
+    const long long zt = toff[3 * (tidx + 1) + 0] - Z;
+    const long long C = toff[3 * ntiles + 2];
+    const bool any = zt > 0 && C > 0;
+    // uniform bases and 32-bit offsets: one VGPR per address
+    const int *const rept = rep + base;
+    // clamped, unconditional loads, all in flight before their first use
+    // (a guarded load waits for itself); padding counts as kept
+    {
+        int v[kItems];
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const int k = j * kBlock + static_cast<int>(threadIdx.x);
+            v[j] = rept[static_cast<unsigned>(k < nvalid ? k : nvalid - 1)];
+        }
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const int k = j * kBlock + static_cast<int>(threadIdx.x);
+            keep_loaded(v[j]);
+            lds[lds_pad(k)] = k < nvalid ? v[j] : 1;
+        }
+    }
+    const int wave = static_cast<int>(threadIdx.x) / kWave;
+    if (tq) {
+        if (threadIdx.x < 2)
+            qrange[threadIdx.x] = tq[2 * tidx + threadIdx.x];
+    } else if (any && wave < 2) {
+        // the source tiles whose extras hold the first and the last
+        // childless rank: the last tile whose extras start at or before it
+        const long long x = wave == 0 ? Z : Z + zt - 1;
+        long long sx = wave_last_le([&](long long t) { return toff[3 * t + 1]; },
+            ntiles, x);
+        sx = sx < 0 ? 0 : sx;
+        if ((threadIdx.x & (kWave - 1)) == 0)
+            qrange[wave] = wave == 0 ? toff[3 * sx + 2] : toff[3 * (sx + 1) + 2];
+    }
+    __syncthreads();
+    // blocked order: thread t owns elements t kItems .. t kItems + kItems - 1;
+    // bit j of cmask: element kb + j is childless
+    const int kb = static_cast<int>(threadIdx.x) * kItems;
+    unsigned cmask = 0;
+    int z = 0;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const bool c = lds[lds_pad(kb + j)] == 0;
+        cmask |= c ? 1u << j : 0u;
+        z += c;
+    }
+    int ztot;
+    const int zex = block_exclusive_scan(z, fold::OpAddI(), 0, ired, &ztot);
+    // (the scan ends with a barrier: every rep has been read; ztot == zt
+    // for counts that sum to M)
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+        lds[lds_pad(j * kBlock + static_cast<int>(threadIdx.x))] = -1;
+    __syncthreads();
+    int out[kItems];
+    if (any && ztot > 0) {
+        const long long q0 = qrange[0];
+        const long long q1 = qrange[1] < C ? qrange[1] : C;
+        // each source starting after Z marks its first rank of the tile; the
+        // one covering rank Z (the last starting at or before it) marks rank
+        // 0, by a maximum. All loads of a thread in flight at once, a tile's
+        // worth of sources per pass.
+        int first = -1;
+        for (long long c0 = q0; c0 < q1; c0 += kTile) {
+            const int *const cEc = cE + c0;
+            const int nq = q1 - c0 < kTile ? static_cast<int>(q1 - c0) : kTile;
+            int e[kItems];
+#pragma unroll
+            for (int j = 0; j < kItems; ++j) {
+                const int o = j * kBlock + static_cast<int>(threadIdx.x);
+                e[j] = cEc[static_cast<unsigned>(o < nq ? o : nq - 1)];
+            }
+#pragma unroll
+            for (int j = 0; j < kItems; ++j) {
+                const int o = j * kBlock + static_cast<int>(threadIdx.x);
+                keep_loaded(e[j]);
+                e[j] = o < nq ? e[j] : INT_MAX;
+            }
+#pragma unroll
+            for (int j = 0; j < kItems; ++j) {
+                const int q = static_cast<int>(c0) + j * kBlock + static_cast<int>(threadIdx.x);
+                const long long pq = static_cast<long long>(e[j]) - Z;
+                if (pq <= 0)
+                    first = q > first ? q : first;
+                else if (pq < ztot)
+                    lds[lds_pad(static_cast<int>(pq))] = q;
+            }
+        }
+        first = wave_reduce(first, OpMaxI());
+        if ((threadIdx.x & (kWave - 1)) == 0 && first >= 0)
+            atomicMax(&lds[lds_pad(0)], first);
+        __syncthreads();
+        // ... and a running maximum carries them over the ranks they cover
+        int run = -1;
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const int v = lds[lds_pad(kb + j)];
+            run = v > run ? v : run;
+        }
+        int tot;
+        run = block_exclusive_scan(run, OpMaxI(), -1, ired, &tot);
+        if (threadIdx.x == 0) {
+            int q = lds[lds_pad(0)];
+            q = q < 0 ? 0 : (q >= C ? static_cast<int>(C - 1) : q);
+            src0s = csrc[static_cast<unsigned>(q)];
+        }
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const int v = lds[lds_pad(kb + j)];
+            run = v > run ? v : run;
+            lds[lds_pad(kb + j)] = run;
+        }
+        __syncthreads();
+        // every element loads (a kept one the source of a valid rank), so
+        // that no load sits behind a branch
+        int zr = zex;
+        int sv[kItems];
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const bool c = (cmask >> j) & 1u;
+            int q = lds[lds_pad(zr < ztot ? zr : ztot - 1)];
+            q = q < 0 ? 0 : (q >= C ? static_cast<int>(C - 1) : q);
+            sv[j] = csrc[static_cast<unsigned>(q)];
+            zr += c;
+        }
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            keep_loaded(sv[j]);
+            out[j] = (cmask >> j) & 1u ? sv[j] : static_cast<int>(base + kb + j);
+        }
+        __syncthreads();   // every rank's source read before the outputs go in
+    } else {
+        // no childless slot (or no source to take one from): identity
+#pragma unroll
+        for (int j = 0; j < kItems; ++j)
+            out[j] = static_cast<int>(base + kb + j);
+    }
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+        lds[lds_pad(kb + j)] = out[j];
+    __syncthreads();
+    int *const idxt = idx ? idx + base : nullptr;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const int k = j * kBlock + static_cast<int>(threadIdx.x);
+        out[j] = lds[lds_pad(k)];
+        if (idxt && k < nvalid)
+            idxt[static_cast<unsigned>(k)] = out[j];
+    }
+    if (state && any && ztot > 0) {
+        const int src0 = src0s;
+        // one component at a time: all loads of a thread, then its stores
+        for (int d = 0; d < dim; ++d) {
+            E *const col = state + d * ld;
+            E *const colt = col + base;
+            E v[kItems];
+            // unconditional loads: a kept or padding element reads the
+            // tile's first source (one line, shared) instead of waiting
+#pragma unroll
+            for (int j = 0; j < kItems; ++j) {
+                const int k = j * kBlock + static_cast<int>(threadIdx.x);
+                const bool c = k < nvalid && out[j] != base + k;
+                v[j] = col[static_cast<unsigned>(c ? out[j] : src0)];
+            }
+#pragma unroll
+            for (int j = 0; j < kItems; ++j)
+                keep_loaded(v[j]);
+#pragma unroll
+            for (int j = 0; j < kItems; ++j) {
+                const int k = j * kBlock + static_cast<int>(threadIdx.x);
+                if (k < nvalid && out[j] != base + k)
+                    colt[static_cast<unsigned>(k)] = v[j];
+            }
+        }
+    }
 }
 
 __global__ void k_index_rep(const int *idx, long long N, int *rep)
@@ -1794,11 +2261,18 @@ __global__ void __launch_bounds__(kBlock) k_integ_sum(WSrc src, long long M,
     __shared__ long long lds[kWaves];
     long long acc = 0;
     const long long base = static_cast<long long>(blockIdx.x) * kTile;
+    // clamped, unconditional loads, all in flight before their first use
+    long long v[kItems];
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
         const long long g = base + j * kBlock + threadIdx.x;
-        if (g < M)
-            acc += src.integ(g);
+        v[j] = src.integ(g < M ? g : M - 1);
+    }
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const long long g = base + j * kBlock + threadIdx.x;
+        keep_loaded(v[j]);
+        acc += g < M ? v[j] : 0;
     }
     acc = block_reduce(acc, OpAddL(), lds);
     if (threadIdx.x == 0)
@@ -1878,6 +2352,7 @@ bool Workspace::ensure(long long n)
     A(&zdst, sizeof(int) * nn);
     A(&cE, sizeof(int) * nn);
     A(&csrc, sizeof(int) * nn);
+    A(&tq, sizeof(int) * 2 * tiles);
     A(&v, sizeof(double) * nn);
     A(&aux, sizeof(double) * nn);
     A(&rep, sizeof(int) * nn);
@@ -1897,7 +2372,7 @@ bool Workspace::ensure(long long n)
 bool Workspace::all_allocated() const
 {
     const void *ps[] = {tsum, tpre, segs, xstart, xgstart, gmode, groups, tagg,
-        toff, zdst, cE, csrc, v, aux, rep, hard, prm, scal};
+        toff, zdst, cE, csrc, tq, v, aux, rep, hard, prm, scal};
     for (const void *p : ps)
         if (!p)
             return false;
@@ -1909,13 +2384,13 @@ void Workspace::release()
     if (fixed)
         return;
     void *ps[] = {tsum, tpre, segs, xstart, xgstart, gmode, groups, tagg, toff,
-        zdst, cE, csrc, v, aux, rep, hard, prm, scal};
+        zdst, cE, csrc, tq, v, aux, rep, hard, prm, scal};
     for (void *p : ps)
         if (p)
             (void)hipFree(p);
     tsum = tpre = xstart = xgstart = v = aux = nullptr;
     segs = nullptr;
-    gmode = zdst = cE = csrc = rep = hard = nullptr;
+    gmode = zdst = cE = csrc = tq = rep = hard = nullptr;
     groups = nullptr;
     tagg = toff = scal = nullptr;
     prm = nullptr;
@@ -2380,28 +2855,34 @@ int trans_rep_index(Workspace &ws, long long M, long long N, const int *rep,
         hipLaunchKernelGGL(k_ridx_agg<false>, dim3(ntiles), dim3(kBlock), 0,
             st, rep, M, prm, ws.tagg);
     }
-    hipLaunchKernelGGL(k_ridx_scan, dim3(1), dim3(kScanThreads), 0, st, ws.tagg,
-        ntiles, prm, ws.toff);
     if (expand) {
+        hipLaunchKernelGGL(k_ridx_scan, dim3(1), dim3(kScanThreads), 0, st,
+            ws.tagg, ntiles, prm, ws.toff);
         hipLaunchKernelGGL(k_ridx_scatter<true>, dim3(ntiles), dim3(kBlock), 0,
             st, rep, M, prm, ws.toff, ws.zdst, ws.cE, ws.csrc, idx);
-    } else {
-        hipLaunchKernelGGL(k_ridx_scatter<false>, dim3(ntiles), dim3(kBlock),
-            0, st, rep, M, prm, ws.toff, ws.zdst, ws.cE, ws.csrc, idx);
+        // the gather's block 0 ends the event (one launch fewer per step)
+        hipLaunchKernelGGL((k_expand<true, float>), dim3(grid_tiles(N)),
+            dim3(kBlock), 0, st, prm, ws.toff, ntiles, ws.zdst, ws.cE, ws.csrc,
+            idx, static_cast<float *>(nullptr), 0, 0LL, ws.prm, end);
+        VSMCH_HIP(hipGetLastError());
+        return kOk;
     }
-    const unsigned int ge = grid_tiles(N);
-    // the gather's block 0 ends the event (one launch fewer per step)
-    if (expand) {
-        hipLaunchKernelGGL((k_expand<true, float>), dim3(ge), dim3(kBlock), 0,
-            st, prm, ws.toff, ntiles, ws.zdst, ws.cE, ws.csrc, idx,
-            static_cast<float *>(nullptr), 0, 0LL, ws.prm, end);
-    } else if (state == nullptr || elem_bytes == 4) {
-        hipLaunchKernelGGL((k_expand<false, float>), dim3(ge), dim3(kBlock), 0,
-            st, prm, ws.toff, ntiles, ws.zdst, ws.cE, ws.csrc, idx,
+    // M == N: the prefixes (and each tile's slice of sources), the
+    // compacted sources, then every tile's indices (and gathered rows) in
+    // one pass; its block 0 ends the event
+    const bool use_tq = ntiles <= kPrefixLdsTiles;
+    hipLaunchKernelGGL(k_ridx_prefix, dim3(1), dim3(kScanThreads), 0, st,
+        ws.tagg, ntiles, prm, ws.toff, ws.tq);
+    hipLaunchKernelGGL(k_ridx_sources, dim3(ntiles), dim3(kBlock), 0, st, rep,
+        M, prm, ws.toff, ws.cE, ws.csrc);
+    const int *tq = use_tq ? ws.tq : nullptr;
+    if (state == nullptr || elem_bytes == 4) {
+        hipLaunchKernelGGL((k_ridx_fill<float>), dim3(ntiles), dim3(kBlock), 0,
+            st, prm, rep, M, ws.toff, tq, ntiles, ws.cE, ws.csrc, idx,
             static_cast<float *>(state), dim, ld, ws.prm, end);
     } else {
-        hipLaunchKernelGGL((k_expand<false, double>), dim3(ge), dim3(kBlock),
-            0, st, prm, ws.toff, ntiles, ws.zdst, ws.cE, ws.csrc, idx,
+        hipLaunchKernelGGL((k_ridx_fill<double>), dim3(ntiles), dim3(kBlock),
+            0, st, prm, rep, M, ws.toff, tq, ntiles, ws.cE, ws.csrc, idx,
             static_cast<double *>(state), dim, ld, ws.prm, end);
     }
     VSMCH_HIP(hipGetLastError());
