@@ -103,6 +103,34 @@ def time_iterate(va, torch, n, scheme, thr, warmup, steps, ox, oy, seed=1):
     return f, t0 + steps, ms, nres
 
 
+def scheme_events(va, torch, n, threshold, ox, oy, steps=20):
+    """The iterate loop with each scheme: never resampling, resampling every
+    step (systematic), and stratified / multinomial / residual when ESS <
+    threshold N. event_us: the step time above the never-resampling loop,
+    per resampling event; vs_systematic: that against systematic's."""
+    extra = {}
+    for name, sch, thr in (("systematic_never", va.SYSTEMATIC, 0.0),
+                           ("systematic_always", va.SYSTEMATIC, float("inf")),
+                           ("stratified", va.STRATIFIED, threshold),
+                           ("multinomial", va.MULTINOMIAL, threshold),
+                           ("residual", va.RESIDUAL, threshold)):
+        g, _, ms2, nres2 = time_iterate(va, torch, n, sch, thr, 2, steps, ox, oy)
+        g.close()
+        extra[name] = {"ms_per_step": ms2 / steps, "resampled_steps": nres2,
+                       "updates_per_s": n * steps / (ms2 / 1e3),
+                       "step_hbm_frac": n * (MOVE_BYTES * steps + RESAMPLE_BYTES * nres2)
+                       / (ms2 / 1e3) / 1e9 / HBM_PEAK_GBS}
+    base = extra["systematic_never"]["ms_per_step"]
+    for name, e in extra.items():
+        if e["resampled_steps"]:
+            e["event_us"] = (e["ms_per_step"] - base) * steps / e["resampled_steps"] * 1e3
+    sys_us = extra["systematic_always"]["event_us"]
+    for name, e in extra.items():
+        if "event_us" in e:
+            e["vs_systematic"] = e["event_us"] / sys_us
+    return extra
+
+
 def move_kernel_ms(va, torch, f, t, reps, ox, oy):
     """Average duration of the fused move (+ its one-block weight merge),
     HIP events on the library stream."""
@@ -521,7 +549,9 @@ def main():
     if "step" not in sections:
         # one workload alone (e.g. under a profiler): its figures only
         out = {}
-        for name, fn in (("extra", None), ("dropin", lambda va_, t_: dropin_path()),
+        for name, fn in (("extra", lambda va_, t_: scheme_events(va_, t_, a.n, a.threshold,
+                                                                  ox, oy)),
+                         ("dropin", lambda va_, t_: dropin_path()),
                          ("micro", microbench), ("learned", learned_vs_plain),
                          ("batched", batched_filters), ("gmm", tempered_gmm)):
             if name in sections and fn is not None:
@@ -585,19 +615,7 @@ def main():
         "copy_gbs": copy_gbs,
     }
     if "extra" in sections:
-        extra = {}
-        for name, sch, thr in (("systematic_never", va.SYSTEMATIC, 0.0),
-                               ("systematic_always", va.SYSTEMATIC, float("inf")),
-                               ("stratified", va.STRATIFIED, a.threshold),
-                               ("multinomial", va.MULTINOMIAL, a.threshold),
-                               ("residual", va.RESIDUAL, a.threshold)):
-            g, _, ms2, nres2 = time_iterate(va, torch, a.n, sch, thr, 2, 20, ox, oy)
-            g.close()
-            extra[name] = {"ms_per_step": ms2 / 20, "resampled_steps": nres2,
-                           "updates_per_s": a.n * 20 / (ms2 / 1e3),
-                           "step_hbm_frac": a.n * (MOVE_BYTES * 20 + RESAMPLE_BYTES * nres2)
-                           / (ms2 / 1e3) / 1e9 / HBM_PEAK_GBS}
-        out["extra"] = extra
+        out["extra"] = scheme_events(va, torch, a.n, a.threshold, ox, oy)
     if "dropin" in sections:
         out["dropin_functor_path"] = dropin_path()
     if "micro" in sections:

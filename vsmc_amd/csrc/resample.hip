@@ -905,6 +905,32 @@ __global__ void __launch_bounds__(kWave) k_starts(long long nh,
     }
 }
 
+// Last index i in [0, n) with at(i) <= x for a non-decreasing at(), or -1:
+// one whole wave, 64 probes a round (every lane returns the result; the loop
+// bounds are wave-uniform).
+template <typename At, typename X>
+__device__ inline long long wave_last_le(At at, long long n, X x)
+{
+    const long long lane = threadIdx.x & (kWave - 1);
+    long long lo = 0, hi = n;   // the first index with at(i) > x lies in [lo, hi]
+    while (lo < hi) {
+        const long long step = (hi - lo + kWave - 1) / kWave;
+        const long long p = lo + lane * step;
+        const bool le = p < hi && at(p) <= x;
+        const long long c = __popcll(__ballot(le));
+        if (c == 0)
+            break;
+        const long long nhi = lo + c * step;
+        lo = lo + (c - 1) * step + 1;
+        hi = nhi < hi ? nhi : hi;
+    }
+    return lo - 1;
+}
+
+// Draws of a tile's slice of materialised sorted uniforms held in LDS at a
+// time.
+constexpr int kVSlice = 4096;
+
 struct FoldBufs {
     const double *xstart;   // exact start of every tile, total at [ntiles]
 };
@@ -963,18 +989,109 @@ __device__ __forceinline__ double count_tile(WSrc src, USrc us, long long M,
     const int nvalid = M - base < kTile ? static_cast<int>(M - base) : kTile;
     // count before the tile's first element: that of the previous sum
     // (0 before particle 0: the merge starts at j = 0)
-    if (threadIdx.x == 0)
+    // (materialised draws: found with the tile's slice below)
+    if (!std::is_same<USrc, UArr>::value && threadIdx.x == 0)
         gfirst = (tidx == 0) ? 0 : us.count(x0);
     // each sum s_k is replaced by its count G(s_k); the last particle takes
     // every remaining draw
-    for (int j = 0; j < kItems; ++j) {
-        const int k = j * kBlock + static_cast<int>(threadIdx.x);
-        if (k < nvalid) {
-            const double s = tile[lds_pad(k)];
-            if (k == nvalid - 1)
-                xend = s;
-            const long long g = (base + k == M - 1) ? nd : us.count(s);
-            tile[lds_pad(k)] = __builtin_bit_cast(double, g);
+    if constexpr (std::is_same<USrc, UArr>::value) {
+        // Materialised sorted draws: the tile's counts lie in [G(x0),
+        // G(xend)], so only the slice v[G(x0), G(xend)) decides them; it is
+        // staged in LDS when it fits and every sum searched there, else
+        // searched in place (instead of every sum searching all draws).
+        __shared__ long long grange[2];
+        __shared__ double vs[kVSlice];
+        const int wave = static_cast<int>(threadIdx.x) / kWave;
+        const double xl = tile[lds_pad(nvalid - 1)];
+        if (wave < 2) {
+            const double x = wave == 0 ? x0 : xl;
+            const long long g = (wave == 0 && tidx == 0) ? 0 :
+                wave_last_le([&](long long q) { return us.v[q]; }, nd, x) + 1;
+            if ((threadIdx.x & (kWave - 1)) == 0)
+                grange[wave] = g;
+        }
+        if (threadIdx.x == 0)
+            xend = xl;
+        __syncthreads();
+        const long long glo = grange[0];
+        const long long ghi = grange[1] > glo ? grange[1] : glo;
+        const long long m = ghi - glo;
+        // the slice in chunks of kVSlice draws: a sum's count is in the
+        // first chunk whose last draw exceeds it (else it is m). Dense
+        // draws (at least one per two sums): every sum searches the chunk
+        // (strided order, the same number of steps on every lane). Sparse
+        // draws: each thread takes kItems consecutive sums (blocked order),
+        // which are sorted, a binary search for the first of them in a
+        // chunk, then short forward steps from the previous answer (a
+        // binary search past 8 steps).
+        const bool dense = 2 * m >= nvalid;
+        const int kb = static_cast<int>(threadIdx.x) * kItems;
+        long long cnt[kItems];
+        unsigned open = 0;   // bit j: the count of sum j not found yet
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const int k = dense ? j * kBlock + static_cast<int>(threadIdx.x) : kb + j;
+            cnt[j] = m;
+            open |= k < nvalid ? 1u << j : 0u;
+        }
+        for (long long c0 = 0; c0 < m; c0 += kVSlice) {
+            const int cn = m - c0 < kVSlice ? static_cast<int>(m - c0) : kVSlice;
+            for (int q = threadIdx.x; q < cn; q += kBlock)
+                vs[q] = us.v[glo + c0 + q];
+            __syncthreads();
+            const double vlast = vs[cn - 1];
+            int lo = -1;   // blocked: the previous answer in this chunk
+#pragma unroll
+            for (int j = 0; j < kItems; ++j) {
+                if ((open >> j) & 1u) {
+                    const int k = dense ? j * kBlock + static_cast<int>(threadIdx.x) : kb + j;
+                    const double sk = tile[lds_pad(k)];
+                    if (sk < vlast) {
+                        // first index with v > sk in [a, cn); vs[cn - 1] > sk
+                        int a = (dense || lo < 0) ? 0 : lo;
+                        int steps = (dense || lo < 0) ? 8 : 0;
+                        while (steps < 8 && vs[a] <= sk) {
+                            ++a;
+                            ++steps;
+                        }
+                        if (steps == 8) {
+                            int hi = cn - 1;
+                            while (a < hi) {
+                                const int mid = (a + hi) >> 1;
+                                if (vs[mid] <= sk)
+                                    a = mid + 1;
+                                else
+                                    hi = mid;
+                            }
+                        }
+                        lo = a;
+                        cnt[j] = c0 + a;
+                        open &= ~(1u << j);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const int k = dense ? j * kBlock + static_cast<int>(threadIdx.x) : kb + j;
+            if (k < nvalid) {
+                const long long g = (base + k == M - 1) ? nd : glo + cnt[j];
+                tile[lds_pad(k)] = __builtin_bit_cast(double, g);
+            }
+        }
+        if (threadIdx.x == 0)
+            gfirst = glo;
+    } else {
+        for (int j = 0; j < kItems; ++j) {
+            const int k = j * kBlock + static_cast<int>(threadIdx.x);
+            if (k < nvalid) {
+                const double sk = tile[lds_pad(k)];
+                if (k == nvalid - 1)
+                    xend = sk;
+                const long long g = (base + k == M - 1) ? nd : us.count(sk);
+                tile[lds_pad(k)] = __builtin_bit_cast(double, g);
+            }
         }
     }
     __syncthreads();
@@ -1704,28 +1821,6 @@ __global__ void __launch_bounds__(kBlock) k_ridx_sources(const int *rep,
         for (int q = threadIdx.x; q < nsrc; q += kBlock)
             csrc[c0 + q] = lds[q];
     }
-}
-
-// Last index i in [0, n) with at(i) <= x for a non-decreasing at(), or -1:
-// one whole wave, 64 probes a round (every lane returns the result; the loop
-// bounds are wave-uniform).
-template <typename At>
-__device__ inline long long wave_last_le(At at, long long n, long long x)
-{
-    const long long lane = threadIdx.x & (kWave - 1);
-    long long lo = 0, hi = n;   // the first index with at(i) > x lies in [lo, hi]
-    while (lo < hi) {
-        const long long step = (hi - lo + kWave - 1) / kWave;
-        const long long p = lo + lane * step;
-        const bool le = p < hi && at(p) <= x;
-        const long long c = __popcll(__ballot(le));
-        if (c == 0)
-            break;
-        const long long nhi = lo + c * step;
-        lo = lo + (c - 1) * step + 1;
-        hi = nhi < hi ? nhi : hi;
-    }
-    return lo - 1;
 }
 
 // Ancestor indices of tile tidx when M == N (the rep -> index layout of
