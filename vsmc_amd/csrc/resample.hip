@@ -928,8 +928,9 @@ __device__ inline long long wave_last_le(At at, long long n, X x)
 }
 
 // Draws of a tile's slice of materialised sorted uniforms held in LDS at a
-// time.
+// time, and the most chunks a tile walks that way.
 constexpr int kVSlice = 4096;
+constexpr int kVChunks = 4;
 
 struct FoldBufs {
     const double *xstart;   // exact start of every tile, total at [ntiles]
@@ -1025,6 +1026,10 @@ __device__ __forceinline__ double count_tile(WSrc src, USrc us, long long M,
         // chunk, then short forward steps from the previous answer (a
         // binary search past 8 steps).
         const bool dense = 2 * m >= nvalid;
+        // a slice of more than kVChunks chunks (weight concentrated in the
+        // tile) is searched in place by every sum instead, so that no block
+        // walks a long slice alone
+        const bool in_lds = m <= static_cast<long long>(kVChunks) * kVSlice;
         const int kb = static_cast<int>(threadIdx.x) * kItems;
         long long cnt[kItems];
         unsigned open = 0;   // bit j: the count of sum j not found yet
@@ -1034,7 +1039,26 @@ __device__ __forceinline__ double count_tile(WSrc src, USrc us, long long M,
             cnt[j] = m;
             open |= k < nvalid ? 1u << j : 0u;
         }
-        for (long long c0 = 0; c0 < m; c0 += kVSlice) {
+        if (!in_lds) {
+#pragma unroll
+            for (int j = 0; j < kItems; ++j) {
+                if ((open >> j) & 1u) {
+                    const int k = dense ? j * kBlock + static_cast<int>(threadIdx.x) : kb + j;
+                    const double sk = tile[lds_pad(k)];
+                    long long lo = 0, hi = m;   // first slice index with v > sk
+                    while (lo < hi) {
+                        const long long mid = (lo + hi) >> 1;
+                        if (us.v[glo + mid] <= sk)
+                            lo = mid + 1;
+                        else
+                            hi = mid;
+                    }
+                    cnt[j] = lo;
+                }
+            }
+            open = 0;
+        }
+        for (long long c0 = 0; c0 < (in_lds ? m : 0); c0 += kVSlice) {
             const int cn = m - c0 < kVSlice ? static_cast<int>(m - c0) : kVSlice;
             for (int q = threadIdx.x; q < cn; q += kBlock)
                 vs[q] = us.v[glo + c0 + q];
