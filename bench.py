@@ -4,16 +4,19 @@ MI355X: N = 2^24 particles, FP32 SoA state (x, y, vx, vy), FP64 log-weights,
 systematic resampling when ESS < N/2, observations simulated from a seed
 (synthetic data).
 
-A step is one pass of the hot path over the particle set: the fused move +
-log-likelihood + weight pass, the weight merge (ESS, log-evidence), and, when
-the device decides so, the exact-count resampling and in-place gather. The
-timed region brackets exactly --steps steps with inputs resident in HBM.
+A step is one pass of the hot path over the particle set, as the Sampler loop
+runs it on the device (vsmch_cv_iterate): the fused move + log-likelihood +
+weight pass (reading every particle through the previous event's ancestors),
+the finalize (ESS, log-evidence, decision, histories, monitor), and, when the
+device decides so, the exact-count resampling event. The timed region brackets
+exactly --steps steps with inputs resident in HBM.
 
 Prints one JSON line (the metric contract) with:
   value       particle-updates/s (N * steps / seconds)
-  roofline    achieved HBM GB/s of the dominant kernel (the fused move,
-              48 algorithmic bytes per particle) from HIP events on the
-              library stream, against 8 TB/s
+  roofline    the whole step: algorithmic bytes N (48 + 48 r) per step (r:
+              the fraction of steps that resample) over the step time from
+              HIP events on the library stream, against 8 TB/s; traffic from
+              the HBM counters; roofline_move: the fused move alone
   cpu_baseline  the oracle's OpenMP restatement of the reference path on the
               host cores, on a bounded sample of the same workload
 """
