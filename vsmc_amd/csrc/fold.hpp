@@ -136,7 +136,7 @@ __device__ inline bool apply_segments(const TileSeg &sg, double &x)
 }
 
 // Runs of one tile's multi-run pass (run starts at possible crossings).
-constexpr int kMaxRuns = 256;
+constexpr int kMaxRuns = 64;
 constexpr int kZeroTag = -4000;    // sum_tag of 0
 constexpr int kInfTag = 4000;      // sum_tag of +inf
 constexpr int kNoTag = -8000;      // run of one element
