@@ -145,6 +145,82 @@ __device__ inline T block_reduce_n(T v, Op op, T *lds)
     return r;
 }
 
+// Partials a merge thread holds in registers: thread t has partials
+// t kMergePer .. t kMergePer + kMergePer - 1, loaded together.
+constexpr int kMergePer = 4;
+
+// (max, S, Q) of the block partials part[0, nblocks) over kThreads threads,
+// broadcast to every thread: the maximum first, then independent scaled sums
+// (no serial chain of exponentials through a running maximum). With nblocks
+// <= kThreads kMergePer every thread loads its partials at once (clamped,
+// unconditional loads) and keeps them: held is set and sb[k] = s_b e^{m_b -
+// max} of its k-th partial (0 past nblocks). Otherwise strided loops.
+template <int kThreads>
+__device__ inline Part3 merge_partials(const Part3 *part, long long nblocks,
+    double (&sb)[kMergePer], bool &held, double *dlds)
+{
+    if (nblocks <= static_cast<long long>(kThreads) * kMergePer) {
+        held = true;
+        const long long b0 = static_cast<long long>(threadIdx.x) * kMergePer;
+        double m[kMergePer], s[kMergePer], q[kMergePer];
+#pragma unroll
+        for (int k = 0; k < kMergePer; ++k) {
+            const long long b = b0 + k < nblocks ? b0 + k : nblocks - 1;
+            m[k] = part[b].m;
+            s[k] = part[b].s;
+            q[k] = part[b].q;
+        }
+        double mx = -INFINITY;
+#pragma unroll
+        for (int k = 0; k < kMergePer; ++k) {
+            keep_loaded(m[k]);
+            keep_loaded(s[k]);
+            keep_loaded(q[k]);
+            if (b0 + k < nblocks)
+                mx = m[k] > mx ? m[k] : mx;
+        }
+        mx = block_reduce_n<kThreads>(mx, OpMaxD(), dlds);
+        double S = 0.0, Q = 0.0;
+#pragma unroll
+        for (int k = 0; k < kMergePer; ++k) {
+            double r = 0.0;
+            if (b0 + k < nblocks && m[k] > -INFINITY && mx > -INFINITY)
+                r = exp(m[k] - mx);
+            sb[k] = s[k] * r;
+            S += sb[k];
+            Q += q[k] * r * r;
+        }
+        S = block_reduce_n<kThreads>(S, OpAddD(), dlds);
+        Q = block_reduce_n<kThreads>(Q, OpAddD(), dlds);
+        return Part3{mx, S, Q};
+    }
+    held = false;
+    double mx = -INFINITY;
+    for (long long b = threadIdx.x; b < nblocks; b += kThreads) {
+        const double m = part[b].m;
+        mx = m > mx ? m : mx;
+    }
+    mx = block_reduce_n<kThreads>(mx, OpMaxD(), dlds);
+    double S = 0.0, Q = 0.0;
+    if (mx > -INFINITY) {
+        for (long long b = threadIdx.x; b < nblocks; b += kThreads) {
+            const Part3 p = part[b];
+            if (p.m > -INFINITY) {
+                const double r = exp(p.m - mx);
+                S += p.s * r;
+                Q += p.q * r * r;
+            }
+        }
+    }
+    S = block_reduce_n<kThreads>(S, OpAddD(), dlds);
+    Q = block_reduce_n<kThreads>(Q, OpAddD(), dlds);
+    return Part3{mx, S, Q};
+}
+
+// The weight scalars from a merged (max, S, Q): thread 0 only.
+__device__ inline void weight_merge_store(Part3 acc, long long n, int mode,
+    PDev *pd, int guard);
+
 // The body of k_weight_finalize (one block of kThreads).
 template <int kThreads>
 __device__ inline void weight_merge(const Part3 *part, long long nblocks,
@@ -153,31 +229,17 @@ __device__ inline void weight_merge(const Part3 *part, long long nblocks,
     __shared__ double dlds[kThreads / kWave];
     Part3 acc{-INFINITY, 0.0, 0.0};
     if (mode != 2) {
-        // the maximum first, then independent scaled sums (no serial chain
-        // of exponentials through a running maximum)
-        double mx = -INFINITY;
-        for (long long b = threadIdx.x; b < nblocks; b += kThreads) {
-            const double m = part[b].m;
-            mx = m > mx ? m : mx;
-        }
-        mx = block_reduce_n<kThreads>(mx, OpMaxD(), dlds);
-        double s = 0.0, q = 0.0;
-        if (mx > -INFINITY) {
-            for (long long b = threadIdx.x; b < nblocks; b += kThreads) {
-                const Part3 p = part[b];
-                if (p.m > -INFINITY) {
-                    const double r = exp(p.m - mx);
-                    s += p.s * r;
-                    q += p.q * r * r;
-                }
-            }
-        }
-        acc = Part3{mx, s, q};
-        acc.s = block_reduce_n<kThreads>(acc.s, OpAddD(), dlds);
-        acc.q = block_reduce_n<kThreads>(acc.q, OpAddD(), dlds);
+        double sb[kMergePer];
+        bool held;
+        acc = merge_partials<kThreads>(part, nblocks, sb, held, dlds);
     }
-    if (threadIdx.x != 0)
-        return;
+    if (threadIdx.x == 0)
+        weight_merge_store(acc, n, mode, pd, guard);
+}
+
+__device__ inline void weight_merge_store(Part3 acc, long long n, int mode,
+    PDev *pd, int guard)
+{
     if (mode == 2) {
         pd->equal = 1;
         pd->log_equal = -log(static_cast<double>(n));
@@ -317,34 +379,110 @@ __global__ void __launch_bounds__(kMergeThreads) k_step_finalize(
     int *rs_hist, double *logz_hist, const double *mpart, double *pos_prev,
     int scheme, int prep, double *tpre)
 {
-    weight_merge<kMergeThreads>(part, nb, n, mode, pd);
     __shared__ int act;
     __shared__ int sh_equal;
     __shared__ double sh_lmax, sh_inv;
     __shared__ double lds[kMergeThreads / kWave];
+    // the "pos" monitor's partials load with the weight partials (a thread's
+    // four consecutive blocks, clamped, unconditional loads)
+    const bool mheld = pos_prev && nb <= static_cast<long long>(kMergeThreads) * kMergePer;
+    double mp[2][kMergePer];
+    if (mheld) {
+        const long long b0 = static_cast<long long>(threadIdx.x) * kMergePer;
+#pragma unroll
+        for (int k = 0; k < kMergePer; ++k) {
+            const long long b = b0 + k < nb ? b0 + k : nb - 1;
+            mp[0][k] = mpart[b * 8 + 0];
+            mp[1][k] = mpart[b * 8 + 1];
+        }
+    }
+    // the particle set's scalars the step reads, loaded before the merge so
+    // that thread 0's tail below runs from registers
+    double logz0 = 0.0;
+    unsigned int rkey = 0;
+    unsigned long long rword = 0;
+    if (threadIdx.x == 0) {
+        logz0 = pd->logz;
+        rkey = pd->rkey;
+        rword = pd->rword;
+    }
+    Part3 acc{-INFINITY, 0.0, 0.0};
+    double sb[kMergePer];
+    bool held = false;
+    if (mode != 2)
+        acc = merge_partials<kMergeThreads>(part, nb, sb, held, lds);
     if (pos_prev) {
         for (int d = 0; d < 2; ++d) {
-            double acc = 0.0;
-            for (long long b = threadIdx.x; b < nb; b += kMergeThreads)
-                acc += mpart[b * 8 + d];
-            acc = block_reduce_n<kMergeThreads>(acc, OpAddD(), lds);
+            double a = 0.0;
+            if (mheld) {
+                const long long b0 = static_cast<long long>(threadIdx.x) * kMergePer;
+#pragma unroll
+                for (int k = 0; k < kMergePer; ++k) {
+                    keep_loaded(mp[d][k]);
+                    a += b0 + k < nb ? mp[d][k] : 0.0;
+                }
+            } else {
+                for (long long b = threadIdx.x; b < nb; b += kMergeThreads)
+                    a += mpart[b * 8 + d];
+            }
+            a = block_reduce_n<kMergeThreads>(a, OpAddD(), lds);
             if (threadIdx.x == 0)
-                pos_prev[d] = acc;
+                pos_prev[d] = a;
         }
     }
     if (threadIdx.x == 0) {
+        // weight_merge_store and decide_event on registers: the same
+        // arithmetic, every result stored once
+        const double nd = static_cast<double>(n);
+        const int equal = mode == 2 ? 1 : 0;
+        const double log_equal = -log(nd);
+        const double lmax = equal ? log_equal : acc.m;
+        const double sum = equal ? 1.0 : acc.s;
+        const double c = equal ? log_equal : acc.m + log(acc.s);
+        const double ess = equal ? nd : acc.s * acc.s / acc.q;
+        const double inc = mode == 0 ? c - log(nd) : c;
+        const double logz = equal ? logz0 : logz0 + inc;
+        pd->equal = equal;
+        pd->lmax = lmax;
+        pd->sum = sum;
+        pd->c = c;
+        pd->ess = ess;
+        if (equal) {
+            pd->log_equal = log_equal;
+        } else {
+            pd->logz_inc = inc;
+            pd->logz = logz;
+        }
+        if (mode == 0)
+            pd->fallback = 0;
         pd->anc_valid = 0;
         if (logz_hist)
-            logz_hist[h] = pd->logz;
+            logz_hist[h] = logz;
         if (ess_hist)
-            ess_hist[h] = pd->ess;
-        const int active = decide_event(pd, prm, n, threshold, scheme);
+            ess_hist[h] = ess;
+        const int active = ess < threshold * nd ? 1 : 0;
+        pd->resampled = active;
+        prm->active = active;
+        prm->M = n;
+        prm->nsample = n;
+        prm->ndraw = n;
+        prm->key = rkey;
+        prm->word0 = rword;
+        prm->err = 0;
+        prm->integ_sum = 0;
+        prm->words_used = 0;
+        prm->equal = equal;
+        prm->lmax = lmax;
+        prm->inv_sum = equal ? 1.0 : 1 / sum;
+        const double delta = 1 / nd;
+        prm->delta = delta;
+        prm->u0 = (scheme == kSystematic) ? u01_u32d(philox_word(rkey, rword)) * delta : 0.0;
         if (rs_hist)
             rs_hist[h] = active;
         act = active;
-        sh_equal = pd->equal;
-        sh_lmax = pd->lmax;
-        sh_inv = pd->equal ? 1.0 : 1 / pd->sum;
+        sh_equal = equal;
+        sh_lmax = lmax;
+        sh_inv = equal ? 1.0 : 1 / sum;
     }
     __syncthreads();
     if (!act || !prep)
@@ -353,6 +491,26 @@ __global__ void __launch_bounds__(kMergeThreads) k_step_finalize(
     const double lmax = sh_lmax;
     const double inv = sh_inv;
     const double inv_n = 1 / static_cast<double>(n);
+    if (held && !equal) {
+        // the approximate tile prefix from the scaled sums already held
+        const long long b0 = static_cast<long long>(threadIdx.x) * kMergePer;
+        double v[kMergePer];
+        double loc = 0.0;
+#pragma unroll
+        for (int k = 0; k < kMergePer; ++k) {
+            v[k] = sb[k] * inv;
+            loc += v[k];
+        }
+        double tot;
+        double run = block_exclusive_scan_n<kMergeThreads>(loc, OpAddD(), 0.0, lds, &tot);
+#pragma unroll
+        for (int k = 0; k < kMergePer; ++k) {
+            if (b0 + k < nb)
+                tpre[b0 + k] = run;
+            run += v[k];
+        }
+        return;
+    }
     block_scan_chunked_n<kMergeThreads, 4>(nb,
         [&](long long b) {
             if (equal) {
