@@ -30,7 +30,7 @@ namespace fold {
 
 constexpr long long kCap = 1LL << 54;
 constexpr long long kTwo53 = 1LL << 53;
-constexpr int kSegMax = 15;       // crossings a tile summary can hold
+constexpr int kSegMax = 47;       // crossings a tile summary can hold
 constexpr int kEmpty = -100000;   // binade tag of an empty run
 
 struct FMap {
