@@ -528,6 +528,16 @@ def cpu_baseline(budget_s):
                       f"{secs:.2f}s"}
 
 
+_T0 = time.time()
+
+
+def progress(section, ms_per_step=None):
+    """One line on stderr as each section ends, so a long run shows where it is."""
+    extra = "" if ms_per_step is None else f" ({ms_per_step:.4f} ms/step)"
+    print(f"[bench] {section} done at {time.time() - _T0:.1f} s{extra}", file=sys.stderr,
+          flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -617,18 +627,17 @@ def main():
                      / (ms_api / 1e3) / 1e9 / HBM_PEAK_GBS},
         "copy_gbs": copy_gbs,
     }
-    if "extra" in sections:
-        out["extra"] = scheme_events(va, torch, a.n, a.threshold, ox, oy)
-    if "dropin" in sections:
-        out["dropin_functor_path"] = dropin_path()
-    if "micro" in sections:
-        out["microbench"] = microbench(va, torch)
-    if "learned" in sections:
-        out["learned_proposal"] = learned_vs_plain(va, torch)
-    if "batched" in sections:
-        out["batched_filters"] = batched_filters(va, torch)
-    if "gmm" in sections:
-        out["tempered_gmm"] = tempered_gmm(va, torch)
+    progress("step", ms / a.steps)
+    for sec, key, fn in (
+            ("extra", "extra", lambda: scheme_events(va, torch, a.n, a.threshold, ox, oy)),
+            ("dropin", "dropin_functor_path", dropin_path),
+            ("micro", "microbench", lambda: microbench(va, torch)),
+            ("learned", "learned_proposal", lambda: learned_vs_plain(va, torch)),
+            ("batched", "batched_filters", lambda: batched_filters(va, torch)),
+            ("gmm", "tempered_gmm", lambda: tempered_gmm(va, torch))):
+        if sec in sections:
+            out[key] = fn()
+            progress(sec)
     out["cpu_baseline"] = None if a.no_cpu else cpu_baseline(a.cpu_seconds)
     print(json.dumps(out), flush=True)
 
