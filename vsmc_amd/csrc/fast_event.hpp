@@ -1,0 +1,693 @@
+// fast_event.hpp -- replication counts of a particle set's resampling event:
+// weights from the log-weights (W_i = e^(l_i - lmax) / sum), systematic or
+// stratified uniforms, M == N. Included by resample.hip after the fold
+// passes it falls back on, inside namespace vsmch.
+//
+// The sequential cumulative weights of resample_trans_u01_rep
+// (include/vsmc/resample/transform.hpp:61-69) are reproduced exactly, as in
+// fold.hpp (within one binade a sum is an integer multiple of the binade's
+// ulp and adding w adds the rounded step; a sum that leaves the binade is
+// one real FP64 add), by three launches:
+//  * k_fe_summ summarises every tile from the approximate start tpre[t]
+//    (the move's block partials): a tile whose approximate sums stay in
+//    one binade without half-ulp ties by the integer total of its rounded
+//    steps, from registers; tile 0 (sums from 0, across many binades) and
+//    every other tile by the full summary of fold.hpp (runs, their maps and
+//    the crossing weights), in the same block, which overlaps the others.
+//  * k_fe_resolve (one workgroup): consecutive one-binade tiles form a
+//    segment whose map is the composition of theirs (a segmented block
+//    scan); one thread walks the segments and the other tiles in order,
+//    verifying every predicted binade against the exact values, and every
+//    tile's exact start follows. A failed verification falls back, in the
+//    same workgroup, to the general resolver of the fold passes.
+//  * k_fe_count: every tile's sums from its exact start: one block scan of
+//    the rounded steps when the tile stays in the start's binade without
+//    ties; otherwise an FP64 block scan with a proven error bound decides
+//    every count unless a draw lies within the bound of a sum, and only then
+//    is the tile folded exactly by fold.hpp's passes.
+#pragma once
+
+constexpr int kFeSegs = 32;            // full summaries a resolver chunk holds
+constexpr int kFePer = 16;             // resolver: tiles per thread per chunk
+constexpr long long kFeChunk = static_cast<long long>(kBlock) * kFePer;
+constexpr int kFeChainMax = 512;       // chain items per chunk
+constexpr int kFeListTag = -100000;    // chain item tags below: listed tiles
+
+// Tile summary kinds.
+enum {
+    kFeMap = 0,     // sums stay in binade e: map (d0, d1) by the start's parity
+    kFeList = 1,    // full summary in segs[t] (fold::apply_segments)
+};
+
+struct FeSum {
+    int kind;
+    int e;
+    long long d0;
+    long long d1;
+};
+
+// A tile's weights, element j kBlock + tid in w[j] (0 past n): the
+// log-weights are loaded first (every load in flight before the first is
+// used; clamped indices in a partial tile), then W from each, with e^x from
+// the LDS copy tab of the table, which this copies and synchronises.
+__device__ __forceinline__ void fe_load_w(WLogV s, long long base, long long n,
+    double *tab, double (&w)[kItems])
+{
+    double l[kItems];
+    const bool full = base + kTile <= n;
+    if (full) {
+#pragma unroll
+        for (int j = 0; j < kItems; ++j)
+            l[j] = s.raw(base + j * kBlock + threadIdx.x);
+    } else {
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const long long g = base + j * kBlock + threadIdx.x;
+            l[j] = s.raw(g < n ? g : n - 1);
+        }
+    }
+    wexp_table_to_lds(tab);
+    __syncthreads();
+    s.tab = tab;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+        keep_loaded(l[j]);
+    if (full) {
+#pragma unroll
+        for (int j = 0; j < kItems; ++j)
+            w[j] = s.from_raw(l[j]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const long long g = base + j * kBlock + threadIdx.x;
+            w[j] = g < n ? s.from_raw(l[j]) : 0.0;
+        }
+    }
+}
+
+// Loads a tile's weights and leaves them in blocked order: thread tid holds
+// elements tid kItems + j in w[j] (through the LDS tile, element k at
+// lds_pad(k), which keeps them afterwards).
+__device__ __forceinline__ void fe_load_blocked(WLogV s, long long base,
+    long long n, double *tile, double *tab, double (&w)[kItems])
+{
+    fe_load_w(s, base, n, tab, w);
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+        tile[lds_pad(j * kBlock + threadIdx.x)] = w[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+        w[j] = tile[lds_pad(threadIdx.x * kItems + j)];
+}
+
+// Rounded steps in binade e: q = w / 2^(e - 52) rounded to the nearest
+// integer, ties to even, and whether q lies exactly halfway. For 0 <= q <
+// 2^52 (every step of a tile that stays in binade e) (q + 2^52) - 2^52 is
+// that rounding (the library builds without FP contraction). A larger q
+// gives a step of at least 2^52, which fails every check that the sums stay
+// below 2^53, so its value does not matter. ok: 2^(52 - e) is a normal
+// double (e >= -971; a tile whose sums are smaller takes another path).
+struct FeSteps {
+    double sc;    // 2^(52 - e)
+    double ulp;   // 2^(e - 52)
+    int e;
+    bool ok;
+    __device__ explicit FeSteps(int e_) : e(e_)
+    {
+        ok = 52 - e_ <= 1023;
+        sc = ldexp(1.0, ok ? 52 - e_ : 0);
+        ulp = ldexp(1.0, e_ - 52);
+    }
+    __device__ double q(double w) const { return w * sc; }
+    __device__ static double round(double q)
+    {
+        const double M = 0x1p52;
+        return (q + M) - M;
+    }
+    __device__ static bool half(double q, double r)
+    {
+        const double d = q - r;
+        return d == 0.5 || d == -0.5;
+    }
+    // a * 2^(e - 52) for an integer a < 2^53 (exact)
+    __device__ double value(double a) const { return a * ulp; }
+};
+
+struct FeSummShared {
+    fold::SegShared seg;         // seg.tile: the tile; the rest: a full summary
+    double tab[128];
+    double dred[kWaves];
+    double dred2[kWaves];
+};
+
+template <class WSrc>
+__global__ void __launch_bounds__(kBlock) k_fe_summ(WSrc src, long long n,
+    const ResampleDev *prm, const double *tpre, TileSeg *segs, FeSum *fs)
+{
+    if (!prm->active)
+        return;
+    const long long t = blockIdx.x;
+    const long long base = t * kTile;
+    if (base >= n)
+        return;
+    __shared__ FeSummShared sh;
+    const int tid = static_cast<int>(threadIdx.x);
+    const long long rem = n - base;
+    const int nvalid = rem < kTile ? static_cast<int>(rem) : kTile;
+    double w[kItems];
+    fe_load_blocked(src.loaded(), base, n, sh.seg.tile, sh.tab, w);
+    const double x0 = t == 0 ? 0.0 : tpre[t];
+    const bool pos = x0 > 0.0 && x0 < INFINITY;
+    const FeSteps st(pos ? fold::binade(x0) : 0);
+    double wsum = 0.0, dsum = 0.0;
+    bool tie = false;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const double q = st.q(w[j]);
+        const double r = FeSteps::round(q);
+        tie = tie || FeSteps::half(q, r);
+        wsum += w[j];
+        dsum += r;
+    }
+    const double T = block_reduce(wsum, OpAddD(), sh.dred);
+    const double D = block_reduce(dsum, OpAddD(), sh.dred2);
+    const bool anytie = __syncthreads_or(tie);
+    const double end = x0 + T;
+    if (pos && st.ok && !anytie && end < INFINITY && fold::binade(end) == st.e) {
+        // one binade, no ties: the map adds the integer total of the steps
+        // (saturated at 2^54: such a tile certainly leaves the binade)
+        if (tid == 0) {
+            const long long d = D < static_cast<double>(fold::kCap) ?
+                static_cast<long long>(D) : fold::kCap;
+            segs[t].ncross = 0;
+            segs[t].e[0] = st.e;
+            segs[t].map[0] = FMap{d, d};
+            fs[t] = FeSum{kFeMap, st.e, d, d};
+        }
+        return;
+    }
+    // the full summary of fold.hpp (tile 0, whose sums start at 0, a tile
+    // with a crossing or a tie, ...), from the tile still in LDS
+    __syncthreads();
+    fold::tile_segments(nvalid, x0, &segs[t], sh.seg);
+    __syncthreads();
+    if (tid == 0) {
+        const int nc = segs[t].ncross;
+        const int e0 = segs[t].e[0];
+        const FMap m = segs[t].map[0];
+        fs[t] = (nc == 0 && e0 != fold::kEmpty) ? FeSum{kFeMap, e0, m.d0, m.d1} :
+                                                 FeSum{kFeList, fold::kEmpty, 0, 0};
+    }
+}
+
+constexpr int kFeThreads = kBlock;     // resolver block (its fallback is kBlock-wide)
+constexpr int kFeWaves = kFeThreads / kWave;
+
+struct FeResolveShared {
+    TileSeg seg[kFeSegs];       // full summaries of the chunk's listed tiles
+    int segtile[kFeSegs];
+    // the chain: segments of one-binade tiles and the listed tiles, in order
+    int ce[kFeChainMax];        // segment binade, or kFeListTag - slot in seg[]
+    FMap cT[kFeChainMax];       // segment map
+    double cx[kFeChainMax];     // exact value before the item
+    fold::SegPair pscan[kFeWaves];
+    int iscan[kFeWaves];
+    int lastkind[kFeThreads];   // each thread's last tile: its kind, or -1 past the end
+    int laste[kFeThreads];
+    int firsthead[kFeThreads];  // each thread's first tile starts a chain item
+    double carry;
+    int fail;
+};
+
+union FeResolveLds {
+    FeResolveShared f;
+    ResolveShared r;
+};
+
+// The double a * 2^(e - 52) of an integer 2^52 <= a < 2^53 (or a < 2^53 in
+// the lowest binade, e = -1022) from its bits; fold::from_mant otherwise.
+__device__ inline double fe_from_mant(long long a, int e)
+{
+    if (e == -1022)
+        return __longlong_as_double(a);
+    if (a >= (1LL << 52))
+        return __longlong_as_double((static_cast<long long>(e + 1022) << 52) + a);
+    return fold::from_mant(a, e);
+}
+
+// Exact tile starts from the summaries; see the file comment. One block.
+template <class WSrc>
+__global__ void __launch_bounds__(kFeThreads) k_fe_resolve(WSrc src, long long n,
+    const ResampleDev *prm, const TileSeg *segs, const FeSum *fs,
+    GroupSeg *groups, double *xstart, double *xgstart, int *gmode)
+{
+    __shared__ FeResolveLds u;
+    FeResolveShared &sh = u.f;
+    if (!prm->active)
+        return;
+    const long long ntiles = ceil_div(n, kTile);
+    const int tid = static_cast<int>(threadIdx.x);
+    if (tid == 0) {
+        sh.carry = 0.0;
+        sh.fail = 0;
+    }
+    __syncthreads();
+    for (long long c0 = 0; c0 < ntiles && !sh.fail; c0 += kFeChunk) {
+        const long long tb = c0 + static_cast<long long>(tid) * kFePer;
+        // kind per tile, -1 past the end; a map tile joins the segment of
+        // the map tile before it when both lie in one binade
+        int kind[kFePer];
+        int ee[kFePer];
+        FMap mm[kFePer];
+#pragma unroll
+        for (int k = 0; k < kFePer; ++k) {
+            const long long t = tb + k;
+            const FeSum v = fs[t < ntiles ? t : ntiles - 1];
+            kind[k] = t >= ntiles ? -1 : v.kind;
+            ee[k] = v.e;
+            mm[k] = FMap{v.d0, v.d1};
+        }
+        sh.lastkind[tid] = kind[kFePer - 1];
+        sh.laste[tid] = ee[kFePer - 1];
+        __syncthreads();
+        int pk = tid == 0 ? kFeList : sh.lastkind[tid - 1];
+        int pe = tid == 0 ? 0 : sh.laste[tid - 1];
+        unsigned head = 0;
+        int nhead = 0, nlist = 0;
+        fold::SegPair agg{0, fold::ident()};
+#pragma unroll
+        for (int k = 0; k < kFePer; ++k) {
+            bool h = false;
+            if (kind[k] == kFeMap)
+                h = (tb + k == c0) || pk != kFeMap || pe != ee[k];
+            else if (kind[k] >= 0)
+                h = true;
+            head |= h ? (1u << k) : 0u;
+            nhead += h ? 1 : 0;
+            nlist += kind[k] == kFeList ? 1 : 0;
+            agg = fold::OpSeg()(agg, fold::SegPair{h ? 1LL : 0LL,
+                kind[k] == kFeMap ? mm[k] : fold::ident()});
+            pk = kind[k];
+            pe = ee[k];
+        }
+        // one scan for both counts (each at most kFeChunk < 2^15)
+        int tot_hl;
+        const int hl = block_exclusive_scan_n<kFeThreads>(nhead | (nlist << 16),
+            fold::OpAddI(), 0, sh.iscan, &tot_hl);
+        const int hrank = hl & 0xFFFF, lrank = hl >> 16;
+        const int tot_head = tot_hl & 0xFFFF, tot_list = tot_hl >> 16;
+        fold::SegPair stot;
+        const fold::SegPair run = block_exclusive_scan_n<kFeThreads>(agg, fold::OpSeg(),
+            fold::SegPair{0, fold::ident()}, sh.pscan, &stot);
+        if (tot_head > kFeChainMax || tot_list > kFeSegs) {
+            if (tid == 0)
+                sh.fail = 1;
+            __syncthreads();
+            break;
+        }
+        sh.firsthead[tid] = ((head & 1u) != 0 || kind[0] < 0) ? 1 : 0;
+        __syncthreads();
+        {
+            int r = hrank - 1;   // chain item of the current tile
+            int li = lrank;
+            fold::SegPair acc = run;
+#pragma unroll
+            for (int k = 0; k < kFePer; ++k) {
+                if (kind[k] < 0)
+                    break;
+                const bool h = (head >> k) & 1u;
+                acc = fold::OpSeg()(acc, fold::SegPair{h ? 1LL : 0LL,
+                    kind[k] == kFeMap ? mm[k] : fold::ident()});
+                if (h) {
+                    ++r;
+                    if (kind[k] == kFeList) {
+                        sh.ce[r] = kFeListTag - li;
+                        sh.segtile[li] = static_cast<int>(tb + k);
+                        ++li;
+                    } else {
+                        sh.ce[r] = ee[k];
+                    }
+                }
+                bool next_head;
+                if (k + 1 < kFePer)
+                    next_head = ((head >> (k + 1)) & 1u) != 0 || kind[k + 1] < 0;
+                else
+                    next_head = tid + 1 == kFeThreads || sh.firsthead[tid + 1] != 0;
+                if (kind[k] == kFeMap && next_head)
+                    sh.cT[r] = acc.m;
+            }
+        }
+        __syncthreads();
+        // full summaries of the listed tiles into LDS
+        constexpr int kWords = static_cast<int>(sizeof(TileSeg) / 4);
+        for (int q = tid; q < tot_list * kWords; q += kFeThreads) {
+            const int k = q / kWords, wd = q % kWords;
+            reinterpret_cast<int *>(&sh.seg[k])[wd] =
+                reinterpret_cast<const int *>(&segs[sh.segtile[k]])[wd];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            // the last item of the data: only its start is needed (the
+            // counting pass recomputes the last tile from it, and each
+            // tile's start is checked below), so its prediction past its
+            // start is not verified (the total lies at 1, a binade boundary,
+            // where predictions often fail)
+            const bool last_chunk = c0 + kFeChunk >= ntiles;
+            double x = sh.carry;
+            int ok = 1;
+            for (int r = 0; r < tot_head; ++r) {
+                const bool final_item = last_chunk && r == tot_head - 1;
+                sh.cx[r] = x;
+                const int ce = sh.ce[r];
+                if (ce > kFeListTag) {
+                    // a segment in binade ce
+                    const FMap T = sh.cT[r];
+                    if (!(x > 0.0) || !(x < INFINITY) || fold::binade(x) != ce) {
+                        ok = 0;
+                        break;
+                    }
+                    const long long a0 = fold::mant(x);
+                    const long long a = a0 + ((a0 & 1) ? T.d1 : T.d0);
+                    if (a >= fold::kTwo53) {
+                        if (!final_item)
+                            ok = 0;
+                        break;
+                    }
+                    x = fe_from_mant(a, ce);
+                } else {
+                    if (!fold::apply_segments(sh.seg[kFeListTag - ce], x)) {
+                        if (!final_item)
+                            ok = 0;
+                        break;
+                    }
+                }
+            }
+            sh.carry = x;
+            sh.fail = ok ? 0 : 1;
+        }
+        __syncthreads();
+        if (sh.fail)
+            break;
+        // every tile's exact start
+        {
+            int r = hrank - 1;
+            FMap P = fold::ident();   // map from the segment start to this tile
+            double xs = 0.0;
+            long long a0 = 0;
+            int e = 0;
+            bool bad = false;
+            if (!((head & 1u) != 0) && kind[0] == kFeMap) {
+                // continues the previous thread's segment
+                P = run.m;
+                xs = sh.cx[r];
+                a0 = fold::mant(xs);
+                e = ee[0];
+            }
+#pragma unroll
+            for (int k = 0; k < kFePer; ++k) {
+                if (kind[k] < 0)
+                    break;
+                const long long t = tb + k;
+                const bool h = (head >> k) & 1u;
+                if (h) {
+                    ++r;
+                    P = fold::ident();
+                    xs = sh.cx[r];
+                    a0 = fold::mant(xs);
+                    e = ee[k];
+                }
+                if (kind[k] != kFeMap) {
+                    xstart[t] = xs;
+                } else {
+                    const long long a = a0 + ((a0 & 1) ? P.d1 : P.d0);
+                    // a start past its binade (only possible in the last
+                    // segment, whose end is not checked)
+                    bad = bad || a >= fold::kTwo53;
+                    xstart[t] = h ? xs : fe_from_mant(a, e);
+                    P = fold::compose(P, mm[k]);
+                }
+            }
+            if (__syncthreads_or(bad) && tid == 0)
+                sh.fail = 1;
+        }
+        __syncthreads();
+    }
+    if (sh.fail) {
+        // the general resolver: group summaries, the windowed walk, the
+        // starts of the groups advanced whole
+        __syncthreads();
+        const long long ngroups = ceil_div(ntiles, kGroup);
+        for (long long g = threadIdx.x / kWave; g < ngroups; g += kFeWaves)
+            group_summary(segs, ntiles, g, groups);
+        __syncthreads();
+        resolve_body(src, n, segs, groups, xstart, xgstart, gmode, u.r);
+        __syncthreads();
+        for (long long g = threadIdx.x / kWave; g < ngroups; g += kFeWaves)
+            starts_group(g, ntiles, segs, xgstart, gmode, xstart);
+    }
+}
+
+// Systematic draws with the parameters in registers: u_j = u0 + j delta
+// without FMA (u01_sequence.hpp:246, as USys). Counts are below 2^31
+// (particle indices are int32).
+struct FeSys {
+    int n;
+    double u0;
+    double delta;
+    __device__ explicit FeSys(const USys &us)
+        : n(static_cast<int>(us.p->ndraw)), u0(us.p->u0), delta(us.p->delta) {}
+    __device__ double ui(int j) const
+    {
+        return rn_add(u0, rn_mul(static_cast<double>(j), delta));
+    }
+    // Number of draws u_j <= x. The estimate (x - u0) n is within n 2^-50 of
+    // its real value and every u_j within 2^-51 of u0 + j / n (x <= 2), so a
+    // fractional part farther than n 2^-48 from an integer fixes the count;
+    // otherwise the draws next to the estimate are compared (USys::count).
+    __device__ int count(double x) const
+    {
+        const double nd = static_cast<double>(n);
+        const double est = (x - u0) * nd;
+        const double fl = floor(est);
+        const double fr = est - fl;
+        const double m = nd * 0x1p-48;
+        if (x <= 2.0 && est >= 0.0 && est < nd && fr > m && fr < 1.0 - m)
+            return static_cast<int>(fl) + 1;
+        return slow(x, est);
+    }
+    __device__ int slow(double x, double est) const
+    {
+        if (n <= 0)
+            return 0;
+        if (!(x < INFINITY))
+            return n;
+        int g;
+        if (!(est > 0.0))
+            g = 0;
+        else if (est >= static_cast<double>(n))
+            g = n;
+        else
+            g = static_cast<int>(est) + 1;
+        while (g > 0 && ui(g - 1) > x)
+            --g;
+        while (g < n && ui(g) <= x)
+            ++g;
+        return g;
+    }
+};
+
+// Stratified draws: UStrat's search (two Philox blocks per count).
+struct FeStrat {
+    UStrat us;
+    __device__ explicit FeStrat(const UStrat &u) : us(u) {}
+    __device__ int count(double x) const { return static_cast<int>(us.count(x)); }
+};
+
+template <class U> struct FeCounter;
+template <> struct FeCounter<USys> { using type = FeSys; };
+template <> struct FeCounter<UStrat> { using type = FeStrat; };
+
+struct FeAgg {
+    int e;
+    int z;
+    int c;
+    int pad;
+};
+struct OpFeAgg {
+    __device__ FeAgg operator()(FeAgg a, FeAgg b) const
+    {
+        return FeAgg{a.e + b.e, a.z + b.z, a.c + b.c, 0};
+    }
+};
+
+// Relative bound between an FP64 block-scan prefix of a tile (exact start
+// plus a blocked scan: at most 16 + 6 + 3 + 18 roundings along any path)
+// and the sequential fold over at most 4096 weights from the same exact
+// start: (4097 + 43) 2^-53 (1 + 2^-40) < 2^-40.9; twice that, so that the
+// rounding of the bounds themselves is covered.
+constexpr double kFeCertRel = 0x1p-39;
+
+struct FeCountShared {
+    double tile[kTileLds];
+    fold::ExactShared ex;
+    double tab[128];
+    double dlds[kWaves];
+    FeAgg alds[kWaves];
+};
+
+// Counts of every tile from its exact start xstart[t] (see the file
+// comment); rep -> index aggregates (childless, extras, sources) per tile.
+template <class WSrc, class USrc>
+__global__ void __launch_bounds__(kBlock, 4) k_fe_count(WSrc src, USrc us,
+    long long M, const ResampleDev *prm, double *xstart, int *rep,
+    long long *tagg)
+{
+    if (!prm->active)
+        return;
+    const long long t = blockIdx.x;
+    const long long base = t * kTile;
+    if (base >= M)
+        return;
+    __shared__ FeCountShared sh;
+    const int tid = static_cast<int>(threadIdx.x);
+    const long long rem = M - base;
+    const int nvalid = rem < kTile ? static_cast<int>(rem) : kTile;
+    const bool last_tile = base + kTile >= M;
+    const double x0 = xstart[t];
+    double w[kItems];
+    fe_load_blocked(src.loaded(), base, M, sh.tile, sh.tab, w);
+    const bool xpos = x0 > 0.0 && x0 < INFINITY;
+    const FeSteps st(xpos ? fold::binade(x0) : 0);
+    double dsum = 0.0;
+    bool tie = false;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const double q = st.q(w[j]);
+        const double r = FeSteps::round(q);
+        tie = tie || FeSteps::half(q, r);
+        dsum += r;
+    }
+    double D;
+    double run = block_exclusive_scan(dsum, OpAddD(), 0.0, sh.dlds, &D);
+    const bool anytie = __syncthreads_or(tie);
+    const double a0d = xpos ? static_cast<double>(fold::mant(x0)) : 0.0;
+    const bool fast = xpos && st.ok && !anytie &&
+        a0d + D < static_cast<double>(fold::kTwo53);
+    const typename FeCounter<USrc>::type cnt(us);
+    const int nd = static_cast<int>(us.ndraw());
+    const int k0 = tid * kItems;
+    // g[j] = G(s_k) for element k = k0 + j, gp = G before the thread's first
+    int g[kItems];
+    int gp;
+    if (fast) {
+        // s_k = x0 advanced by the steps up to k, all in x0's binade
+        const double sprev = tid == 0 ? x0 : st.value(a0d + run);
+        gp = (base + k0 == 0) ? 0 : cnt.count(sprev);
+        double s[kItems];
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            run += FeSteps::round(st.q(w[j]));
+            s[j] = st.value(a0d + run);
+            g[j] = cnt.count(s[j]);
+        }
+        // the total after the last tile, which the resolver leaves
+        // unverified (the fast event's callers do not read it)
+        if (last_tile) {
+#pragma unroll
+            for (int j = 0; j < kItems; ++j)
+                if (k0 + j == nvalid - 1)
+                    xstart[t + 1] = s[j];
+        }
+    } else {
+        // an FP64 prefix within kFeCertRel of every exact sum: a count is
+        // certain when both ends of the interval give it
+        double loc = 0.0;
+#pragma unroll
+        for (int j = 0; j < kItems; ++j)
+            loc += w[j];
+        double tw;
+        const double ex = block_exclusive_scan(loc, OpAddD(), 0.0, sh.dlds, &tw);
+        double p = x0 + ex;
+        bool amb = false;
+        auto certain = [&](double v) {
+            if (!(v < INFINITY))
+                return cnt.count(v);
+            const double m = v * kFeCertRel;
+            const int lo = cnt.count(v - m);
+            const int hi = cnt.count(v + m);
+            amb = amb || lo != hi;
+            return lo;
+        };
+        if (base + k0 == 0)
+            gp = 0;
+        else if (tid == 0)
+            gp = cnt.count(x0);
+        else
+            gp = certain(p);
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            p = p + w[j];
+            g[j] = certain(p);
+        }
+        if (__syncthreads_or(amb)) {
+            // a draw within the bound of a sum: the exact fold of fold.hpp
+            // (one binade with ties in order, runs between crossings, and
+            // element by element where a run's prediction fails)
+            fold::tile_exact(sh.tile, x0, sh.ex);
+            gp = (base + k0 == 0) ? 0 :
+                cnt.count(tid == 0 ? x0 : sh.tile[lds_pad(k0 - 1)]);
+#pragma unroll
+            for (int j = 0; j < kItems; ++j)
+                g[j] = cnt.count(sh.tile[lds_pad(k0 + j)]);
+        }
+    }
+    // counts: rep_k = G(s_k) - G(s_{k-1}); G before particle 0 is 0, and the
+    // last particle takes every remaining draw
+    FeAgg a{0, 0, 0, 0};
+    int r[kItems];
+    if (!last_tile) {
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const int rr = g[j] - gp;
+            r[j] = rr;
+            gp = g[j];
+            a.z += rr == 0;
+            a.e += rr > 1 ? rr - 1 : 0;
+            a.c += rr > 1;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const int k = k0 + j;
+            int gk = gp;
+            if (k < nvalid)
+                gk = (base + k == M - 1) ? nd : g[j];
+            const int rr = gk - gp;
+            r[j] = rr;
+            gp = gk;
+            if (k < nvalid) {
+                a.z += rr == 0;
+                a.e += rr > 1 ? rr - 1 : 0;
+                a.c += rr > 1;
+            }
+        }
+    }
+    if (k0 + kItems <= nvalid && (reinterpret_cast<uintptr_t>(rep) & 15) == 0) {
+        int4 *dst = reinterpret_cast<int4 *>(rep + base + k0);
+#pragma unroll
+        for (int q = 0; q < kItems / 4; ++q)
+            dst[q] = make_int4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < kItems; ++j)
+            if (k0 + j < nvalid)
+                rep[base + k0 + j] = r[j];
+    }
+    const FeAgg tot = block_reduce(a, OpFeAgg(), sh.alds);
+    if (tid == 0) {
+        tagg[3 * t + 0] = tot.z;
+        tagg[3 * t + 1] = tot.e;
+        tagg[3 * t + 2] = tot.c;
+    }
+}

@@ -80,6 +80,7 @@ struct Workspace {
     double *aux = nullptr;       // scratch N doubles
     int *rep = nullptr;          // scratch counts
     int *hard = nullptr;         // batched filters: [0] count, then tiles to fold in full
+    struct FeSum *fsum = nullptr;  // dense per-tile summaries (particle-set events)
     int *integ_dummy = nullptr;
     ResampleDev *prm = nullptr;
     long long *scal = nullptr;   // small device scalars

@@ -583,16 +583,16 @@ __global__ void __launch_bounds__(kPB) k_moments_finalize(const double *part,
 }
 
 // Normalised linear weights, the numbers the resampling passes use:
-// W_i = exp(l_i - lmax) * (1 / sum), the reference's order of operations
-// (weight.hpp:200-222: subtract the maximum, exponentiate, scale by the
-// reciprocal of the sum).
+// W_i = e^(l_i - lmax) * (1 / sum) (e^x by wexp), the reference's order of
+// operations (weight.hpp:200-222: subtract the maximum, exponentiate, scale
+// by the reciprocal of the sum).
 __global__ void k_weight_read(const double *l, long long n, const PDev *pd,
     double *W)
 {
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n)
         W[i] = pd->equal ? 1.0 / static_cast<double>(n) :
-                           (1 / pd->sum) * exp(l[i] - pd->lmax);
+                           (1 / pd->sum) * wexp(l[i] - pd->lmax);
 }
 
 // ------------------------------------------------------------ CV model

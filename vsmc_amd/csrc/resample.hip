@@ -44,15 +44,21 @@ struct WLogV {
     double lmax;
     double eqw;
     int equal;
-    __device__ double operator()(long long i) const
+    const double *tab;   // kWexpTab or a copy in LDS
+    __device__ double operator()(long long i) const { return from_raw(raw(i)); }
+    // the log-weight alone (so that a tile's loads can be issued together)
+    // and the weight from it
+    __device__ double raw(long long i) const { return l[i]; }
+    __device__ double from_raw(double li) const
     {
-        const double v = inv_sum * exp(l[i] - lmax);
+        const double v = inv_sum * wexp(li - lmax, tab);
         return equal ? eqw : v;
     }
     __device__ long long integ(long long) const { return 0; }
 };
 
-// W_i = inv_sum * exp(l_i - lmax) (Weight::normalize, weight.hpp:208-222)
+// W_i = inv_sum * e^(l_i - lmax) (Weight::normalize, weight.hpp:208-222;
+// e^x by wexp)
 struct WLog {
     const double *l;
     const ResampleDev *p;
@@ -60,13 +66,13 @@ struct WLog {
     {
         if (p->equal)
             return 1.0 / static_cast<double>(p->M);
-        return p->inv_sum * exp(l[i] - p->lmax);
+        return p->inv_sum * wexp(l[i] - p->lmax);
     }
     __device__ long long integ(long long) const { return 0; }
     __device__ WLogV loaded() const
     {
         return WLogV{l, p->inv_sum, p->lmax, 1.0 / static_cast<double>(p->M),
-            p->equal};
+            p->equal, kWexpTab};
     }
 };
 
@@ -389,7 +395,8 @@ __global__ void __launch_bounds__(kBlock) k_scan_tiles(const double *tsum,
 __device__ __forceinline__ void group_summary(const TileSeg *segs,
     long long ntiles, long long g, GroupSeg *groups)
 {
-    const long long t = g * kGroup + threadIdx.x;
+    const int lane = static_cast<int>(threadIdx.x % kWave);
+    const long long t = g * kGroup + lane;
     const bool valid = t < ntiles;
     int nc = 0, e = fold::kEmpty;
     FMap m = fold::ident();
@@ -403,7 +410,7 @@ __device__ __forceinline__ void group_summary(const TileSeg *segs,
     const unsigned long long bal = __ballot(ok);
     const FMap P = wave_inclusive_scan(m, fold::OpCompose());
     const FMap tot = shfl_any(P, kWave - 1);
-    if (threadIdx.x == 0) {
+    if (lane == 0) {
         GroupSeg gs;
         gs.flat = (bal == ~0ull) ? 1 : 0;
         gs.e = e0;
@@ -827,16 +834,12 @@ __device__ __forceinline__ void resolve_walk(ResolveShared &sh,
 // wave 0 walks them, and the whole block evaluates a tile exactly where a
 // prediction fails.
 template <class WSrc>
-__global__ void __launch_bounds__(kBlock) k_resolve(WSrc src, long long nh,
-    const long long *ndev, const ResampleDev *prm, const TileSeg *segs,
-    const GroupSeg *groups, double *xstart, double *xgstart, int *gmode)
+__device__ __forceinline__ void resolve_body(WSrc src, long long n,
+    const TileSeg *segs, const GroupSeg *groups, double *xstart,
+    double *xgstart, int *gmode, ResolveShared &sh)
 {
-    if (!prm->active)
-        return;
-    const long long n = eff_n(ndev, nh);
     const long long ntiles = ceil_div(n, kTile);
     const long long ngroups = ceil_div(ntiles, kGroup);
-    __shared__ ResolveShared sh;
     if (threadIdx.x == 0)
         sh.x = 0.0;
     for (long long gw0 = 0; gw0 < ngroups; gw0 += kRGroups) {
@@ -874,35 +877,53 @@ __global__ void __launch_bounds__(kBlock) k_resolve(WSrc src, long long nh,
     }
 }
 
+template <class WSrc>
+__global__ void __launch_bounds__(kBlock) k_resolve(WSrc src, long long nh,
+    const long long *ndev, const ResampleDev *prm, const TileSeg *segs,
+    const GroupSeg *groups, double *xstart, double *xgstart, int *gmode)
+{
+    if (!prm->active)
+        return;
+    __shared__ ResolveShared sh;
+    resolve_body(src, eff_n(ndev, nh), segs, groups, xstart, xgstart, gmode, sh);
+}
+
 // Exact tile starts inside the groups the resolver advanced whole (gmode):
 // tile t0 + k starts at the group start mapped by the first k tile maps,
 // all in the group's binade and below its top. One wave per group; the
 // resolver wrote xstart for the tiles of every other group.
+__device__ __forceinline__ void starts_group(long long g, long long ntiles,
+    const TileSeg *segs, const double *xgstart, const int *gmode,
+    double *xstart)
+{
+    const int lane = static_cast<int>(threadIdx.x % kWave);
+    const long long t0 = g * kGroup;
+    if (t0 >= ntiles || !gmode[g])
+        return;
+    const long long t = t0 + lane;
+    FMap m = fold::ident();
+    if (t < ntiles)
+        m = segs[t].map[0];
+    const FMap P = wave_inclusive_scan(m, fold::OpCompose());
+    FMap E = shfl_up_any(P, 1);
+    if (lane == 0)
+        E = fold::ident();
+    if (t < ntiles) {
+        const double x = xgstart[g];
+        const long long a0 = fold::mant(x);
+        xstart[t] = fold::from_mant(a0 + ((a0 & 1) ? E.d1 : E.d0),
+            fold::binade(x));
+    }
+}
+
 __global__ void __launch_bounds__(kWave) k_starts(long long nh,
     const long long *ndev, const ResampleDev *prm, const TileSeg *segs,
     const double *xgstart, const int *gmode, double *xstart)
 {
     if (!prm->active)
         return;
-    const long long n = eff_n(ndev, nh);
-    const long long ntiles = ceil_div(n, kTile);
-    const long long t0 = static_cast<long long>(blockIdx.x) * kGroup;
-    if (t0 >= ntiles || !gmode[blockIdx.x])
-        return;
-    const long long t = t0 + threadIdx.x;
-    FMap m = fold::ident();
-    if (t < ntiles)
-        m = segs[t].map[0];
-    const FMap P = wave_inclusive_scan(m, fold::OpCompose());
-    FMap E = shfl_up_any(P, 1);
-    if (threadIdx.x == 0)
-        E = fold::ident();
-    if (t < ntiles) {
-        const double x = xgstart[blockIdx.x];
-        const long long a0 = fold::mant(x);
-        xstart[t] = fold::from_mant(a0 + ((a0 & 1) ? E.d1 : E.d0),
-            fold::binade(x));
-    }
+    const long long ntiles = ceil_div(eff_n(ndev, nh), kTile);
+    starts_group(blockIdx.x, ntiles, segs, xgstart, gmode, xstart);
 }
 
 // Last index i in [0, n) with at(i) <= x for a non-decreasing at(), or -1:
@@ -2417,7 +2438,7 @@ __global__ void k_words_used(ResampleDev *p, const int *rep, int scheme,
         if (resid) {
             double w = wkind == kWeightLog ?
                 (p->equal ? 1.0 / static_cast<double>(M) :
-                            p->inv_sum * exp(lw[M - 1] - p->lmax)) :
+                            p->inv_sum * wexp(lw[M - 1] - p->lmax)) :
                 W[M - 1];
             double ip;
             modf(static_cast<double>(p->nsample) * w, &ip);
@@ -2428,6 +2449,8 @@ __global__ void k_words_used(ResampleDev *p, const int *rep, int scheme,
     }
     p->words_used = used;
 }
+
+#include "fast_event.hpp"
 
 // ------------------------------------------------------------ host side
 
@@ -2476,6 +2499,7 @@ bool Workspace::ensure(long long n)
     A(&aux, sizeof(double) * nn);
     A(&rep, sizeof(int) * nn);
     A(&hard, sizeof(int) * (tiles + 1));
+    A(&fsum, sizeof(FeSum) * tiles);
     A(&prm, sizeof(ResampleDev));
     A(&scal, sizeof(long long) * 16);
     if (!ok || !all_allocated()) {
@@ -2491,7 +2515,7 @@ bool Workspace::ensure(long long n)
 bool Workspace::all_allocated() const
 {
     const void *ps[] = {tsum, tpre, segs, xstart, xgstart, gmode, groups, tagg,
-        toff, zdst, cE, csrc, tq, v, aux, rep, hard, prm, scal};
+        toff, zdst, cE, csrc, tq, v, aux, rep, hard, fsum, prm, scal};
     for (const void *p : ps)
         if (!p)
             return false;
@@ -2503,13 +2527,14 @@ void Workspace::release()
     if (fixed)
         return;
     void *ps[] = {tsum, tpre, segs, xstart, xgstart, gmode, groups, tagg, toff,
-        zdst, cE, csrc, tq, v, aux, rep, hard, prm, scal};
+        zdst, cE, csrc, tq, v, aux, rep, hard, fsum, prm, scal};
     for (void *p : ps)
         if (p)
             (void)hipFree(p);
     tsum = tpre = xstart = xgstart = v = aux = nullptr;
     segs = nullptr;
     gmode = zdst = cE = csrc = tq = rep = hard = nullptr;
+    fsum = nullptr;
     groups = nullptr;
     tagg = toff = scal = nullptr;
     prm = nullptr;
@@ -2633,6 +2658,23 @@ static int run_count(Workspace &ws, WSrc src, USrc us, long long M, int *rep,
     hipLaunchKernelGGL((k_count_all<WSrc, USrc>), dim3(gt), dim3(kBlock), 0,
         ws.stream, src, us, M, static_cast<const long long *>(nullptr), ws.prm,
         fold_bufs(ws), rep, ws.tagg);
+    VSMCH_HIP(hipGetLastError());
+    return kOk;
+}
+
+// The counts of a particle set's event (fast_event.hpp): log-weights,
+// systematic or stratified draws prepared on the device, M == N.
+template <class USrc>
+static int fast_count(Workspace &ws, WLog wb, USrc us, long long M, int *rep)
+{
+    hipStream_t st = ws.stream;
+    const unsigned int gt = grid_tiles(M);
+    hipLaunchKernelGGL(k_fe_summ<WLog>, dim3(gt), dim3(kBlock), 0, st, wb, M,
+        ws.prm, ws.tpre, ws.segs, ws.fsum);
+    hipLaunchKernelGGL(k_fe_resolve<WLog>, dim3(1), dim3(kBlock), 0, st, wb, M,
+        ws.prm, ws.segs, ws.fsum, ws.groups, ws.xstart, ws.xgstart, ws.gmode);
+    hipLaunchKernelGGL((k_fe_count<WLog, USrc>), dim3(gt), dim3(kBlock), 0, st,
+        wb, us, M, ws.prm, ws.xstart, rep, ws.tagg);
     VSMCH_HIP(hipGetLastError());
     return kOk;
 }
@@ -2847,6 +2889,13 @@ static int scheme_counts(Workspace &ws, int scheme, long long M, long long N,
     int rc;
     if (prepared) {
         // draw parameters and the weights' tile prefix were set on the device
+        if constexpr (std::is_same<Base, WLog>::value) {
+            if (M == N) {
+                if (scheme == kStratified)
+                    return fast_count(ws, wb, UStrat{ws.prm}, M, rep);
+                return fast_count(ws, wb, USys{ws.prm}, M, rep);
+            }
+        }
         if (scheme == kStratified)
             return run_count(ws, wb, UStrat{ws.prm}, M, rep, true);
         return run_count(ws, wb, USys{ws.prm}, M, rep, true);
