@@ -7,13 +7,17 @@
 // (include/vsmc/resample/transform.hpp:61-69) are reproduced exactly, as in
 // fold.hpp (within one binade a sum is an integer multiple of the binade's
 // ulp and adding w adds the rounded step; a sum that leaves the binade is
-// one real FP64 add), by three launches:
-//  * k_fe_summ summarises every tile from the approximate start tpre[t]
-//    (the move's block partials): a tile whose approximate sums stay in
-//    one binade without half-ulp ties by the integer total of its rounded
-//    steps, from registers; tile 0 (sums from 0, across many binades) and
-//    every other tile by the full summary of fold.hpp (runs, their maps and
-//    the crossing weights), in the same block, which overlaps the others.
+// one real FP64 add):
+//  * k_fe_summ summarises every tile from its approximate start tpre[t]
+//    (the move's block partials): a tile whose approximate sums stay in one
+//    binade without half-ulp ties by the integer total of its rounded steps,
+//    from registers; a tile with one crossing of the approximate sums and no
+//    ties by the totals before and after the crossing and the weight that
+//    crosses; any other (tile 0, whose sums start at 0) by the full summary
+//    of fold.hpp (runs, their maps and the crossing weights). Those take a
+//    block tens of microseconds, so the tiles predicted to need them (from
+//    tpre alone, listed by the finalize) run in the grid's first blocks,
+//    beside the others.
 //  * k_fe_resolve (one workgroup): consecutive one-binade tiles form a
 //    segment whose map is the composition of theirs (a segmented block
 //    scan); one thread walks the segments and the other tiles in order,
@@ -27,16 +31,18 @@
 //    is the tile folded exactly by fold.hpp's passes.
 #pragma once
 
-constexpr int kFeSegs = 32;            // full summaries a resolver chunk holds
+constexpr int kFeSegs = 16;            // full summaries a resolver chunk holds
 constexpr int kFePer = 16;             // resolver: tiles per thread per chunk
 constexpr long long kFeChunk = static_cast<long long>(kBlock) * kFePer;
 constexpr int kFeChainMax = 512;       // chain items per chunk
-constexpr int kFeListTag = -100000;    // chain item tags below: listed tiles
+constexpr int kFeListTag = -100000;    // chain item tags below: full summaries
+constexpr int kFeCrossTag = -200000;   // and below these: one-crossing tiles
 
 // Tile summary kinds.
 enum {
     kFeMap = 0,     // sums stay in binade e: map (d0, d1) by the start's parity
     kFeList = 1,    // full summary in segs[t] (fold::apply_segments)
+    kFeCross = 2,   // binade e (total d0) up to one weight, then FeCross
 };
 
 struct FeSum {
@@ -44,6 +50,15 @@ struct FeSum {
     int e;
     long long d0;
     long long d1;
+};
+
+// The rest of a one-crossing tile: the crossing weight wc, then binade e2
+// with total d2.
+struct FeCross {
+    int e2;
+    int pad;
+    long long d2;
+    double wc;
 };
 
 // A tile's weights, element j kBlock + tid in w[j] (0 past n): the
@@ -139,56 +154,138 @@ struct FeSummShared {
     double tab[128];
     double dred[kWaves];
     double dred2[kWaves];
+    int ired[kWaves];
+    double cross[2];             // the sum after the crossing element, its weight
 };
 
+// Tile summaries. Blocks [0, kFePred) take the tiles listed in fpred ([0]
+// the count; none when it exceeds kFePred), block kFePred + t takes tile t
+// unless a first block took it.
 template <class WSrc>
 __global__ void __launch_bounds__(kBlock) k_fe_summ(WSrc src, long long n,
-    const ResampleDev *prm, const double *tpre, TileSeg *segs, FeSum *fs)
+    const ResampleDev *prm, const double *tpre, const int *fpred,
+    TileSeg *segs, FeSum *fs, FeCross *fx)
 {
     if (!prm->active)
         return;
-    const long long t = blockIdx.x;
+    const long long ntiles = ceil_div(n, kTile);
+    const int npred = fpred[0];
+    const bool early = npred <= kFePred;
+    long long t;
+    if (blockIdx.x < kFePred) {
+        if (!early || static_cast<int>(blockIdx.x) >= npred)
+            return;
+        t = fpred[1 + blockIdx.x];
+    } else {
+        t = static_cast<long long>(blockIdx.x) - kFePred;
+        if (t >= ntiles || (early && fold::predicted_crossing(tpre, t, ntiles)))
+            return;
+    }
     const long long base = t * kTile;
-    if (base >= n)
-        return;
     __shared__ FeSummShared sh;
     const int tid = static_cast<int>(threadIdx.x);
     const long long rem = n - base;
     const int nvalid = rem < kTile ? static_cast<int>(rem) : kTile;
     double w[kItems];
-    fe_load_blocked(src.loaded(), base, n, sh.seg.tile, sh.tab, w);
+    fe_load_w(src.loaded(), base, n, sh.tab, w);
     const double x0 = t == 0 ? 0.0 : tpre[t];
     const bool pos = x0 > 0.0 && x0 < INFINITY;
     const FeSteps st(pos ? fold::binade(x0) : 0);
-    double wsum = 0.0, dsum = 0.0;
-    bool tie = false;
+    {
+        double wsum = 0.0, dsum = 0.0;
+        bool tie = false;
 #pragma unroll
-    for (int j = 0; j < kItems; ++j) {
-        const double q = st.q(w[j]);
-        const double r = FeSteps::round(q);
-        tie = tie || FeSteps::half(q, r);
-        wsum += w[j];
-        dsum += r;
-    }
-    const double T = block_reduce(wsum, OpAddD(), sh.dred);
-    const double D = block_reduce(dsum, OpAddD(), sh.dred2);
-    const bool anytie = __syncthreads_or(tie);
-    const double end = x0 + T;
-    if (pos && st.ok && !anytie && end < INFINITY && fold::binade(end) == st.e) {
-        // one binade, no ties: the map adds the integer total of the steps
-        // (saturated at 2^54: such a tile certainly leaves the binade)
-        if (tid == 0) {
-            const long long d = D < static_cast<double>(fold::kCap) ?
-                static_cast<long long>(D) : fold::kCap;
-            segs[t].ncross = 0;
-            segs[t].e[0] = st.e;
-            segs[t].map[0] = FMap{d, d};
-            fs[t] = FeSum{kFeMap, st.e, d, d};
+        for (int j = 0; j < kItems; ++j) {
+            const double q = st.q(w[j]);
+            const double r = FeSteps::round(q);
+            tie = tie || FeSteps::half(q, r);
+            wsum += w[j];
+            dsum += r;
         }
-        return;
+        const double T = block_reduce(wsum, OpAddD(), sh.dred);
+        const double D = block_reduce(dsum, OpAddD(), sh.dred2);
+        const bool anytie = __syncthreads_or(tie);
+        const double end = x0 + T;
+        if (pos && st.ok && !anytie && end < INFINITY && fold::binade(end) == st.e) {
+            // one binade, no ties: the map adds the integer total of the
+            // steps (saturated at 2^54: such a tile certainly leaves it)
+            if (tid == 0) {
+                const long long d = D < static_cast<double>(fold::kCap) ?
+                    static_cast<long long>(D) : fold::kCap;
+                segs[t].ncross = 0;
+                segs[t].e[0] = st.e;
+                segs[t].map[0] = FMap{d, d};
+                fs[t] = FeSum{kFeMap, st.e, d, d};
+            }
+            return;
+        }
     }
-    // the full summary of fold.hpp (tile 0, whose sums start at 0, a tile
-    // with a crossing or a tie, ...), from the tile still in LDS
+    // the tile in blocked order (and in LDS, element k at lds_pad(k))
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+        sh.seg.tile[lds_pad(j * kBlock + tid)] = w[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+        w[j] = sh.seg.tile[lds_pad(tid * kItems + j)];
+    if (pos && st.ok) {
+        // one crossing of the approximate sums: an element whose sum lies
+        // in another binade than the sum before it
+        double loc = 0.0;
+#pragma unroll
+        for (int j = 0; j < kItems; ++j)
+            loc += w[j];
+        double tw;
+        const double ex = block_exclusive_scan(loc, OpAddD(), 0.0, sh.dred, &tw);
+        int ncr = 0;
+        int kc = kTile;
+        double pre = x0 + ex;
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const double post = pre + w[j];
+            const bool c = fold::sum_tag(pre) != fold::sum_tag(post);
+            ncr += c ? 1 : 0;
+            if (c && kc == kTile) {
+                kc = tid * kItems + j;
+                sh.cross[0] = post;
+                sh.cross[1] = w[j];
+            }
+            pre = post;
+        }
+        const int nc_tot = block_reduce(ncr, fold::OpAddI(), sh.ired);
+        const int c = block_reduce(kc, OpMinI(), sh.ired);
+        if (nc_tot == 1 && c < nvalid) {
+            const double after = sh.cross[0];
+            const double wcr = sh.cross[1];
+            const bool aok = after < INFINITY && after > 0.0;
+            const FeSteps st2(aok ? fold::binade(after) : 0);
+            double d1s = 0.0, d2s = 0.0;
+            bool ties = false;
+#pragma unroll
+            for (int j = 0; j < kItems; ++j) {
+                const int k = tid * kItems + j;
+                const double q = k < c ? st.q(w[j]) : st2.q(w[j]);
+                const double r = FeSteps::round(q);
+                ties = ties || (FeSteps::half(q, r) && k != c);
+                d1s += k < c ? r : 0.0;
+                d2s += k > c ? r : 0.0;
+            }
+            const double D1 = block_reduce(d1s, OpAddD(), sh.dred);
+            const double D2 = block_reduce(d2s, OpAddD(), sh.dred2);
+            const bool anyt = __syncthreads_or(ties);
+            if (aok && st2.ok && !anyt && D1 < static_cast<double>(fold::kCap) &&
+                D2 < static_cast<double>(fold::kCap)) {
+                if (tid == 0) {
+                    const long long a = static_cast<long long>(D1);
+                    segs[t].ncross = -1;
+                    fs[t] = FeSum{kFeCross, st.e, a, a};
+                    fx[t] = FeCross{st2.e, 0, static_cast<long long>(D2), wcr};
+                }
+                return;
+            }
+        }
+    }
+    // the full summary, from the tile in LDS
     __syncthreads();
     fold::tile_segments(nvalid, x0, &segs[t], sh.seg);
     __syncthreads();
@@ -201,18 +298,33 @@ __global__ void __launch_bounds__(kBlock) k_fe_summ(WSrc src, long long n,
     }
 }
 
+// A thread's tiles for the resolver's scan: heads and full summaries
+// (packed counts) and the composed map since its last head; combined, the
+// map restarts at a right operand that holds a head.
+struct FeSegScan {
+    int hl;
+    FMap m;
+};
+struct OpFeSegScan {
+    __device__ FeSegScan operator()(FeSegScan a, FeSegScan b) const
+    {
+        return FeSegScan{a.hl + b.hl, (b.hl & 0xFFFF) ? b.m : fold::compose(a.m, b.m)};
+    }
+};
+
 constexpr int kFeThreads = kBlock;     // resolver block (its fallback is kBlock-wide)
 constexpr int kFeWaves = kFeThreads / kWave;
 
 struct FeResolveShared {
     TileSeg seg[kFeSegs];       // full summaries of the chunk's listed tiles
     int segtile[kFeSegs];
-    // the chain: segments of one-binade tiles and the listed tiles, in order
-    int ce[kFeChainMax];        // segment binade, or kFeListTag - slot in seg[]
-    FMap cT[kFeChainMax];       // segment map
+    // the chain: segments of one-binade tiles and the other tiles, in order
+    int ce[kFeChainMax];        // segment binade; listed: kFeListTag - slot;
+                                // one crossing: kFeCrossTag - binade index
+    FMap cT[kFeChainMax];       // segment map, or the crossing tile's d0
+    FeCross cx2[kFeChainMax];   // a crossing tile's second run
     double cx[kFeChainMax];     // exact value before the item
-    fold::SegPair pscan[kFeWaves];
-    int iscan[kFeWaves];
+    FeSegScan pscan[kFeWaves];
     int lastkind[kFeThreads];   // each thread's last tile: its kind, or -1 past the end
     int laste[kFeThreads];
     int firsthead[kFeThreads];  // each thread's first tile starts a chain item
@@ -236,18 +348,72 @@ __device__ inline double fe_from_mant(long long a, int e)
     return fold::from_mant(a, e);
 }
 
+// Advances the exact value x by a run in binade e with map m (false: x is
+// not in that binade, or the run leaves it).
+__device__ inline bool fe_run(double &x, int e, FMap m)
+{
+    if (!(x > 0.0) || !(x < INFINITY) || fold::binade(x) != e)
+        return false;
+    const long long a0 = fold::mant(x);
+    const long long a = a0 + ((a0 & 1) ? m.d1 : m.d0);
+    if (a >= fold::kTwo53)
+        return false;
+    x = fe_from_mant(a, e);
+    return true;
+}
+
+// fold::apply_segments with the runs' data read four at a time (a chain
+// step waits on the previous value, not on LDS).
+__device__ inline bool fe_apply_segments(const TileSeg &sg, double &x)
+{
+    const int nc = sg.ncross;
+    if (nc < 0)
+        return false;
+    for (int r0 = 0; r0 <= nc; r0 += 4) {
+        int e4[4];
+        FMap m4[4];
+        double c4[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int r = r0 + k <= fold::kSegMax ? r0 + k : fold::kSegMax;
+            e4[k] = sg.e[r];
+            m4[k] = sg.map[r];
+            c4[k] = sg.cross[r < fold::kSegMax ? r : fold::kSegMax - 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int r = r0 + k;
+            if (r > nc)
+                return true;
+            if (e4[k] != fold::kEmpty && !fe_run(x, e4[k], m4[k]))
+                return false;
+            if (r < nc)
+                x = x + c4[k];
+        }
+    }
+    return true;
+}
+
 // Exact tile starts from the summaries; see the file comment. One block.
 template <class WSrc>
 __global__ void __launch_bounds__(kFeThreads) k_fe_resolve(WSrc src, long long n,
     const ResampleDev *prm, const TileSeg *segs, const FeSum *fs,
-    GroupSeg *groups, double *xstart, double *xgstart, int *gmode)
+    const FeCross *fx, GroupSeg *groups, double *xstart, double *xgstart,
+    int *gmode)
 {
     __shared__ FeResolveLds u;
     FeResolveShared &sh = u.f;
-    if (!prm->active)
-        return;
     const long long ntiles = ceil_div(n, kTile);
     const int tid = static_cast<int>(threadIdx.x);
+    // the first chunk's summaries load with the event flag
+    FeSum v[kFePer];
+#pragma unroll
+    for (int k = 0; k < kFePer; ++k) {
+        const long long t = static_cast<long long>(tid) * kFePer + k;
+        v[k] = fs[t < ntiles ? t : ntiles - 1];
+    }
+    if (!prm->active)
+        return;
     if (tid == 0) {
         sh.carry = 0.0;
         sh.fail = 0;
@@ -255,6 +421,13 @@ __global__ void __launch_bounds__(kFeThreads) k_fe_resolve(WSrc src, long long n
     __syncthreads();
     for (long long c0 = 0; c0 < ntiles && !sh.fail; c0 += kFeChunk) {
         const long long tb = c0 + static_cast<long long>(tid) * kFePer;
+        if (c0 > 0) {
+#pragma unroll
+            for (int k = 0; k < kFePer; ++k) {
+                const long long t = tb + k;
+                v[k] = fs[t < ntiles ? t : ntiles - 1];
+            }
+        }
         // kind per tile, -1 past the end; a map tile joins the segment of
         // the map tile before it when both lie in one binade
         int kind[kFePer];
@@ -262,11 +435,9 @@ __global__ void __launch_bounds__(kFeThreads) k_fe_resolve(WSrc src, long long n
         FMap mm[kFePer];
 #pragma unroll
         for (int k = 0; k < kFePer; ++k) {
-            const long long t = tb + k;
-            const FeSum v = fs[t < ntiles ? t : ntiles - 1];
-            kind[k] = t >= ntiles ? -1 : v.kind;
-            ee[k] = v.e;
-            mm[k] = FMap{v.d0, v.d1};
+            kind[k] = tb + k >= ntiles ? -1 : v[k].kind;
+            ee[k] = v[k].e;
+            mm[k] = FMap{v[k].d0, v[k].d1};
         }
         sh.lastkind[tid] = kind[kFePer - 1];
         sh.laste[tid] = ee[kFePer - 1];
@@ -275,7 +446,7 @@ __global__ void __launch_bounds__(kFeThreads) k_fe_resolve(WSrc src, long long n
         int pe = tid == 0 ? 0 : sh.laste[tid - 1];
         unsigned head = 0;
         int nhead = 0, nlist = 0;
-        fold::SegPair agg{0, fold::ident()};
+        FMap agg = fold::ident();
 #pragma unroll
         for (int k = 0; k < kFePer; ++k) {
             bool h = false;
@@ -286,20 +457,18 @@ __global__ void __launch_bounds__(kFeThreads) k_fe_resolve(WSrc src, long long n
             head |= h ? (1u << k) : 0u;
             nhead += h ? 1 : 0;
             nlist += kind[k] == kFeList ? 1 : 0;
-            agg = fold::OpSeg()(agg, fold::SegPair{h ? 1LL : 0LL,
-                kind[k] == kFeMap ? mm[k] : fold::ident()});
+            const FMap mk = kind[k] == kFeMap ? mm[k] : fold::ident();
+            agg = h ? mk : fold::compose(agg, mk);
             pk = kind[k];
             pe = ee[k];
         }
-        // one scan for both counts (each at most kFeChunk < 2^15)
-        int tot_hl;
-        const int hl = block_exclusive_scan_n<kFeThreads>(nhead | (nlist << 16),
-            fold::OpAddI(), 0, sh.iscan, &tot_hl);
-        const int hrank = hl & 0xFFFF, lrank = hl >> 16;
-        const int tot_head = tot_hl & 0xFFFF, tot_list = tot_hl >> 16;
-        fold::SegPair stot;
-        const fold::SegPair run = block_exclusive_scan_n<kFeThreads>(agg, fold::OpSeg(),
-            fold::SegPair{0, fold::ident()}, sh.pscan, &stot);
+        // one scan: the segmented maps and both counts (each below 2^15)
+        FeSegScan stot;
+        const FeSegScan sc = block_exclusive_scan_n<kFeThreads>(
+            FeSegScan{nhead | (nlist << 16), agg}, OpFeSegScan(),
+            FeSegScan{0, fold::ident()}, sh.pscan, &stot);
+        const int hrank = sc.hl & 0xFFFF, lrank = sc.hl >> 16;
+        const int tot_head = stot.hl & 0xFFFF, tot_list = stot.hl >> 16;
         if (tot_head > kFeChainMax || tot_list > kFeSegs) {
             if (tid == 0)
                 sh.fail = 1;
@@ -311,20 +480,24 @@ __global__ void __launch_bounds__(kFeThreads) k_fe_resolve(WSrc src, long long n
         {
             int r = hrank - 1;   // chain item of the current tile
             int li = lrank;
-            fold::SegPair acc = run;
+            FMap acc = sc.m;
 #pragma unroll
             for (int k = 0; k < kFePer; ++k) {
                 if (kind[k] < 0)
                     break;
                 const bool h = (head >> k) & 1u;
-                acc = fold::OpSeg()(acc, fold::SegPair{h ? 1LL : 0LL,
-                    kind[k] == kFeMap ? mm[k] : fold::ident()});
+                const FMap mk = kind[k] == kFeMap ? mm[k] : fold::ident();
+                acc = h ? mk : fold::compose(acc, mk);
                 if (h) {
                     ++r;
                     if (kind[k] == kFeList) {
                         sh.ce[r] = kFeListTag - li;
                         sh.segtile[li] = static_cast<int>(tb + k);
                         ++li;
+                    } else if (kind[k] == kFeCross) {
+                        sh.ce[r] = kFeCrossTag - (ee[k] + 1100);
+                        sh.cT[r] = mm[k];
+                        sh.cx2[r] = fx[tb + k];
                     } else {
                         sh.ce[r] = ee[k];
                     }
@@ -335,7 +508,7 @@ __global__ void __launch_bounds__(kFeThreads) k_fe_resolve(WSrc src, long long n
                 else
                     next_head = tid + 1 == kFeThreads || sh.firsthead[tid + 1] != 0;
                 if (kind[k] == kFeMap && next_head)
-                    sh.cT[r] = acc.m;
+                    sh.cT[r] = acc;
             }
         }
         __syncthreads();
@@ -360,27 +533,24 @@ __global__ void __launch_bounds__(kFeThreads) k_fe_resolve(WSrc src, long long n
                 const bool final_item = last_chunk && r == tot_head - 1;
                 sh.cx[r] = x;
                 const int ce = sh.ce[r];
+                bool good;
                 if (ce > kFeListTag) {
-                    // a segment in binade ce
-                    const FMap T = sh.cT[r];
-                    if (!(x > 0.0) || !(x < INFINITY) || fold::binade(x) != ce) {
-                        ok = 0;
-                        break;
-                    }
-                    const long long a0 = fold::mant(x);
-                    const long long a = a0 + ((a0 & 1) ? T.d1 : T.d0);
-                    if (a >= fold::kTwo53) {
-                        if (!final_item)
-                            ok = 0;
-                        break;
-                    }
-                    x = fe_from_mant(a, ce);
+                    good = fe_run(x, ce, sh.cT[r]);
+                } else if (ce > kFeCrossTag) {
+                    good = fe_apply_segments(sh.seg[kFeListTag - ce], x);
                 } else {
-                    if (!fold::apply_segments(sh.seg[kFeListTag - ce], x)) {
-                        if (!final_item)
-                            ok = 0;
-                        break;
+                    // one crossing: a run, the real add, a run
+                    const FeCross c2 = sh.cx2[r];
+                    good = fe_run(x, kFeCrossTag - ce - 1100, sh.cT[r]);
+                    if (good) {
+                        x = x + c2.wc;
+                        good = fe_run(x, c2.e2, FMap{c2.d2, c2.d2});
                     }
+                }
+                if (!good) {
+                    if (!final_item)
+                        ok = 0;
+                    break;
                 }
             }
             sh.carry = x;
@@ -399,7 +569,7 @@ __global__ void __launch_bounds__(kFeThreads) k_fe_resolve(WSrc src, long long n
             bool bad = false;
             if (!((head & 1u) != 0) && kind[0] == kFeMap) {
                 // continues the previous thread's segment
-                P = run.m;
+                P = sc.m;
                 xs = sh.cx[r];
                 a0 = fold::mant(xs);
                 e = ee[0];

@@ -692,6 +692,18 @@ __device__ __forceinline__ void tile_exact(double *t, double x0, ExactShared &sh
     }
 }
 
+// Whether tile t (of ntiles) of an approximate tile prefix tpre is
+// predicted to need more than a one-binade summary: tile 0 (sums from 0),
+// the last tile (its end, the total, lies at a binade boundary), or
+// approximate start and end in different binades.
+__device__ inline bool predicted_crossing(const double *tpre, long long t,
+    long long ntiles)
+{
+    if (t == 0 || t + 1 >= ntiles)
+        return true;
+    return sum_tag(tpre[t]) != sum_tag(tpre[t + 1]);
+}
+
 struct SegShared {
     double tile[kTileLds];
     double dscan[kWaves];

@@ -81,6 +81,9 @@ struct Workspace {
     int *rep = nullptr;          // scratch counts
     int *hard = nullptr;         // batched filters: [0] count, then tiles to fold in full
     struct FeSum *fsum = nullptr;  // dense per-tile summaries (particle-set events)
+    struct FeCross *fcross = nullptr;  // their one-crossing remainders
+    int *fpred = nullptr;        // [0] count, then the tiles a particle set's
+                                 // finalize predicts to cross a binade
     int *integ_dummy = nullptr;
     ResampleDev *prm = nullptr;
     long long *scal = nullptr;   // small device scalars
@@ -116,6 +119,10 @@ struct GroupSeg {
 };
 
 constexpr int kGroup = 64;
+
+// A particle set's event summarises first the tiles its finalize predicts
+// to cross a binade (fold::predicted_crossing), at most this many.
+constexpr int kFePred = 64;
 
 // Status codes (mirrors vsmch_status in include/vsmch.h)
 enum {

@@ -331,9 +331,31 @@ __device__ inline int decide_event(PDev *pd, ResampleDev *prm, long long n,
     return active;
 }
 
+// After a block has written the approximate tile prefix tpre[0, nb): the
+// tiles predicted to cross a binade (fold::predicted_crossing) into fpred
+// ([0] their count, which may exceed the kFePred entries), in any order.
+__device__ inline void list_predicted(const double *tpre, long long nb, int *fpred,
+    int *cnt_lds)
+{
+    if (threadIdx.x == 0)
+        *cnt_lds = 0;
+    __syncthreads();   // tpre complete for the block
+    for (long long t = threadIdx.x; t < nb; t += blockDim.x) {
+        if (fold::predicted_crossing(tpre, t, nb)) {
+            const int k = atomicAdd(cnt_lds, 1);
+            if (k < kFePred)
+                fpred[1 + k] = static_cast<int>(t);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        fpred[0] = *cnt_lds;
+}
+
 __global__ void __launch_bounds__(kMergeThreads) k_prepare(PDev *pd, ResampleDev *prm,
     const Part3 *part, long long nb, long long n, double threshold,
-    long long t, double *ess_hist, int *rs_hist, int scheme, double *tpre)
+    long long t, double *ess_hist, int *rs_hist, int scheme, double *tpre,
+    int *fpred)
 {
     __shared__ int act;
     __shared__ double lds[kMergeThreads / kWave];
@@ -362,6 +384,8 @@ __global__ void __launch_bounds__(kMergeThreads) k_prepare(PDev *pd, ResampleDev
             return q.s > 0.0 ? q.s * exp(q.m - lmax) * inv : 0.0;
         },
         [&](long long b, double ex) { tpre[b] = ex; }, OpAddD(), 0.0, lds);
+    __shared__ int npred;
+    list_predicted(tpre, nb, fpred, &npred);
 }
 
 // A filter step's bookkeeping after its move, one block of kMergeThreads:
@@ -377,7 +401,7 @@ __global__ void __launch_bounds__(kMergeThreads) k_step_finalize(
     const Part3 *part, long long nb, long long n, int mode, PDev *pd,
     ResampleDev *prm, double threshold, long long h, double *ess_hist,
     int *rs_hist, double *logz_hist, const double *mpart, double *pos_prev,
-    int scheme, int prep, double *tpre)
+    int scheme, int prep, double *tpre, int *fpred)
 {
     __shared__ int act;
     __shared__ int sh_equal;
@@ -509,18 +533,20 @@ __global__ void __launch_bounds__(kMergeThreads) k_step_finalize(
                 tpre[b0 + k] = run;
             run += v[k];
         }
-        return;
+    } else {
+        block_scan_chunked_n<kMergeThreads, 4>(nb,
+            [&](long long b) {
+                if (equal) {
+                    const long long rem = n - b * kPTile;
+                    return static_cast<double>(rem < kPTile ? rem : kPTile) * inv_n;
+                }
+                const Part3 q = part[b];
+                return q.s > 0.0 ? q.s * exp(q.m - lmax) * inv : 0.0;
+            },
+            [&](long long b, double ex) { tpre[b] = ex; }, OpAddD(), 0.0, lds);
     }
-    block_scan_chunked_n<kMergeThreads, 4>(nb,
-        [&](long long b) {
-            if (equal) {
-                const long long rem = n - b * kPTile;
-                return static_cast<double>(rem < kPTile ? rem : kPTile) * inv_n;
-            }
-            const Part3 q = part[b];
-            return q.s > 0.0 ? q.s * exp(q.m - lmax) * inv : 0.0;
-        },
-        [&](long long b, double ex) { tpre[b] = ex; }, OpAddD(), 0.0, lds);
+    __shared__ int npred;
+    list_predicted(tpre, nb, fpred, &npred);
 }
 
 // The state after the last move of cv_iterate, into the particle set's own
@@ -1509,7 +1535,8 @@ int resample_step(ParticleSet *p, int scheme, double threshold, long long t,
     if (prep)
         hipLaunchKernelGGL(k_prepare, dim3(1), dim3(kMergeThreads), 0, st, p->pd,
             p->ws.prm, static_cast<const Part3 *>(p->part), pblocks(p->n),
-            p->n, threshold, t, ess_hist, rs_hist, scheme, p->ws.tpre);
+            p->n, threshold, t, ess_hist, rs_hist, scheme, p->ws.tpre,
+            p->ws.fpred);
     else
         hipLaunchKernelGGL(k_decide, dim3(1), dim3(1), 0, st, p->pd, p->ws.prm,
             p->n, threshold, t, ess_hist, rs_hist);
@@ -1803,7 +1830,7 @@ static int cv_iterate_impl(ParticleSet *p, const CvModelDev &m, long long t0,
             p->ws.prm, threshold, s, ess_d, rs_d, logz_d,
             static_cast<const double *>(p->mpart),
             mon ? pos_d + 2 * (s - 1) : nullptr, scheme, prep ? 1 : 0,
-            p->ws.tpre);
+            p->ws.tpre, p->ws.fpred);
         VSMCH_HIP(hipGetLastError());
         const int rc = resample_event(p, scheme, prep, nullptr, nullptr, true);
         if (rc != kOk)

@@ -1722,7 +1722,7 @@ __global__ void __launch_bounds__(kScanThreads) k_ridx_prefix(const long long *t
     __shared__ int zp[kPrefixLdsTiles + 1];   // childless prefix
     __shared__ int ep[kPrefixLdsTiles + 1];   // extras prefix
     __shared__ int cp[kPrefixLdsTiles + 1];   // sources prefix
-    const bool in_lds = ntiles <= kPrefixLdsTiles;
+    const bool in_lds = tq != nullptr && ntiles <= kPrefixLdsTiles;
     I3 carry{0, 0, 0};
     for (long long c0 = 0; c0 < ntiles; c0 += static_cast<long long>(kScanThreads) * kPer) {
         const long long b0 = c0 + static_cast<long long>(threadIdx.x) * kPer;
@@ -2500,11 +2500,20 @@ bool Workspace::ensure(long long n)
     A(&rep, sizeof(int) * nn);
     A(&hard, sizeof(int) * (tiles + 1));
     A(&fsum, sizeof(FeSum) * tiles);
+    A(&fcross, sizeof(FeCross) * tiles);
+    A(&fpred, sizeof(int) * (tiles + 1));
     A(&prm, sizeof(ResampleDev));
     A(&scal, sizeof(long long) * 16);
     if (!ok || !all_allocated()) {
         release();
         set_error("workspace allocation failed");
+        return false;
+    }
+    // no predicted tiles until a finalize lists them (a count above the
+    // list's capacity: every tile in order)
+    if (hipMemset(fpred, 0x7F, sizeof(int)) != hipSuccess) {
+        release();
+        set_error("workspace initialisation failed");
         return false;
     }
     cap_n = nn;
@@ -2515,7 +2524,7 @@ bool Workspace::ensure(long long n)
 bool Workspace::all_allocated() const
 {
     const void *ps[] = {tsum, tpre, segs, xstart, xgstart, gmode, groups, tagg,
-        toff, zdst, cE, csrc, tq, v, aux, rep, hard, fsum, prm, scal};
+        toff, zdst, cE, csrc, tq, v, aux, rep, hard, fsum, fcross, fpred, prm, scal};
     for (const void *p : ps)
         if (!p)
             return false;
@@ -2527,7 +2536,7 @@ void Workspace::release()
     if (fixed)
         return;
     void *ps[] = {tsum, tpre, segs, xstart, xgstart, gmode, groups, tagg, toff,
-        zdst, cE, csrc, tq, v, aux, rep, hard, fsum, prm, scal};
+        zdst, cE, csrc, tq, v, aux, rep, hard, fsum, fcross, fpred, prm, scal};
     for (void *p : ps)
         if (p)
             (void)hipFree(p);
@@ -2535,6 +2544,8 @@ void Workspace::release()
     segs = nullptr;
     gmode = zdst = cE = csrc = tq = rep = hard = nullptr;
     fsum = nullptr;
+    fcross = nullptr;
+    fpred = nullptr;
     groups = nullptr;
     tagg = toff = scal = nullptr;
     prm = nullptr;
@@ -2663,16 +2674,18 @@ static int run_count(Workspace &ws, WSrc src, USrc us, long long M, int *rep,
 }
 
 // The counts of a particle set's event (fast_event.hpp): log-weights,
-// systematic or stratified draws prepared on the device, M == N.
+// systematic or stratified draws prepared on the device (ws.tpre and
+// ws.fpred written by the finalize), M == N.
 template <class USrc>
 static int fast_count(Workspace &ws, WLog wb, USrc us, long long M, int *rep)
 {
     hipStream_t st = ws.stream;
     const unsigned int gt = grid_tiles(M);
-    hipLaunchKernelGGL(k_fe_summ<WLog>, dim3(gt), dim3(kBlock), 0, st, wb, M,
-        ws.prm, ws.tpre, ws.segs, ws.fsum);
-    hipLaunchKernelGGL(k_fe_resolve<WLog>, dim3(1), dim3(kBlock), 0, st, wb, M,
-        ws.prm, ws.segs, ws.fsum, ws.groups, ws.xstart, ws.xgstart, ws.gmode);
+    hipLaunchKernelGGL(k_fe_summ<WLog>, dim3(kFePred + gt), dim3(kBlock), 0, st, wb,
+        M, ws.prm, ws.tpre, ws.fpred, ws.segs, ws.fsum, ws.fcross);
+    hipLaunchKernelGGL(k_fe_resolve<WLog>, dim3(1), dim3(kFeThreads), 0, st, wb, M,
+        ws.prm, ws.segs, ws.fsum, ws.fcross, ws.groups, ws.xstart, ws.xgstart,
+        ws.gmode);
     hipLaunchKernelGGL((k_fe_count<WLog, USrc>), dim3(gt), dim3(kBlock), 0, st,
         wb, us, M, ws.prm, ws.xstart, rep, ws.tagg);
     VSMCH_HIP(hipGetLastError());
@@ -3038,9 +3051,11 @@ int trans_rep_index(Workspace &ws, long long M, long long N, const int *rep,
     // M == N: the prefixes (and each tile's slice of sources), the
     // compacted sources, then every tile's indices (and gathered rows) in
     // one pass; its block 0 ends the event
-    const bool use_tq = ntiles <= kPrefixLdsTiles;
+    // the tiles find their slice of sources themselves (two wave searches
+    // over the prefixes, in flight with their rep loads)
+    const bool use_tq = false;
     hipLaunchKernelGGL(k_ridx_prefix, dim3(1), dim3(kScanThreads), 0, st,
-        ws.tagg, ntiles, prm, ws.toff, ws.tq);
+        ws.tagg, ntiles, prm, ws.toff, static_cast<int *>(nullptr));
     hipLaunchKernelGGL(k_ridx_sources, dim3(ntiles), dim3(kBlock), 0, st, rep,
         M, prm, ws.toff, ws.cE, ws.csrc);
     const int *tq = use_tq ? ws.tq : nullptr;
