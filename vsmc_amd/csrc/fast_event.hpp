@@ -678,19 +678,6 @@ template <class U> struct FeCounter;
 template <> struct FeCounter<USys> { using type = FeSys; };
 template <> struct FeCounter<UStrat> { using type = FeStrat; };
 
-struct FeAgg {
-    int e;
-    int z;
-    int c;
-    int pad;
-};
-struct OpFeAgg {
-    __device__ FeAgg operator()(FeAgg a, FeAgg b) const
-    {
-        return FeAgg{a.e + b.e, a.z + b.z, a.c + b.c, 0};
-    }
-};
-
 // Relative bound between an FP64 block-scan prefix of a tile (exact start
 // plus a blocked scan: at most 16 + 6 + 3 + 18 roundings along any path)
 // and the sequential fold over at most 4096 weights from the same exact
@@ -703,15 +690,16 @@ struct FeCountShared {
     fold::ExactShared ex;
     double tab[128];
     double dlds[kWaves];
-    FeAgg alds[kWaves];
+    long long lscan[kWaves];
 };
 
 // Counts of every tile from its exact start xstart[t] (see the file
-// comment); rep -> index aggregates (childless, extras, sources) per tile.
+// comment); rep -> index aggregates (childless, extras, sources) per tile
+// and the tile's sources (csrc, cE) in its own region.
 template <class WSrc, class USrc>
 __global__ void __launch_bounds__(kBlock, 4) k_fe_count(WSrc src, USrc us,
     long long M, const ResampleDev *prm, double *xstart, int *rep,
-    long long *tagg)
+    long long *tagg, int *csrc, int *cE)
 {
     if (!prm->active)
         return;
@@ -814,17 +802,12 @@ __global__ void __launch_bounds__(kBlock, 4) k_fe_count(WSrc src, USrc us,
     }
     // counts: rep_k = G(s_k) - G(s_{k-1}); G before particle 0 is 0, and the
     // last particle takes every remaining draw
-    FeAgg a{0, 0, 0, 0};
     int r[kItems];
     if (!last_tile) {
 #pragma unroll
         for (int j = 0; j < kItems; ++j) {
-            const int rr = g[j] - gp;
-            r[j] = rr;
+            r[j] = g[j] - gp;
             gp = g[j];
-            a.z += rr == 0;
-            a.e += rr > 1 ? rr - 1 : 0;
-            a.c += rr > 1;
         }
     } else {
 #pragma unroll
@@ -833,14 +816,8 @@ __global__ void __launch_bounds__(kBlock, 4) k_fe_count(WSrc src, USrc us,
             int gk = gp;
             if (k < nvalid)
                 gk = (base + k == M - 1) ? nd : g[j];
-            const int rr = gk - gp;
-            r[j] = rr;
+            r[j] = gk - gp;
             gp = gk;
-            if (k < nvalid) {
-                a.z += rr == 0;
-                a.e += rr > 1 ? rr - 1 : 0;
-                a.c += rr > 1;
-            }
         }
     }
     if (k0 + kItems <= nvalid && (reinterpret_cast<uintptr_t>(rep) & 15) == 0) {
@@ -854,10 +831,45 @@ __global__ void __launch_bounds__(kBlock, 4) k_fe_count(WSrc src, USrc us,
             if (k0 + j < nvalid)
                 rep[base + k0 + j] = r[j];
     }
-    const FeAgg tot = block_reduce(a, OpFeAgg(), sh.alds);
+    // rep -> index: the tile's childless slots, extra copies and sources
+    // (per-tile aggregates), and its sources with their exclusive prefix of
+    // extra copies within the tile, in the tile's region of csrc / cE
+    // (elements tidx kTile ..), staged in LDS so the writes are coalesced
+    long long loc = 0;   // extras << 32 | childless << 16 | sources
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        if (k0 + j < nvalid) {
+            const int rr = r[j];
+            loc += rr == 0 ? (1LL << 16) : 0;
+            loc += rr > 1 ? ((static_cast<long long>(rr - 1) << 32) | 1) : 0;
+        }
+    }
+    long long tot;
+    const long long ex = block_exclusive_scan(loc, OpAddL(), 0LL, sh.lscan, &tot);
+    const int nsrc = static_cast<int>(tot & 0xFFFF);
+    int *const ls = reinterpret_cast<int *>(sh.tile);        // sources
+    int *const le = ls + kTile;                              // extras before each
+    {
+        int cr = static_cast<int>(ex & 0xFFFF);
+        int er = static_cast<int>(ex >> 32);
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            if (k0 + j < nvalid && r[j] > 1) {
+                ls[cr] = static_cast<int>(base + k0 + j);
+                le[cr] = er;
+                ++cr;
+                er += r[j] - 1;
+            }
+        }
+    }
+    __syncthreads();
+    for (int q = tid; q < nsrc; q += kBlock) {
+        csrc[base + q] = ls[q];
+        cE[base + q] = le[q];
+    }
     if (tid == 0) {
-        tagg[3 * t + 0] = tot.z;
-        tagg[3 * t + 1] = tot.e;
-        tagg[3 * t + 2] = tot.c;
+        tagg[3 * t + 0] = (tot >> 16) & 0xFFFF;
+        tagg[3 * t + 1] = tot >> 32;
+        tagg[3 * t + 2] = nsrc;
     }
 }

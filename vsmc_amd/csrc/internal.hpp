@@ -176,10 +176,12 @@ int trans_u01_rep(Workspace &ws, long long M, long long N, const double *W,
 // force_expand: lay out the expanded list (src repeated rep[src] times)
 // even when M == N (resample_trans_u01_index). agg_ready: the per-tile
 // aggregates in ws.tagg were produced by the counting pass that wrote rep.
+// tile_sources: (M == N, agg_ready) the counting pass also left every
+// tile's sources in its own region of ws.csrc / ws.cE (the fast event).
 int trans_rep_index(Workspace &ws, long long M, long long N, const int *rep,
     int *idx, void *state, int dim, long long ld, int elem_bytes,
     bool device_active, bool force_expand = false, bool agg_ready = false,
-    const EventEnd &end = EventEnd());
+    const EventEnd &end = EventEnd(), bool tile_sources = false);
 
 int trans_index_rep(Workspace &ws, long long M, long long N, const int *idx,
     int *rep);

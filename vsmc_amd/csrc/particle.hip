@@ -1577,9 +1577,11 @@ static int resample_event(ParticleSet *p, int scheme, bool prep, int *rep_out,
     end.scheme = scheme;
     end.words = prep ? 1 : 0;
     end.deferred = defer ? 1 : 0;
+    // a prepared event counted with the fast event, which leaves every
+    // tile's sources in its own region
     return trans_rep_index(p->ws, p->n, p->n, rep, defer ? p->anc : idx_out,
         defer ? nullptr : p->state, p->dim, p->n, p->bytes, true, false,
-        p->n >= 2, end);
+        p->n >= 2, end, prep);
 }
 
 // r_d = sum_i W_i buf[i * dim + d0 + d] for 8 components at a time.

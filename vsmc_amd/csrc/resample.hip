@@ -1880,7 +1880,10 @@ __global__ void __launch_bounds__(kBlock) k_ridx_sources(const int *rep,
 // With `state`, the childless slots' rows are gathered in place (sources
 // always have copies, so they are never written). Block 0 ends the event
 // when end.pd is set; nothing in this kernel reads what that writes.
-template <typename E>
+// kReg: the sources come per tile (csrc / cE hold tile u's sources and
+// their extras prefix within the tile at u kTile .., written by the fast
+// event's counting pass) instead of compacted over all tiles.
+template <typename E, bool kReg = false>
 __global__ void __launch_bounds__(kBlock) k_ridx_fill(const ResampleDev *prm,
     const int *rep, long long M, const long long *toff, const int *tq,
     long long ntiles, const int *cE, const int *csrc, int *idx, E *state,
@@ -1926,7 +1929,7 @@ __global__ void __launch_bounds__(kBlock) k_ridx_fill(const ResampleDev *prm,
         }
     }
     const int wave = static_cast<int>(threadIdx.x) / kWave;
-    if (tq) {
+    if (!kReg && tq) {
         if (threadIdx.x < 2)
             qrange[threadIdx.x] = tq[2 * tidx + threadIdx.x];
     } else if (any && wave < 2) {
@@ -1936,8 +1939,12 @@ __global__ void __launch_bounds__(kBlock) k_ridx_fill(const ResampleDev *prm,
         long long sx = wave_last_le([&](long long t) { return toff[3 * t + 1]; },
             ntiles, x);
         sx = sx < 0 ? 0 : sx;
-        if ((threadIdx.x & (kWave - 1)) == 0)
-            qrange[wave] = wave == 0 ? toff[3 * sx + 2] : toff[3 * (sx + 1) + 2];
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+            if (kReg)
+                qrange[wave] = sx;
+            else
+                qrange[wave] = wave == 0 ? toff[3 * sx + 2] : toff[3 * (sx + 1) + 2];
+        }
     }
     __syncthreads();
     // blocked order: thread t owns elements t kItems .. t kItems + kItems - 1;
@@ -1968,7 +1975,40 @@ __global__ void __launch_bounds__(kBlock) k_ridx_fill(const ResampleDev *prm,
         // 0, by a maximum. All loads of a thread in flight at once, a tile's
         // worth of sources per pass.
         int first = -1;
-        for (long long c0 = q0; c0 < q1; c0 += kTile) {
+        if (kReg) {
+            // source tiles q0 .. qrange[1]; each rank holds its source's
+            // index itself
+            for (long long u = q0; u <= qrange[1]; ++u) {
+                const int *const cEu = cE + u * kTile;
+                const int *const csu = csrc + u * kTile;
+                const long long Eu = toff[3 * u + 1];
+                const int nq = static_cast<int>(toff[3 * (u + 1) + 2] - toff[3 * u + 2]);
+                if (nq <= 0)
+                    continue;
+                int e[kItems], sid[kItems];
+#pragma unroll
+                for (int j = 0; j < kItems; ++j) {
+                    const int o = j * kBlock + static_cast<int>(threadIdx.x);
+                    const unsigned oc = static_cast<unsigned>(o < nq ? o : nq - 1);
+                    e[j] = cEu[oc];
+                    sid[j] = csu[oc];
+                }
+#pragma unroll
+                for (int j = 0; j < kItems; ++j) {
+                    const int o = j * kBlock + static_cast<int>(threadIdx.x);
+                    keep_loaded(e[j]);
+                    keep_loaded(sid[j]);
+                    if (o < nq) {
+                        const long long pq = Eu + e[j] - Z;
+                        if (pq <= 0)
+                            first = sid[j] > first ? sid[j] : first;
+                        else if (pq < ztot)
+                            lds[lds_pad(static_cast<int>(pq))] = sid[j];
+                    }
+                }
+            }
+        }
+        for (long long c0 = q0; c0 < (kReg ? q0 : q1); c0 += kTile) {
             const int *const cEc = cE + c0;
             const int nq = q1 - c0 < kTile ? static_cast<int>(q1 - c0) : kTile;
             int e[kItems];
@@ -2008,8 +2048,12 @@ __global__ void __launch_bounds__(kBlock) k_ridx_fill(const ResampleDev *prm,
         run = block_exclusive_scan(run, OpMaxI(), -1, ired, &tot);
         if (threadIdx.x == 0) {
             int q = lds[lds_pad(0)];
-            q = q < 0 ? 0 : (q >= C ? static_cast<int>(C - 1) : q);
-            src0s = csrc[static_cast<unsigned>(q)];
+            if (kReg)
+                src0s = q < 0 ? static_cast<int>(base) : q;
+            else {
+                q = q < 0 ? 0 : (q >= C ? static_cast<int>(C - 1) : q);
+                src0s = csrc[static_cast<unsigned>(q)];
+            }
         }
 #pragma unroll
         for (int j = 0; j < kItems; ++j) {
@@ -2026,8 +2070,13 @@ __global__ void __launch_bounds__(kBlock) k_ridx_fill(const ResampleDev *prm,
         for (int j = 0; j < kItems; ++j) {
             const bool c = (cmask >> j) & 1u;
             int q = lds[lds_pad(zr < ztot ? zr : ztot - 1)];
-            q = q < 0 ? 0 : (q >= C ? static_cast<int>(C - 1) : q);
-            sv[j] = csrc[static_cast<unsigned>(q)];
+            if (kReg) {
+                // (a rank always has a source for counts that sum to M)
+                sv[j] = q < 0 ? static_cast<int>(base + kb + j) : q;
+            } else {
+                q = q < 0 ? 0 : (q >= C ? static_cast<int>(C - 1) : q);
+                sv[j] = csrc[static_cast<unsigned>(q)];
+            }
             zr += c;
         }
 #pragma unroll
@@ -2687,7 +2736,7 @@ static int fast_count(Workspace &ws, WLog wb, USrc us, long long M, int *rep)
         ws.prm, ws.segs, ws.fsum, ws.fcross, ws.groups, ws.xstart, ws.xgstart,
         ws.gmode);
     hipLaunchKernelGGL((k_fe_count<WLog, USrc>), dim3(gt), dim3(kBlock), 0, st,
-        wb, us, M, ws.prm, ws.xstart, rep, ws.tagg);
+        wb, us, M, ws.prm, ws.xstart, rep, ws.tagg, ws.csrc, ws.cE);
     VSMCH_HIP(hipGetLastError());
     return kOk;
 }
@@ -3019,7 +3068,8 @@ int read_words_used(Workspace &ws, unsigned long long *out)
 
 int trans_rep_index(Workspace &ws, long long M, long long N, const int *rep,
     int *idx, void *state, int dim, long long ld, int elem_bytes,
-    bool device_active, bool force_expand, bool agg_ready, const EventEnd &end)
+    bool device_active, bool force_expand, bool agg_ready, const EventEnd &end,
+    bool tile_sources)
 {
     if (M <= 0 || N <= 0)
         return kOk;
@@ -3053,12 +3103,24 @@ int trans_rep_index(Workspace &ws, long long M, long long N, const int *rep,
     // one pass; its block 0 ends the event
     // the tiles find their slice of sources themselves (two wave searches
     // over the prefixes, in flight with their rep loads)
-    const bool use_tq = false;
     hipLaunchKernelGGL(k_ridx_prefix, dim3(1), dim3(kScanThreads), 0, st,
         ws.tagg, ntiles, prm, ws.toff, static_cast<int *>(nullptr));
+    const int *tq = nullptr;
+    if (tile_sources && agg_ready) {
+        if (state == nullptr || elem_bytes == 4) {
+            hipLaunchKernelGGL((k_ridx_fill<float, true>), dim3(ntiles), dim3(kBlock), 0,
+                st, prm, rep, M, ws.toff, tq, ntiles, ws.cE, ws.csrc, idx,
+                static_cast<float *>(state), dim, ld, ws.prm, end);
+        } else {
+            hipLaunchKernelGGL((k_ridx_fill<double, true>), dim3(ntiles), dim3(kBlock),
+                0, st, prm, rep, M, ws.toff, tq, ntiles, ws.cE, ws.csrc, idx,
+                static_cast<double *>(state), dim, ld, ws.prm, end);
+        }
+        VSMCH_HIP(hipGetLastError());
+        return kOk;
+    }
     hipLaunchKernelGGL(k_ridx_sources, dim3(ntiles), dim3(kBlock), 0, st, rep,
         M, prm, ws.toff, ws.cE, ws.csrc);
-    const int *tq = use_tq ? ws.tq : nullptr;
     if (state == nullptr || elem_bytes == 4) {
         hipLaunchKernelGGL((k_ridx_fill<float>), dim3(ntiles), dim3(kBlock), 0,
             st, prm, rep, M, ws.toff, tq, ntiles, ws.cE, ws.csrc, idx,
