@@ -24,12 +24,6 @@ struct u32x4 {
     uint32_t v[4];
 };
 
-// High half of the 32x32 product (v_mul_hi_u32 on the device).
-__host__ __device__ inline uint32_t mulhi32(uint32_t a, uint32_t b)
-{
-    return static_cast<uint32_t>((static_cast<uint64_t>(a) * b) >> 32);
-}
-
 __host__ __device__ inline u32x4 philox4x32_10(
     uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
     uint32_t k1)
@@ -38,14 +32,14 @@ __host__ __device__ inline u32x4 philox4x32_10(
     const uint32_t M1 = 0xCD9E8D57u;
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        const uint32_t hi0 = mulhi32(c2, M1) ^ k0;
-        const uint32_t lo1 = c2 * M1;
-        const uint32_t hi2 = mulhi32(c0, M0) ^ k1;
-        const uint32_t lo3 = c0 * M0;
-        c0 = hi0 ^ c1;
-        c1 = lo1;
-        c2 = hi2 ^ c3;
-        c3 = lo3;
+        // each product whole (one v_mad_u64_u32 on the device instead of a
+        // v_mul_hi_u32 and a v_mul_lo_u32)
+        const uint64_t p1 = static_cast<uint64_t>(c2) * M1;
+        const uint64_t p0 = static_cast<uint64_t>(c0) * M0;
+        c0 = static_cast<uint32_t>(p1 >> 32) ^ k0 ^ c1;
+        c1 = static_cast<uint32_t>(p1);
+        c2 = static_cast<uint32_t>(p0 >> 32) ^ k1 ^ c3;
+        c3 = static_cast<uint32_t>(p0);
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }

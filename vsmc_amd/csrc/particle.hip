@@ -689,6 +689,24 @@ struct alignas(4 * sizeof(Real)) Vec4 {
     Real v[4];
 };
 
+// Four log-weights stored nontemporally: the move's weights are next read
+// by the resampling passes only, while its state, stored normally, stays in
+// the last-level cache for the next move's gather through the ancestors
+// (measured at N = 2^24: both kinds nontemporal speed the plain move by
+// 9 %, but slow the move after an event by 18 %). Two 16-byte halves, each
+// made opaque first: the optimiser would otherwise split a store of a
+// vector it sees assembled from scalars into scalar stores and drop the
+// nontemporal flag.
+__device__ inline void store_nt(double *p, const double (&a)[4])
+{
+    typedef double R2 __attribute__((ext_vector_type(2)));
+    R2 v0 = {a[0], a[1]}, v1 = {a[2], a[3]};
+    asm volatile("" : "+v"(v0));
+    asm volatile("" : "+v"(v1));
+    __builtin_nontemporal_store(v0, reinterpret_cast<R2 *>(p));
+    __builtin_nontemporal_store(v1, reinterpret_cast<R2 *>(p) + 1);
+}
+
 // seed_key(s, i + 1) from k = seed_key(s, i): keys run 1 .. 2^32 - 1.
 __device__ inline uint32_t next_key(uint32_t k)
 {
@@ -784,6 +802,8 @@ __device__ inline void cv_tile(const Real *SX, const Real *SY, const Real *SVX,
     const double lw_equal = pd->log_equal;
     const int equal = pd->equal;
     const double c = pd->c;
+    // equal weights (the step after an event): one monitor weight for all
+    const double w_equal = mon && equal ? exp(lw_equal) : 0.0;
     const unsigned long long blk = static_cast<unsigned long long>(t) + 1;
     if ((n & 3) == 0 && base + kPTile <= n) {
         // Full tile, every component row 16-byte aligned: each thread takes
@@ -841,7 +861,7 @@ __device__ inline void cv_tile(const Real *SX, const Real *SY, const Real *SVX,
                         vx = vxs[r].v[q];
                         vy = vys[r].v[q];
                         if (mon) {
-                            const double w = exp(lq);
+                            const double w = equal ? w_equal : exp(lq);
                             if (w > 0.0) {
                                 mx += w * static_cast<double>(x);
                                 my += w * static_cast<double>(y);
@@ -861,11 +881,11 @@ __device__ inline void cv_tile(const Real *SX, const Real *SY, const Real *SVX,
                 *reinterpret_cast<Vec4<Real> *>(Y + i0) = ys[r];
                 *reinterpret_cast<Vec4<Real> *>(VX + i0) = vxs[r];
                 *reinterpret_cast<Vec4<Real> *>(VY + i0) = vys[r];
-                Vec4<double> lo;
+                double lo[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    lo.v[q] = lv[(r0 + r) * 4 + q];
-                *reinterpret_cast<Vec4<double> *>(l + i0) = lo;
+                    lo[q] = lv[(r0 + r) * 4 + q];
+                store_nt(l + i0, lo);
             }
         }
         acc = part_of_array(lv);
@@ -885,7 +905,7 @@ __device__ inline void cv_tile(const Real *SX, const Real *SY, const Real *SVX,
                 vy = SVY[a];
                 lold = equal ? lw_equal : l[i] - c;
                 if (mon) {
-                    const double w = exp(lold);
+                    const double w = equal ? w_equal : exp(lold);
                     if (w > 0.0) {
                         mx += w * static_cast<double>(x);
                         my += w * static_cast<double>(y);
