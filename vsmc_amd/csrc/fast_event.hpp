@@ -61,6 +61,26 @@ struct FeCross {
     double wc;
 };
 
+// An array's tile, element j kBlock + tid in w[j] (0 past n), all loads in
+// flight together; the block synchronises, as the log-weight form does.
+__device__ __forceinline__ void fe_load_w(WArray s, long long base, long long n,
+    double *, double (&w)[kItems])
+{
+    const bool full = base + kTile <= n;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const long long g = base + j * kBlock + threadIdx.x;
+        w[j] = s.w[full || g < n ? g : n - 1];
+    }
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        keep_loaded(w[j]);
+        const long long g = base + j * kBlock + threadIdx.x;
+        w[j] = full || g < n ? w[j] : 0.0;
+    }
+    __syncthreads();
+}
+
 // A tile's weights, element j kBlock + tid in w[j] (0 past n): the
 // log-weights are loaded first (every load in flight before the first is
 // used; clamped indices in a partial tile), then W from each, with e^x from
@@ -678,6 +698,145 @@ template <class U> struct FeCounter;
 template <> struct FeCounter<USys> { using type = FeSys; };
 template <> struct FeCounter<UStrat> { using type = FeStrat; };
 
+// Counts of a tile's exact sums (thread tid holds those of elements
+// k0 = tid kItems .. k0 + kItems - 1 in sk, blocked) against materialised
+// sorted draws (multinomial): the tile's counts lie in [glo, ghi], the
+// counts at its exact start and at the next tile's (k_fe_arr_bounds), so
+// only the slice v[glo, ghi) decides them. The slice goes through LDS (buf,
+// the tile buffer) in chunks of kFeSlice draws (a tile's share of the draws
+// and some margin), every open sum searched there (a binary search for a
+// thread's first, then kFeProbes probes past the previous answer, since a
+// thread's sums are sorted); a
+// slice of more than kVChunks chunks (weight concentrated in the tile) is
+// searched in place by every sum, so that no block walks a long slice
+// alone. g[j] = G(sk[j]) and gp = G of the sum before the thread's first
+// element (glo for the first thread).
+constexpr int kFeProbes = 8;                        // draws probed at once
+constexpr int kFeSliceItems = kTileLds / kBlock;     // chunk slots per thread
+constexpr int kFeSlice = kFeSliceItems * kBlock - kFeProbes;   // draws per chunk
+static_assert(kFeSliceItems * kBlock <= kTileLds, "a chunk fits the tile buffer");
+
+__device__ inline void fe_arr_counts(const UArr &us, const double (&sk)[kItems],
+    long long glo, long long ghi, int nvalid, double *buf, int (&g)[kItems],
+    int &gp)
+{
+    const int tid = static_cast<int>(threadIdx.x);
+    const int k0 = tid * kItems;
+    const long long m = ghi > glo ? ghi - glo : 0;
+    const bool in_lds = m <= static_cast<long long>(kVChunks) * kFeSlice;
+    int cnt[kItems];   // counts within the slice (m < 2^31)
+    unsigned open = 0;   // bit j: the count of sum j not found yet
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        cnt[j] = static_cast<int>(m);
+        open |= k0 + j < nvalid ? 1u << j : 0u;
+    }
+    if (!in_lds) {
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            if ((open >> j) & 1u) {
+                long long lo = 0, hi = m;   // first slice index with v > sk
+                while (lo < hi) {
+                    const long long mid = (lo + hi) >> 1;
+                    if (us.v[glo + mid] <= sk[j])
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                cnt[j] = static_cast<int>(lo);
+            }
+        }
+        open = 0;
+    }
+    for (long long c0 = 0; c0 < (in_lds ? m : 0); c0 += kFeSlice) {
+        const int cn = m - c0 < kFeSlice ? static_cast<int>(m - c0) : kFeSlice;
+        {
+            // clamped, unconditional loads, all in flight together
+            const double *const vc = us.v + glo + c0;
+            double x[kFeSliceItems];
+#pragma unroll
+            for (int j = 0; j < kFeSliceItems; ++j) {
+                const int q = j * kBlock + tid;
+                x[j] = vc[q < cn ? q : cn - 1];
+            }
+            // slots past the chunk hold +inf, above every sum
+#pragma unroll
+            for (int j = 0; j < kFeSliceItems; ++j) {
+                keep_loaded(x[j]);
+                buf[j * kBlock + tid] = j * kBlock + tid < cn ? x[j] : INFINITY;
+            }
+        }
+        __syncthreads();
+        // a thread's sums are sorted: a binary search for the first one
+        // resolved in this chunk, then from the previous answer a (the
+        // first draw above the previous sum) kFeProbes probes at once, about
+        // a draw per sum on average (a binary search when all lie below)
+        const double vlast = buf[cn - 1];
+        auto search = [&](int lo, double x) {
+            // first index with buf > x in [lo, cn); buf[cn - 1] > x
+            int hi = cn - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (buf[mid] <= x)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            return lo;
+        };
+        int a = -1;
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            if (((open >> j) & 1u) && sk[j] < vlast) {
+                if (a < 0) {
+                    a = search(0, sk[j]);
+                } else {
+                    // a < cn, so the probes stay in the buffer (slots past
+                    // the chunk: +inf); the draws at or below sk[j] are a
+                    // prefix of them
+                    int n = 0;
+#pragma unroll
+                    for (int q = 0; q < kFeProbes; ++q)
+                        n += buf[a + q] <= sk[j] ? 1 : 0;
+                    a += n;
+                    if (n == kFeProbes)
+                        a = search(a, sk[j]);
+                }
+                cnt[j] = static_cast<int>(c0) + a;
+                open &= ~(1u << j);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+        g[j] = static_cast<int>(glo + cnt[j]);
+    // G before the thread's first element: the previous thread's last
+    int *const last = reinterpret_cast<int *>(buf);
+    last[tid] = g[kItems - 1];
+    __syncthreads();
+    gp = tid == 0 ? static_cast<int>(glo) : last[tid - 1];
+}
+
+// The draw counts G(x) = #{j : v_j <= x} at every tile's exact start x =
+// xstart[t] into gb[t] (0 for tile 0), one wave per tile, 64 probes a
+// round; the counting pass reads its slice bounds there.
+__global__ void __launch_bounds__(kBlock) k_fe_arr_bounds(UArr us, long long M,
+    const ResampleDev *prm, const double *xstart, int *gb)
+{
+    if (!prm->active)
+        return;
+    const long long ntiles = (M + kTile - 1) / kTile;
+    const long long t = static_cast<long long>(blockIdx.x) * kWaves +
+        static_cast<long long>(threadIdx.x) / kWave;
+    if (t >= ntiles)
+        return;
+    const long long g = t == 0 ? 0 :
+        wave_last_le([&](long long q) { return us.v[q]; }, us.ndraw(), xstart[t]) + 1;
+    if ((threadIdx.x & (kWave - 1)) == 0)
+        gb[t] = static_cast<int>(g);
+}
+
 // Relative bound between an FP64 block-scan prefix of a tile (exact start
 // plus a blocked scan: at most 16 + 6 + 3 + 18 roundings along any path)
 // and the sequential fold over at most 4096 weights from the same exact
@@ -699,7 +858,7 @@ struct FeCountShared {
 template <class WSrc, class USrc>
 __global__ void __launch_bounds__(kBlock, 4) k_fe_count(WSrc src, USrc us,
     long long M, const ResampleDev *prm, double *xstart, int *rep,
-    long long *tagg, int *csrc, int *cE)
+    long long *tagg, int *csrc, int *cE, const int *gb)
 {
     if (!prm->active)
         return;
@@ -713,6 +872,13 @@ __global__ void __launch_bounds__(kBlock, 4) k_fe_count(WSrc src, USrc us,
     const int nvalid = rem < kTile ? static_cast<int>(rem) : kTile;
     const bool last_tile = base + kTile >= M;
     const double x0 = xstart[t];
+    // materialised draws: the tile's slice bounds (the last tile's reaches
+    // the end)
+    long long glo = 0, ghi = 0;
+    if constexpr (std::is_same<USrc, UArr>::value) {
+        glo = gb[t];
+        ghi = last_tile ? us.ndraw() : gb[t + 1];
+    }
     double w[kItems];
     fe_load_blocked(src.loaded(), base, M, sh.tile, sh.tab, w);
     const bool xpos = x0 > 0.0 && x0 < INFINITY;
@@ -732,72 +898,92 @@ __global__ void __launch_bounds__(kBlock, 4) k_fe_count(WSrc src, USrc us,
     const double a0d = xpos ? static_cast<double>(fold::mant(x0)) : 0.0;
     const bool fast = xpos && st.ok && !anytie &&
         a0d + D < static_cast<double>(fold::kTwo53);
-    const typename FeCounter<USrc>::type cnt(us);
     const int nd = static_cast<int>(us.ndraw());
     const int k0 = tid * kItems;
     // g[j] = G(s_k) for element k = k0 + j, gp = G before the thread's first
     int g[kItems];
     int gp;
-    if (fast) {
-        // s_k = x0 advanced by the steps up to k, all in x0's binade
-        const double sprev = tid == 0 ? x0 : st.value(a0d + run);
-        gp = (base + k0 == 0) ? 0 : cnt.count(sprev);
-        double s[kItems];
+    if constexpr (std::is_same<USrc, UArr>::value) {
+        // materialised draws: the exact sums first (from the steps, or the
+        // exact fold), then their counts from the tile's slice of draws
+        double sk[kItems];
+        if (fast) {
 #pragma unroll
-        for (int j = 0; j < kItems; ++j) {
-            run += FeSteps::round(st.q(w[j]));
-            s[j] = st.value(a0d + run);
-            g[j] = cnt.count(s[j]);
-        }
-        // the total after the last tile, which the resolver leaves
-        // unverified (the fast event's callers do not read it)
-        if (last_tile) {
-#pragma unroll
-            for (int j = 0; j < kItems; ++j)
-                if (k0 + j == nvalid - 1)
-                    xstart[t + 1] = s[j];
-        }
-    } else {
-        // an FP64 prefix within kFeCertRel of every exact sum: a count is
-        // certain when both ends of the interval give it
-        double loc = 0.0;
-#pragma unroll
-        for (int j = 0; j < kItems; ++j)
-            loc += w[j];
-        double tw;
-        const double ex = block_exclusive_scan(loc, OpAddD(), 0.0, sh.dlds, &tw);
-        double p = x0 + ex;
-        bool amb = false;
-        auto certain = [&](double v) {
-            if (!(v < INFINITY))
-                return cnt.count(v);
-            const double m = v * kFeCertRel;
-            const int lo = cnt.count(v - m);
-            const int hi = cnt.count(v + m);
-            amb = amb || lo != hi;
-            return lo;
-        };
-        if (base + k0 == 0)
-            gp = 0;
-        else if (tid == 0)
-            gp = cnt.count(x0);
-        else
-            gp = certain(p);
-#pragma unroll
-        for (int j = 0; j < kItems; ++j) {
-            p = p + w[j];
-            g[j] = certain(p);
-        }
-        if (__syncthreads_or(amb)) {
-            // a draw within the bound of a sum: the exact fold of fold.hpp
-            // (one binade with ties in order, runs between crossings, and
-            // element by element where a run's prediction fails)
+            for (int j = 0; j < kItems; ++j) {
+                run += FeSteps::round(st.q(w[j]));
+                sk[j] = st.value(a0d + run);
+            }
+        } else {
             fold::tile_exact(sh.tile, x0, sh.ex);
-            gp = (base + k0 == 0) ? 0 :
-                cnt.count(tid == 0 ? x0 : sh.tile[lds_pad(k0 - 1)]);
 #pragma unroll
             for (int j = 0; j < kItems; ++j)
-                g[j] = cnt.count(sh.tile[lds_pad(k0 + j)]);
+                sk[j] = sh.tile[lds_pad(k0 + j)];
+        }
+        __syncthreads();   // every thread's sums read from the tile
+        fe_arr_counts(us, sk, glo, ghi, nvalid, sh.tile, g, gp);
+    } else {
+        const typename FeCounter<USrc>::type cnt(us);
+        if (fast) {
+            // s_k = x0 advanced by the steps up to k, all in x0's binade
+            const double sprev = tid == 0 ? x0 : st.value(a0d + run);
+            gp = (base + k0 == 0) ? 0 : cnt.count(sprev);
+            double s[kItems];
+#pragma unroll
+            for (int j = 0; j < kItems; ++j) {
+                run += FeSteps::round(st.q(w[j]));
+                s[j] = st.value(a0d + run);
+                g[j] = cnt.count(s[j]);
+            }
+            // the total after the last tile, which the resolver leaves
+            // unverified (the fast event's callers do not read it)
+            if (last_tile) {
+#pragma unroll
+                for (int j = 0; j < kItems; ++j)
+                    if (k0 + j == nvalid - 1)
+                        xstart[t + 1] = s[j];
+            }
+        } else {
+            // an FP64 prefix within kFeCertRel of every exact sum: a count is
+            // certain when both ends of the interval give it
+            double loc = 0.0;
+#pragma unroll
+            for (int j = 0; j < kItems; ++j)
+                loc += w[j];
+            double tw;
+            const double ex = block_exclusive_scan(loc, OpAddD(), 0.0, sh.dlds, &tw);
+            double p = x0 + ex;
+            bool amb = false;
+            auto certain = [&](double v) {
+                if (!(v < INFINITY))
+                    return cnt.count(v);
+                const double m = v * kFeCertRel;
+                const int lo = cnt.count(v - m);
+                const int hi = cnt.count(v + m);
+                amb = amb || lo != hi;
+                return lo;
+            };
+            if (base + k0 == 0)
+                gp = 0;
+            else if (tid == 0)
+                gp = cnt.count(x0);
+            else
+                gp = certain(p);
+#pragma unroll
+            for (int j = 0; j < kItems; ++j) {
+                p = p + w[j];
+                g[j] = certain(p);
+            }
+            if (__syncthreads_or(amb)) {
+                // a draw within the bound of a sum: the exact fold of fold.hpp
+                // (one binade with ties in order, runs between crossings, and
+                // element by element where a run's prediction fails)
+                fold::tile_exact(sh.tile, x0, sh.ex);
+                gp = (base + k0 == 0) ? 0 :
+                    cnt.count(tid == 0 ? x0 : sh.tile[lds_pad(k0 - 1)]);
+#pragma unroll
+                for (int j = 0; j < kItems; ++j)
+                    g[j] = cnt.count(sh.tile[lds_pad(k0 + j)]);
+            }
         }
     }
     // counts: rep_k = G(s_k) - G(s_{k-1}); G before particle 0 is 0, and the
@@ -872,4 +1058,201 @@ __global__ void __launch_bounds__(kBlock, 4) k_fe_count(WSrc src, USrc us,
         tagg[3 * t + 1] = tot >> 32;
         tagg[3 * t + 2] = nsrc;
     }
+}
+
+// ------------------------------------------------ sorted draws (multinomial)
+//
+// U01SequenceSorted (u01_sequence.hpp:141-154): lmax_k = lmax_{k-1} +
+// log(U_k) / (n - k), u_k = 1 - exp(lmax_k). With the negated terms
+// a_k = -(log(U_k) / (n - k)) >= 0 the sums s_k = -lmax_k are a sequential
+// FP64 fold of non-negative terms (rounded the same way by symmetry), so
+// the fast event's summaries and resolver give every tile's exact start:
+// k_fe_sorted_terms materialises the terms once (with per-tile sums),
+// k_fe_sorted_scan the approximate tile prefix and the tiles predicted to
+// cross a binade, k_fe_summ / k_fe_resolve the exact starts, and
+// k_fe_sorted_prefix every u_k from its exact sum (the steps in the start's
+// binade, else the exact fold), recording whether u ever decreases: the
+// running maximum (transform.hpp's merge) is then the identity, and the
+// maximum-scan passes run only otherwise.
+
+// Terms a_k for k in [0, n) of the stream (key, word0 + k), four words of a
+// Philox block per thread at a time (a thread's run of four may straddle
+// two blocks when word0 is not a multiple of four), and the tile sums.
+__global__ void __launch_bounds__(kBlock) k_fe_sorted_terms(long long n,
+    ResampleDev *prm, double *terms, double *tsum)
+{
+    if (!prm->active)
+        return;
+    const long long base = static_cast<long long>(blockIdx.x) * kTile;
+    if (base >= n)
+        return;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        prm->nonmono = 0;
+    const unsigned int key = prm->key;
+    const unsigned long long word0 = prm->word0;
+    const double nd = static_cast<double>(n);
+    __shared__ double lds[kWaves];
+    double acc = 0.0;
+#pragma unroll
+    for (int g = 0; g < kItems / 4; ++g) {
+        const long long k0 = base + 4 * (static_cast<long long>(g) * kBlock + threadIdx.x);
+        if (k0 >= n)
+            continue;
+        const unsigned long long w0 = word0 + static_cast<unsigned long long>(k0);
+        const unsigned int off = static_cast<unsigned int>(w0 & 3);
+        const u32x4 b0 = philox_block(key, w0 / 4 + 1);
+        u32x4 b1 = b0;
+        if (off != 0)
+            b1 = philox_block(key, w0 / 4 + 2);
+        // words off .. off + 3 of the two blocks (off is the same for every
+        // thread: constant register indices in each case)
+        uint32_t wq[4];
+        switch (off) {
+        case 0: wq[0] = b0.v[0]; wq[1] = b0.v[1]; wq[2] = b0.v[2]; wq[3] = b0.v[3]; break;
+        case 1: wq[0] = b0.v[1]; wq[1] = b0.v[2]; wq[2] = b0.v[3]; wq[3] = b1.v[0]; break;
+        case 2: wq[0] = b0.v[2]; wq[1] = b0.v[3]; wq[2] = b1.v[0]; wq[3] = b1.v[1]; break;
+        default: wq[0] = b0.v[3]; wq[1] = b1.v[0]; wq[2] = b1.v[1]; wq[3] = b1.v[2]; break;
+        }
+        double a[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double u = u01_u32d(wq[q]);
+            const long long k = k0 + q;
+            a[q] = -(log(u) / (nd - static_cast<double>(k)));
+        }
+        if (k0 + 4 <= n) {
+            typedef double D2 __attribute__((ext_vector_type(2)));
+            D2 *const dst = reinterpret_cast<D2 *>(terms + k0);
+            dst[0] = D2{a[0], a[1]};
+            dst[1] = D2{a[2], a[3]};
+            acc += (a[0] + a[1]) + (a[2] + a[3]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (k0 + q < n) {
+                    terms[k0 + q] = a[q];
+                    acc += a[q];
+                }
+            }
+        }
+    }
+    acc = block_reduce(acc, OpAddD(), lds);
+    if (threadIdx.x == 0)
+        tsum[blockIdx.x] = acc;
+}
+
+// The approximate exclusive tile prefix of the terms, in place over the
+// tile sums, and the tiles predicted to cross a binade (fpred as the
+// finalize writes it for the weights). One block of kFeScanThreads.
+constexpr int kFeScanThreads = 1024;
+__global__ void __launch_bounds__(kFeScanThreads) k_fe_sorted_scan(long long n,
+    const ResampleDev *prm, double *tpre, int *fpred)
+{
+    if (!prm->active)
+        return;
+    const long long nb = (n + kTile - 1) / kTile;
+    __shared__ double lds[kFeScanThreads / kWave];
+    __shared__ int cnt;
+    block_scan_chunked_n<kFeScanThreads, 4>(nb,
+        [&](long long b) { return tpre[b]; },
+        [&](long long b, double ex) { tpre[b] = ex; }, OpAddD(), 0.0, lds);
+    list_predicted(tpre, nb, fpred, &cnt);
+}
+
+struct FeSortedShared {
+    double tile[kTileLds];
+    fold::ExactShared ex;
+    double dlds[kWaves];
+};
+
+// Every u_k = 1 - exp(-s_k) (u01_sequence.hpp:151-152) from the exact sums
+// of its tile (xstart[t] the exact start), written to out; sets
+// prm->nonmono when some u_k < u_{k-1}.
+__global__ void __launch_bounds__(kBlock) k_fe_sorted_prefix(long long n,
+    ResampleDev *prm, const double *terms, const double *xstart, double *out)
+{
+    if (!prm->active)
+        return;
+    const long long t = blockIdx.x;
+    const long long base = t * kTile;
+    if (base >= n)
+        return;
+    __shared__ FeSortedShared sh;
+    const int tid = static_cast<int>(threadIdx.x);
+    const long long rem = n - base;
+    const int nvalid = rem < kTile ? static_cast<int>(rem) : kTile;
+    const double x0 = xstart[t];
+    double w[kItems];
+    {
+        // the tile in blocked order (and in LDS, element k at lds_pad(k))
+        fe_load_w(WArray{terms}, base, n, nullptr, w);
+#pragma unroll
+        for (int j = 0; j < kItems; ++j)
+            sh.tile[lds_pad(j * kBlock + tid)] = w[j];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kItems; ++j)
+            w[j] = sh.tile[lds_pad(tid * kItems + j)];
+    }
+    const bool xpos = x0 > 0.0 && x0 < INFINITY;
+    const FeSteps st(xpos ? fold::binade(x0) : 0);
+    double dsum = 0.0;
+    bool tie = false;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const double q = st.q(w[j]);
+        const double r = FeSteps::round(q);
+        tie = tie || FeSteps::half(q, r);
+        dsum += r;
+    }
+    double D;
+    double run = block_exclusive_scan(dsum, OpAddD(), 0.0, sh.dlds, &D);
+    const bool anytie = __syncthreads_or(tie);
+    const double a0d = xpos ? static_cast<double>(fold::mant(x0)) : 0.0;
+    const bool fast = xpos && st.ok && !anytie &&
+        a0d + D < static_cast<double>(fold::kTwo53);
+    const int k0 = tid * kItems;
+    double u[kItems];
+    if (fast) {
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            run += FeSteps::round(st.q(w[j]));
+            u[j] = 1.0 - exp(-st.value(a0d + run));
+        }
+    } else {
+        fold::tile_exact(sh.tile, x0, sh.ex);
+#pragma unroll
+        for (int j = 0; j < kItems; ++j)
+            u[j] = 1.0 - exp(-sh.tile[lds_pad(k0 + j)]);
+    }
+    // u before the thread's first element: the previous thread's last, or
+    // for the first thread the previous tile's last (from its exact sum);
+    // the tile buffer holds each thread's last once every sum is read
+    __syncthreads();
+    double *const last = sh.tile;
+    last[tid] = u[kItems - 1];
+    __syncthreads();
+    double prev = tid > 0 ? last[tid - 1] : (t > 0 ? 1.0 - exp(-x0) : 0.0);
+    bool dec = false;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        if (k0 + j < nvalid) {
+            dec = dec || u[j] < prev;
+            prev = u[j];
+        }
+    }
+    if (k0 + kItems <= nvalid) {
+        typedef double D2 __attribute__((ext_vector_type(2)));
+        D2 *const dst = reinterpret_cast<D2 *>(out + base + k0);
+#pragma unroll
+        for (int q = 0; q < kItems / 2; ++q)
+            dst[q] = D2{u[2 * q], u[2 * q + 1]};
+    } else {
+#pragma unroll
+        for (int j = 0; j < kItems; ++j)
+            if (k0 + j < nvalid)
+                out[base + k0 + j] = u[j];
+    }
+    if (__syncthreads_or(dec) && tid == 0)
+        atomicOr(&prm->nonmono, 1);
 }

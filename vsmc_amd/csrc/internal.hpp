@@ -29,6 +29,8 @@ struct ResampleDev {
     int active;           // 0: step did not resample, kernels return early
     int err;              // input validation flags
     int equal;            // weights are 1/N (log-weights ignored)
+    int nonmono;          // the sorted draws' epilogue decreased somewhere
+    int pad;
 };
 
 // One filter of a batch: its particles occupy [off, off + n) of the batch's
@@ -124,6 +126,27 @@ constexpr int kGroup = 64;
 // to cross a binade (fold::predicted_crossing), at most this many.
 constexpr int kFePred = 64;
 
+// After a block has written the approximate tile prefix tpre[0, nb): the
+// tiles predicted to cross a binade (fold::predicted_crossing) into fpred
+// ([0] their count, which may exceed the kFePred entries), in any order.
+__device__ inline void list_predicted(const double *tpre, long long nb, int *fpred,
+    int *cnt_lds)
+{
+    if (threadIdx.x == 0)
+        *cnt_lds = 0;
+    __syncthreads();   // tpre complete for the block
+    for (long long t = threadIdx.x; t < nb; t += blockDim.x) {
+        if (fold::predicted_crossing(tpre, t, nb)) {
+            const int k = atomicAdd(cnt_lds, 1);
+            if (k < kFePred)
+                fpred[1 + k] = static_cast<int>(t);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        fpred[0] = *cnt_lds;
+}
+
 // Status codes (mirrors vsmch_status in include/vsmch.h)
 enum {
     kOk = 0,
@@ -164,6 +187,13 @@ enum WeightKind {
     kWeightArray = 0,    // W given (linear, non-negative)
     kWeightLog = 1,      // W = exp(l - lmax) * inv_sum from prm
 };
+
+// Schemes whose particle-set events run the fast event (fast_event.hpp):
+// the finalize prepares their draw parameters and the weights' tile prefix.
+inline bool fast_event_scheme(int scheme)
+{
+    return scheme == kSystematic || scheme == kStratified || scheme == kMultinomial;
+}
 
 // Replication counts of the transform with explicit uniforms:
 // resample_trans_u01_rep(M, N, W, u01, rep) semantics. Device pointers.

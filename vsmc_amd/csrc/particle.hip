@@ -331,27 +331,6 @@ __device__ inline int decide_event(PDev *pd, ResampleDev *prm, long long n,
     return active;
 }
 
-// After a block has written the approximate tile prefix tpre[0, nb): the
-// tiles predicted to cross a binade (fold::predicted_crossing) into fpred
-// ([0] their count, which may exceed the kFePred entries), in any order.
-__device__ inline void list_predicted(const double *tpre, long long nb, int *fpred,
-    int *cnt_lds)
-{
-    if (threadIdx.x == 0)
-        *cnt_lds = 0;
-    __syncthreads();   // tpre complete for the block
-    for (long long t = threadIdx.x; t < nb; t += blockDim.x) {
-        if (fold::predicted_crossing(tpre, t, nb)) {
-            const int k = atomicAdd(cnt_lds, 1);
-            if (k < kFePred)
-                fpred[1 + k] = static_cast<int>(t);
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0)
-        fpred[0] = *cnt_lds;
-}
-
 __global__ void __launch_bounds__(kMergeThreads) k_prepare(PDev *pd, ResampleDev *prm,
     const Part3 *part, long long nb, long long n, double threshold,
     long long t, double *ess_hist, int *rs_hist, int scheme, double *tpre,
@@ -1551,7 +1530,7 @@ int resample_step(ParticleSet *p, int scheme, double threshold, long long t,
     hipStream_t st = p->ws.stream;
     // systematic / stratified with N >= 2: one kernel prepares the decision,
     // the draw parameters and the tile prefix from the weight partials
-    const bool prep = p->n >= 2 && (scheme == kSystematic || scheme == kStratified);
+    const bool prep = p->n >= 2 && fast_event_scheme(scheme);
     if (prep)
         hipLaunchKernelGGL(k_prepare, dim3(1), dim3(kMergeThreads), 0, st, p->pd,
             p->ws.prm, static_cast<const Part3 *>(p->part), pblocks(p->n),
@@ -1831,7 +1810,7 @@ static int cv_iterate_impl(ParticleSet *p, const CvModelDev &m, long long t0,
     Real *const home = static_cast<Real *>(p->state);
     Real *cur = home;
     Real *nxt = static_cast<Real *>(p->state_alt);
-    const bool prep = p->n >= 2 && (scheme == kSystematic || scheme == kStratified);
+    const bool prep = p->n >= 2 && fast_event_scheme(scheme);
     const int batch = move_batch(p->bytes);
     Part3 *part = static_cast<Part3 *>(p->part);
     for (long long s = 0; s < k; ++s) {

@@ -43,7 +43,7 @@ __device__ inline void after_event(PDev *pd, ResampleDev *prm, long long n,
     pd->anc_valid = deferred;
     if (words) {
         unsigned long long used = 1;
-        if (scheme == kStratified) {
+        if (scheme == kStratified || scheme == kMultinomial) {
             const long long jend = n - rep[n - 1];
             used = static_cast<unsigned long long>(jend + 1 < n ? jend + 1 : n);
         }

@@ -1081,8 +1081,22 @@ __device__ __forceinline__ double count_tile(WSrc src, USrc us, long long M,
         }
         for (long long c0 = 0; c0 < (in_lds ? m : 0); c0 += kVSlice) {
             const int cn = m - c0 < kVSlice ? static_cast<int>(m - c0) : kVSlice;
-            for (int q = threadIdx.x; q < cn; q += kBlock)
-                vs[q] = us.v[glo + c0 + q];
+            {
+                // clamped, unconditional loads, all in flight together
+                static_assert(kVSlice == kItems * kBlock, "one load per item");
+                const double *const vc = us.v + glo + c0;
+                double x[kItems];
+#pragma unroll
+                for (int j = 0; j < kItems; ++j) {
+                    const int q = j * kBlock + static_cast<int>(threadIdx.x);
+                    x[j] = vc[q < cn ? q : cn - 1];
+                }
+#pragma unroll
+                for (int j = 0; j < kItems; ++j) {
+                    keep_loaded(x[j]);
+                    vs[j * kBlock + threadIdx.x] = x[j];
+                }
+            }
             __syncthreads();
             const double vlast = vs[cn - 1];
             int lo = -1;   // blocked: the previous answer in this chunk
@@ -1221,10 +1235,12 @@ __global__ void __launch_bounds__(kBlock) k_prefix(WSrc src, long long nh,
 
 // ------------------------------------------------- running maximum (v)
 
+// (gate: when non-null, the running maximum is needed only if *gate is set)
 __global__ void __launch_bounds__(kBlock) k_tile_max(const double *u,
-    long long nh, const long long *ndev, const ResampleDev *prm, double *tmax)
+    long long nh, const long long *ndev, const ResampleDev *prm, double *tmax,
+    const int *gate)
 {
-    if (!prm->active)
+    if (!prm->active || (gate && !*gate))
         return;
     const long long n = eff_n(ndev, nh);
     const long long base = static_cast<long long>(blockIdx.x) * kTile;
@@ -1246,9 +1262,10 @@ __global__ void __launch_bounds__(kBlock) k_tile_max(const double *u,
 }
 
 __global__ void __launch_bounds__(kBlock) k_scan_max(const double *tmax,
-    long long nh, const long long *ndev, const ResampleDev *prm, double *tpre)
+    long long nh, const long long *ndev, const ResampleDev *prm, double *tpre,
+    const int *gate)
 {
-    if (!prm->active)
+    if (!prm->active || (gate && !*gate))
         return;
     const long long n = eff_n(ndev, nh);
     const long long ntiles = ceil_div(n, kTile);
@@ -1267,9 +1284,9 @@ __global__ void __launch_bounds__(kBlock) k_scan_max(const double *tmax,
 
 __global__ void __launch_bounds__(kBlock) k_apply_max(const double *u,
     long long nh, const long long *ndev, const ResampleDev *prm,
-    const double *tpre, double *v)
+    const double *tpre, double *v, const int *gate)
 {
-    if (!prm->active)
+    if (!prm->active || (gate && !*gate))
         return;
     const long long n = eff_n(ndev, nh);
     const long long base = static_cast<long long>(blockIdx.x) * kTile;
@@ -1884,7 +1901,7 @@ __global__ void __launch_bounds__(kBlock) k_ridx_sources(const int *rep,
 // their extras prefix within the tile at u kTile .., written by the fast
 // event's counting pass) instead of compacted over all tiles.
 template <typename E, bool kReg = false>
-__global__ void __launch_bounds__(kBlock) k_ridx_fill(const ResampleDev *prm,
+__global__ void __launch_bounds__(kBlock, 5) k_ridx_fill(const ResampleDev *prm,
     const int *rep, long long M, const long long *toff, const int *tq,
     long long ntiles, const int *cE, const int *csrc, int *idx, E *state,
     int dim, long long ld, ResampleDev *eprm, EventEnd end)
@@ -2735,8 +2752,12 @@ static int fast_count(Workspace &ws, WLog wb, USrc us, long long M, int *rep)
     hipLaunchKernelGGL(k_fe_resolve<WLog>, dim3(1), dim3(kFeThreads), 0, st, wb, M,
         ws.prm, ws.segs, ws.fsum, ws.fcross, ws.groups, ws.xstart, ws.xgstart,
         ws.gmode);
+    if constexpr (std::is_same<USrc, UArr>::value)
+        hipLaunchKernelGGL(k_fe_arr_bounds, dim3((gt + kWaves - 1) / kWaves),
+            dim3(kBlock), 0, st, us, M, ws.prm, ws.xstart, ws.hard);
     hipLaunchKernelGGL((k_fe_count<WLog, USrc>), dim3(gt), dim3(kBlock), 0, st,
-        wb, us, M, ws.prm, ws.xstart, rep, ws.tagg, ws.csrc, ws.cE);
+        wb, us, M, ws.prm, ws.xstart, rep, ws.tagg, ws.csrc, ws.cE,
+        static_cast<const int *>(ws.hard));
     VSMCH_HIP(hipGetLastError());
     return kOk;
 }
@@ -2763,19 +2784,44 @@ static int setup_host(Workspace &ws, long long M, long long N,
     return kOk;
 }
 
-// Running maximum of u (n entries) into ws.v.
+// Running maximum of u (n entries) into v (ws.v when null; v == u runs in
+// place); with gate, only when the device flag *gate is set.
 static int run_maxscan(Workspace &ws, const double *u, long long n,
-    const long long *ndev)
+    const long long *ndev, double *v = nullptr, const int *gate = nullptr)
 {
     const unsigned int gt = grid_tiles(n);
     hipLaunchKernelGGL(k_tile_max, dim3(gt), dim3(kBlock), 0, ws.stream, u, n,
-        ndev, ws.prm, ws.tsum);
+        ndev, ws.prm, ws.tsum, gate);
     hipLaunchKernelGGL(k_scan_max, dim3(1), dim3(kBlock), 0, ws.stream,
-        ws.tsum, n, ndev, ws.prm, ws.tpre);
+        ws.tsum, n, ndev, ws.prm, ws.tpre, gate);
     hipLaunchKernelGGL(k_apply_max, dim3(gt), dim3(kBlock), 0, ws.stream, u,
-        n, ndev, ws.prm, ws.tpre, ws.v);
+        n, ndev, ws.prm, ws.tpre, v ? v : ws.v, gate);
     VSMCH_HIP(hipGetLastError());
     return kOk;
+}
+
+// The sorted draws of a particle set's multinomial event (n draws of the
+// stream prm->key, prm->word0) into out with the fast event's passes
+// (fast_event.hpp); the terms in ws.v, their tile prefix in ws.tsum and
+// predicted crossings in ws.tq. Leaves ws.tpre and ws.fpred (the weights')
+// untouched; the running maximum runs in place only where u decreased.
+static int run_sorted_fast(Workspace &ws, long long n, double *out)
+{
+    hipStream_t st = ws.stream;
+    const unsigned int gt = grid_tiles(n);
+    hipLaunchKernelGGL(k_fe_sorted_terms, dim3(gt), dim3(kBlock), 0, st, n,
+        ws.prm, ws.v, ws.tsum);
+    hipLaunchKernelGGL(k_fe_sorted_scan, dim3(1), dim3(kFeScanThreads), 0, st, n,
+        ws.prm, ws.tsum, ws.tq);
+    hipLaunchKernelGGL(k_fe_summ<WArray>, dim3(kFePred + gt), dim3(kBlock), 0, st,
+        WArray{ws.v}, n, ws.prm, ws.tsum, ws.tq, ws.segs, ws.fsum, ws.fcross);
+    hipLaunchKernelGGL(k_fe_resolve<WArray>, dim3(1), dim3(kFeThreads), 0, st,
+        WArray{ws.v}, n, ws.prm, ws.segs, ws.fsum, ws.fcross, ws.groups,
+        ws.xstart, ws.xgstart, ws.gmode);
+    hipLaunchKernelGGL(k_fe_sorted_prefix, dim3(gt), dim3(kBlock), 0, st, n,
+        ws.prm, ws.v, ws.xstart, out);
+    VSMCH_HIP(hipGetLastError());
+    return run_maxscan(ws, out, n, nullptr, out, &ws.prm->nonmono);
 }
 
 int trans_u01_rep(Workspace &ws, long long M, long long N, const double *W,
@@ -2951,6 +2997,28 @@ static int scheme_counts(Workspace &ws, int scheme, long long M, long long N,
     int rc;
     if (prepared) {
         // draw parameters and the weights' tile prefix were set on the device
+        if (scheme == kMultinomial) {
+            if constexpr (std::is_same<Base, WLog>::value) {
+                if (M == N) {
+                    rc = run_sorted_fast(ws, N, ws.aux);
+                    if (rc != kOk)
+                        return rc;
+                    return fast_count(ws, wb, UArr{ws.aux, ws.prm}, M, rep);
+                }
+            }
+            // the sorted draws first, their fold's tile prefix in the summary
+            // buffer (free until the weights' summary pass; ws.tpre holds the
+            // weights' prefix for that pass)
+            Workspace wd = ws;
+            wd.tpre = reinterpret_cast<double *>(ws.fsum);
+            rc = run_sorted(wd, N, ndraw_dev, ws.aux);
+            if (rc != kOk)
+                return rc;
+            rc = run_maxscan(wd, ws.aux, N, ndraw_dev);
+            if (rc != kOk)
+                return rc;
+            return run_count(ws, wb, UArr{ws.v, ws.prm}, M, rep, true);
+        }
         if constexpr (std::is_same<Base, WLog>::value) {
             if (M == N) {
                 if (scheme == kStratified)
@@ -3036,8 +3104,7 @@ int resample_scheme(Workspace &ws, int scheme, long long M, long long N,
         VSMCH_HIP(hipGetLastError());
         return kOk;
     }
-    const bool prep = prepared && device_active &&
-        (scheme == kSystematic || scheme == kStratified);
+    const bool prep = prepared && device_active && fast_event_scheme(scheme);
     if (weight_kind == kWeightLog)
         rc = scheme_counts(ws, scheme, M, N, WLog{weights, ws.prm}, rep, prep);
     else
