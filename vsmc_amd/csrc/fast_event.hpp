@@ -120,10 +120,27 @@ __device__ __forceinline__ void fe_load_w(WLogV s, long long base, long long n,
     }
 }
 
+// Normalised residual weights inv_resid * modf(N W_i) (WResidV) of a tile
+// from the weights' own loader (0 past n, and when no residual draw
+// remains).
+template <class B>
+__device__ __forceinline__ void fe_load_w(WResidV<B> s, long long base, long long n,
+    double *tab, double (&w)[kItems])
+{
+    fe_load_w(s.b, base, n, tab, w);
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        double ip;
+        const double r = modf(s.coeff * w[j], &ip);
+        w[j] = s.active ? s.inv_resid * r : 0.0;
+    }
+}
+
 // Loads a tile's weights and leaves them in blocked order: thread tid holds
 // elements tid kItems + j in w[j] (through the LDS tile, element k at
 // lds_pad(k), which keeps them afterwards).
-__device__ __forceinline__ void fe_load_blocked(WLogV s, long long base,
+template <class V>
+__device__ __forceinline__ void fe_load_blocked(V s, long long base,
     long long n, double *tile, double *tab, double (&w)[kItems])
 {
     fe_load_w(s, base, n, tab, w);
@@ -180,14 +197,16 @@ struct FeSummShared {
 
 // Tile summaries. Blocks [0, kFePred) take the tiles listed in fpred ([0]
 // the count; none when it exceeds kFePred), block kFePred + t takes tile t
-// unless a first block took it.
+// unless a first block took it. n: *ndev when given (the grid covers nh),
+// else nh.
 template <class WSrc>
-__global__ void __launch_bounds__(kBlock) k_fe_summ(WSrc src, long long n,
-    const ResampleDev *prm, const double *tpre, const int *fpred,
-    TileSeg *segs, FeSum *fs, FeCross *fx)
+__global__ void __launch_bounds__(kBlock) k_fe_summ(WSrc src, long long nh,
+    const long long *ndev, const ResampleDev *prm, const double *tpre,
+    const int *fpred, TileSeg *segs, FeSum *fs, FeCross *fx)
 {
     if (!prm->active)
         return;
+    const long long n = ndev ? *ndev : nh;
     const long long ntiles = ceil_div(n, kTile);
     const int npred = fpred[0];
     const bool early = npred <= kFePred;
@@ -415,14 +434,18 @@ __device__ inline bool fe_apply_segments(const TileSeg &sg, double &x)
 }
 
 // Exact tile starts from the summaries; see the file comment. One block.
+// n as k_fe_summ's.
 template <class WSrc>
-__global__ void __launch_bounds__(kFeThreads) k_fe_resolve(WSrc src, long long n,
-    const ResampleDev *prm, const TileSeg *segs, const FeSum *fs,
-    const FeCross *fx, GroupSeg *groups, double *xstart, double *xgstart,
-    int *gmode)
+__global__ void __launch_bounds__(kFeThreads) k_fe_resolve(WSrc src, long long nh,
+    const long long *ndev, const ResampleDev *prm, const TileSeg *segs,
+    const FeSum *fs, const FeCross *fx, GroupSeg *groups, double *xstart,
+    double *xgstart, int *gmode)
 {
     __shared__ FeResolveLds u;
     FeResolveShared &sh = u.f;
+    const long long n = ndev ? *ndev : nh;
+    if (n <= 0)
+        return;
     const long long ntiles = ceil_div(n, kTile);
     const int tid = static_cast<int>(threadIdx.x);
     // the first chunk's summaries load with the event flag
@@ -693,6 +716,9 @@ struct FeStrat {
     __device__ explicit FeStrat(const UStrat &u) : us(u) {}
     __device__ int count(double x) const { return static_cast<int>(us.count(x)); }
 };
+
+template <class W> struct FeIsResid : std::false_type {};
+template <class B> struct FeIsResid<WResid<B>> : std::true_type {};
 
 template <class U> struct FeCounter;
 template <> struct FeCounter<USys> { using type = FeSys; };
@@ -987,7 +1013,8 @@ __global__ void __launch_bounds__(kBlock, 4) k_fe_count(WSrc src, USrc us,
         }
     }
     // counts: rep_k = G(s_k) - G(s_{k-1}); G before particle 0 is 0, and the
-    // last particle takes every remaining draw
+    // last particle takes every remaining draw (residual schemes add the
+    // integer parts below)
     int r[kItems];
     if (!last_tile) {
 #pragma unroll
@@ -1004,6 +1031,25 @@ __global__ void __launch_bounds__(kBlock, 4) k_fe_count(WSrc src, USrc us,
                 gk = (base + k == M - 1) ? nd : g[j];
             r[j] = gk - gp;
             gp = gk;
+        }
+    }
+    if constexpr (FeIsResid<WSrc>::value) {
+        // residual schemes: rep_k adds floor(N W_k) (transform.hpp:159-161),
+        // from the log-weights of the thread's elements again (all loads in
+        // flight together; the table copy in LDS is still in place)
+        WLogV b = src.b.loaded();
+        b.tab = sh.tab;
+        double lr[kItems];
+#pragma unroll
+        for (int j = 0; j < kItems; ++j)
+            lr[j] = b.raw(base + (k0 + j < nvalid ? k0 + j : nvalid - 1));
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            keep_loaded(lr[j]);
+            double ip;
+            modf(src.coeff * b.from_raw(lr[j]), &ip);
+            if (k0 + j < nvalid)
+                r[j] += static_cast<int>(ip);
         }
     }
     if (k0 + kItems <= nvalid && (reinterpret_cast<uintptr_t>(rep) & 15) == 0) {
@@ -1075,19 +1121,21 @@ __global__ void __launch_bounds__(kBlock, 4) k_fe_count(WSrc src, USrc us,
 // running maximum (transform.hpp's merge) is then the identity, and the
 // maximum-scan passes run only otherwise.
 
-// Terms a_k for k in [0, n) of the stream (key, word0 + k), four words of a
+// Terms a_k for k in [0, n) of the stream (key, word0 + k) (n: *ndev when
+// given, the grid covering nh; as in the next two), four words of a
 // Philox block per thread at a time (a thread's run of four may straddle
 // two blocks when word0 is not a multiple of four), and the tile sums.
-__global__ void __launch_bounds__(kBlock) k_fe_sorted_terms(long long n,
-    ResampleDev *prm, double *terms, double *tsum)
+__global__ void __launch_bounds__(kBlock) k_fe_sorted_terms(long long nh,
+    const long long *ndev, ResampleDev *prm, double *terms, double *tsum)
 {
     if (!prm->active)
         return;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        prm->nonmono = 0;
+    const long long n = ndev ? *ndev : nh;
     const long long base = static_cast<long long>(blockIdx.x) * kTile;
     if (base >= n)
         return;
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        prm->nonmono = 0;
     const unsigned int key = prm->key;
     const unsigned long long word0 = prm->word0;
     const double nd = static_cast<double>(n);
@@ -1145,11 +1193,12 @@ __global__ void __launch_bounds__(kBlock) k_fe_sorted_terms(long long n,
 // tile sums, and the tiles predicted to cross a binade (fpred as the
 // finalize writes it for the weights). One block of kFeScanThreads.
 constexpr int kFeScanThreads = 1024;
-__global__ void __launch_bounds__(kFeScanThreads) k_fe_sorted_scan(long long n,
-    const ResampleDev *prm, double *tpre, int *fpred)
+__global__ void __launch_bounds__(kFeScanThreads) k_fe_sorted_scan(long long nh,
+    const long long *ndev, const ResampleDev *prm, double *tpre, int *fpred)
 {
     if (!prm->active)
         return;
+    const long long n = ndev ? *ndev : nh;
     const long long nb = (n + kTile - 1) / kTile;
     __shared__ double lds[kFeScanThreads / kWave];
     __shared__ int cnt;
@@ -1168,11 +1217,13 @@ struct FeSortedShared {
 // Every u_k = 1 - exp(-s_k) (u01_sequence.hpp:151-152) from the exact sums
 // of its tile (xstart[t] the exact start), written to out; sets
 // prm->nonmono when some u_k < u_{k-1}.
-__global__ void __launch_bounds__(kBlock) k_fe_sorted_prefix(long long n,
-    ResampleDev *prm, const double *terms, const double *xstart, double *out)
+__global__ void __launch_bounds__(kBlock) k_fe_sorted_prefix(long long nh,
+    const long long *ndev, ResampleDev *prm, const double *terms,
+    const double *xstart, double *out)
 {
     if (!prm->active)
         return;
+    const long long n = ndev ? *ndev : nh;
     const long long t = blockIdx.x;
     const long long base = t * kTile;
     if (base >= n)
@@ -1255,4 +1306,86 @@ __global__ void __launch_bounds__(kBlock) k_fe_sorted_prefix(long long n,
     }
     if (__syncthreads_or(dec) && tid == 0)
         atomicOr(&prm->nonmono, 1);
+}
+
+// ---------------------------------------------------- residual schemes
+//
+// The residual transform (transform.hpp:154-168): integer parts
+// floor(N W_i), raw residuals modf(N W_i) folded exactly for their total
+// (the fold passes), R = N - sum of integer parts draws from the
+// normalised residuals. k_fe_resid_tiles gives the raw residuals' tile sums
+// (the fold's approximate prefix) and the integer parts' exact tile sums;
+// k_fe_resid_params the event's residual parameters, and the normalised
+// residuals' approximate tile prefix and predicted crossings for the fast
+// event's counting passes.
+
+__global__ void __launch_bounds__(kBlock) k_fe_resid_tiles(WLog wb, long long M,
+    double coeff, const ResampleDev *prm, double *tsum, long long *tint)
+{
+    if (!prm->active)
+        return;
+    const long long base = static_cast<long long>(blockIdx.x) * kTile;
+    if (base >= M)
+        return;
+    __shared__ double tab[128];
+    __shared__ double dl[kWaves];
+    __shared__ long long il[kWaves];
+    double w[kItems];
+    fe_load_w(wb.loaded(), base, M, tab, w);
+    double rs = 0.0;
+    long long is = 0;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        double ip;
+        rs += modf(coeff * w[j], &ip);
+        is += static_cast<long long>(ip);
+    }
+    rs = block_reduce(rs, OpAddD(), dl);
+    is = block_reduce(is, OpAddL(), il);
+    if (threadIdx.x == 0) {
+        tsum[blockIdx.x] = rs;
+        tint[blockIdx.x] = is;
+    }
+}
+
+// One block: sum of the integer parts, R = N - that sum, the exact residual
+// total (*total, from the fold) and its reciprocal (transform.hpp:161-165),
+// the draw parameters for R draws (u01_sequence.hpp: delta = 1 / R,
+// systematic u0 = U delta); then the raw residuals' tile prefix scaled to
+// the normalised residuals (in place) and the tiles predicted to cross a
+// binade.
+__global__ void __launch_bounds__(kFeScanThreads) k_fe_resid_params(long long M,
+    ResampleDev *prm, const double *total, const long long *tint, double *tpre,
+    int *fpred, int scheme)
+{
+    if (!prm->active)
+        return;
+    const long long nb = (M + kTile - 1) / kTile;
+    __shared__ long long il[kFeScanThreads / kWave];
+    __shared__ double inv_sh;
+    __shared__ int cnt;
+    long long is = 0;
+    for (long long b = threadIdx.x; b < nb; b += kFeScanThreads)
+        is += tint[b];
+    long long tot_is;
+    block_exclusive_scan_n<kFeScanThreads>(is, OpAddL(), 0LL, il, &tot_is);
+    is = tot_is;
+    if (threadIdx.x == 0) {
+        prm->integ_sum = is;
+        const long long nd = prm->nsample - is;
+        prm->ndraw = nd;
+        const double tot = *total;
+        prm->resid_total = tot;
+        prm->inv_resid = 1 / tot;
+        const double delta = 1 / static_cast<double>(nd);
+        prm->delta = delta;
+        prm->u0 = scheme == kResidualSystematic ?
+            u01_u32d(philox_word(prm->key, prm->word0)) * delta : 0.0;
+        inv_sh = 1 / tot;
+    }
+    __syncthreads();
+    const double inv = inv_sh;
+    for (long long b = threadIdx.x; b < nb; b += kFeScanThreads)
+        tpre[b] = tpre[b] * inv;
+    list_predicted(tpre, nb, fpred, &cnt);
 }

@@ -189,10 +189,17 @@ enum WeightKind {
 };
 
 // Schemes whose particle-set events run the fast event (fast_event.hpp):
-// the finalize prepares their draw parameters and the weights' tile prefix.
+// the finalize prepares their draw parameters and the weights' tile prefix
+// (residual schemes then set their own from the residuals).
 inline bool fast_event_scheme(int scheme)
 {
-    return scheme == kSystematic || scheme == kStratified || scheme == kMultinomial;
+    return scheme >= kMultinomial && scheme <= kResidualSystematic;
+}
+
+inline bool residual_scheme(int scheme)
+{
+    return scheme == kResidual || scheme == kResidualStratified ||
+        scheme == kResidualSystematic;
 }
 
 // Replication counts of the transform with explicit uniforms:

@@ -2742,20 +2742,21 @@ static int run_count(Workspace &ws, WSrc src, USrc us, long long M, int *rep,
 // The counts of a particle set's event (fast_event.hpp): log-weights,
 // systematic or stratified draws prepared on the device (ws.tpre and
 // ws.fpred written by the finalize), M == N.
-template <class USrc>
-static int fast_count(Workspace &ws, WLog wb, USrc us, long long M, int *rep)
+template <class WSrc, class USrc>
+static int fast_count(Workspace &ws, WSrc wb, USrc us, long long M, int *rep)
 {
     hipStream_t st = ws.stream;
     const unsigned int gt = grid_tiles(M);
-    hipLaunchKernelGGL(k_fe_summ<WLog>, dim3(kFePred + gt), dim3(kBlock), 0, st, wb,
-        M, ws.prm, ws.tpre, ws.fpred, ws.segs, ws.fsum, ws.fcross);
-    hipLaunchKernelGGL(k_fe_resolve<WLog>, dim3(1), dim3(kFeThreads), 0, st, wb, M,
-        ws.prm, ws.segs, ws.fsum, ws.fcross, ws.groups, ws.xstart, ws.xgstart,
-        ws.gmode);
+    const long long *none = nullptr;
+    hipLaunchKernelGGL(k_fe_summ<WSrc>, dim3(kFePred + gt), dim3(kBlock), 0, st, wb,
+        M, none, ws.prm, ws.tpre, ws.fpred, ws.segs, ws.fsum, ws.fcross);
+    hipLaunchKernelGGL(k_fe_resolve<WSrc>, dim3(1), dim3(kFeThreads), 0, st, wb, M,
+        none, ws.prm, ws.segs, ws.fsum, ws.fcross, ws.groups, ws.xstart,
+        ws.xgstart, ws.gmode);
     if constexpr (std::is_same<USrc, UArr>::value)
         hipLaunchKernelGGL(k_fe_arr_bounds, dim3((gt + kWaves - 1) / kWaves),
             dim3(kBlock), 0, st, us, M, ws.prm, ws.xstart, ws.hard);
-    hipLaunchKernelGGL((k_fe_count<WLog, USrc>), dim3(gt), dim3(kBlock), 0, st,
+    hipLaunchKernelGGL((k_fe_count<WSrc, USrc>), dim3(gt), dim3(kBlock), 0, st,
         wb, us, M, ws.prm, ws.xstart, rep, ws.tagg, ws.csrc, ws.cE,
         static_cast<const int *>(ws.hard));
     VSMCH_HIP(hipGetLastError());
@@ -2801,27 +2802,29 @@ static int run_maxscan(Workspace &ws, const double *u, long long n,
 }
 
 // The sorted draws of a particle set's multinomial event (n draws of the
-// stream prm->key, prm->word0) into out with the fast event's passes
+// stream prm->key, prm->word0; *ndev when given, at most n) into out with
+// the fast event's passes
 // (fast_event.hpp); the terms in ws.v, their tile prefix in ws.tsum and
 // predicted crossings in ws.tq. Leaves ws.tpre and ws.fpred (the weights')
 // untouched; the running maximum runs in place only where u decreased.
-static int run_sorted_fast(Workspace &ws, long long n, double *out)
+static int run_sorted_fast(Workspace &ws, long long n, const long long *ndev,
+    double *out)
 {
     hipStream_t st = ws.stream;
     const unsigned int gt = grid_tiles(n);
-    hipLaunchKernelGGL(k_fe_sorted_terms, dim3(gt), dim3(kBlock), 0, st, n,
+    hipLaunchKernelGGL(k_fe_sorted_terms, dim3(gt), dim3(kBlock), 0, st, n, ndev,
         ws.prm, ws.v, ws.tsum);
     hipLaunchKernelGGL(k_fe_sorted_scan, dim3(1), dim3(kFeScanThreads), 0, st, n,
-        ws.prm, ws.tsum, ws.tq);
+        ndev, ws.prm, ws.tsum, ws.tq);
     hipLaunchKernelGGL(k_fe_summ<WArray>, dim3(kFePred + gt), dim3(kBlock), 0, st,
-        WArray{ws.v}, n, ws.prm, ws.tsum, ws.tq, ws.segs, ws.fsum, ws.fcross);
+        WArray{ws.v}, n, ndev, ws.prm, ws.tsum, ws.tq, ws.segs, ws.fsum, ws.fcross);
     hipLaunchKernelGGL(k_fe_resolve<WArray>, dim3(1), dim3(kFeThreads), 0, st,
-        WArray{ws.v}, n, ws.prm, ws.segs, ws.fsum, ws.fcross, ws.groups,
+        WArray{ws.v}, n, ndev, ws.prm, ws.segs, ws.fsum, ws.fcross, ws.groups,
         ws.xstart, ws.xgstart, ws.gmode);
-    hipLaunchKernelGGL(k_fe_sorted_prefix, dim3(gt), dim3(kBlock), 0, st, n,
+    hipLaunchKernelGGL(k_fe_sorted_prefix, dim3(gt), dim3(kBlock), 0, st, n, ndev,
         ws.prm, ws.v, ws.xstart, out);
     VSMCH_HIP(hipGetLastError());
-    return run_maxscan(ws, out, n, nullptr, out, &ws.prm->nonmono);
+    return run_maxscan(ws, out, n, ndev, out, &ws.prm->nonmono);
 }
 
 int trans_u01_rep(Workspace &ws, long long M, long long N, const double *W,
@@ -2995,12 +2998,12 @@ static int scheme_counts(Workspace &ws, int scheme, long long M, long long N,
     const bool resid = scheme == kResidual || scheme == kResidualStratified ||
         scheme == kResidualSystematic;
     int rc;
-    if (prepared) {
+    if (prepared && !resid) {
         // draw parameters and the weights' tile prefix were set on the device
         if (scheme == kMultinomial) {
             if constexpr (std::is_same<Base, WLog>::value) {
                 if (M == N) {
-                    rc = run_sorted_fast(ws, N, ws.aux);
+                    rc = run_sorted_fast(ws, N, nullptr, ws.aux);
                     if (rc != kOk)
                         return rc;
                     return fast_count(ws, wb, UArr{ws.aux, ws.prm}, M, rep);
@@ -3048,6 +3051,39 @@ static int scheme_counts(Workspace &ws, int scheme, long long M, long long N,
     }
     // residual schemes: exact total of the raw residuals, integer parts
     const double coeff = static_cast<double>(N);
+    if constexpr (std::is_same<Base, WLog>::value) {
+        if (prepared && M == N) {
+            // a particle set's event: the raw residuals' tile sums with the
+            // integer parts', their fold for the exact total, the residual
+            // parameters and the normalised residuals' prefix, then the fast
+            // event's counting passes (R sorted draws first for the
+            // multinomial residual scheme)
+            const long long ntiles = (M + kTile - 1) / kTile;
+            const unsigned int gt = grid_tiles(M);
+            hipLaunchKernelGGL(k_fe_resid_tiles, dim3(gt), dim3(kBlock), 0, st, wb,
+                M, coeff, ws.prm, ws.tsum, ws.tagg);
+            hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, st, ws.tsum,
+                M, static_cast<const long long *>(nullptr), ws.prm, ws.tpre);
+            WResidRaw<Base> raw{wb, coeff};
+            rc = run_fold(ws, raw, M, nullptr, true);
+            if (rc != kOk)
+                return rc;
+            hipLaunchKernelGGL(k_fe_resid_params, dim3(1), dim3(kFeScanThreads), 0,
+                st, M, ws.prm, ws.xstart + ntiles, ws.tagg, ws.tpre, ws.fpred,
+                scheme);
+            VSMCH_HIP(hipGetLastError());
+            WResid<Base> res{wb, coeff, ws.prm};
+            if (scheme == kResidual) {
+                rc = run_sorted_fast(ws, N, ndraw_dev, ws.aux);
+                if (rc != kOk)
+                    return rc;
+                return fast_count(ws, res, UArr{ws.aux, ws.prm}, M, rep);
+            }
+            if (scheme == kResidualStratified)
+                return fast_count(ws, res, UStrat{ws.prm}, M, rep);
+            return fast_count(ws, res, USys{ws.prm}, M, rep);
+        }
+    }
     WResidRaw<Base> raw{wb, coeff};
     rc = run_fold(ws, raw, M, nullptr);
     if (rc != kOk)
@@ -3114,7 +3150,8 @@ int resample_scheme(Workspace &ws, int scheme, long long M, long long N,
     const bool resid = scheme == kResidual || scheme == kResidualStratified ||
         scheme == kResidualSystematic;
     // prepared events count their stream words in the caller's closing kernel
-    if (!prep)
+    // (except residual schemes, whose count depends on the integer parts)
+    if (!prep || resid)
         hipLaunchKernelGGL(k_words_used, dim3(1), dim3(1), 0, ws.stream, ws.prm,
             rep, scheme, resid ? 1 : 0,
             weight_kind == kWeightArray ? weights : nullptr, weight_kind,
