@@ -120,6 +120,20 @@ __device__ __forceinline__ void fe_load_w(WLogV s, long long base, long long n,
     }
 }
 
+// Raw residuals modf(N W_i) (WResidRaw) of a tile from the weights' own
+// loader (0 past n).
+template <class B>
+__device__ __forceinline__ void fe_load_w(WResidRaw<B> s, long long base, long long n,
+    double *tab, double (&w)[kItems])
+{
+    fe_load_w(s.b, base, n, tab, w);
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        double ip;
+        w[j] = modf(s.coeff * w[j], &ip);
+    }
+}
+
 // Normalised residual weights inv_resid * modf(N W_i) (WResidV) of a tile
 // from the weights' own loader (0 past n, and when no residual draw
 // remains).
@@ -883,7 +897,7 @@ struct FeCountShared {
 // and the tile's sources (csrc, cE) in its own region.
 template <class WSrc, class USrc>
 __global__ void __launch_bounds__(kBlock, 4) k_fe_count(WSrc src, USrc us,
-    long long M, const ResampleDev *prm, double *xstart, int *rep,
+    long long M, ResampleDev *prm, double *xstart, int *rep,
     long long *tagg, int *csrc, int *cE, const int *gb)
 {
     if (!prm->active)
@@ -1030,6 +1044,15 @@ __global__ void __launch_bounds__(kBlock, 4) k_fe_count(WSrc src, USrc us,
             if (k < nvalid)
                 gk = (base + k == M - 1) ? nd : g[j];
             r[j] = gk - gp;
+            // words of the stream the event consumed (transform.hpp:61-69:
+            // one per draw up to the first failing comparison; systematic:
+            // the constructor's one), from the draws of the last particle
+            if (base + k == M - 1) {
+                const long long jend = gp;
+                prm->words_used = std::is_same<USrc, USys>::value ? 1ULL :
+                    static_cast<unsigned long long>(nd < 1 ? 0 :
+                        (jend + 1 < nd ? jend + 1 : nd));
+            }
             gp = gk;
         }
     }
@@ -1312,9 +1335,10 @@ __global__ void __launch_bounds__(kBlock) k_fe_sorted_prefix(long long nh,
 //
 // The residual transform (transform.hpp:154-168): integer parts
 // floor(N W_i), raw residuals modf(N W_i) folded exactly for their total
-// (the fold passes), R = N - sum of integer parts draws from the
-// normalised residuals. k_fe_resid_tiles gives the raw residuals' tile sums
-// (the fold's approximate prefix) and the integer parts' exact tile sums;
+// (the fast event's summaries and resolver, then the last tile's fold),
+// R = N - sum of integer parts draws from the normalised residuals.
+// k_fe_resid_tiles gives the raw residuals' tile sums (the fold's
+// approximate prefix) and the integer parts' exact tile sums;
 // k_fe_resid_params the event's residual parameters, and the normalised
 // residuals' approximate tile prefix and predicted crossings for the fast
 // event's counting passes.
@@ -1348,44 +1372,80 @@ __global__ void __launch_bounds__(kBlock) k_fe_resid_tiles(WLog wb, long long M,
     }
 }
 
-// One block: sum of the integer parts, R = N - that sum, the exact residual
-// total (*total, from the fold) and its reciprocal (transform.hpp:161-165),
-// the draw parameters for R draws (u01_sequence.hpp: delta = 1 / R,
-// systematic u0 = U delta); then the raw residuals' tile prefix scaled to
-// the normalised residuals (in place) and the tiles predicted to cross a
-// binade.
-__global__ void __launch_bounds__(kFeScanThreads) k_fe_resid_params(long long M,
-    ResampleDev *prm, const double *total, const long long *tint, double *tpre,
-    int *fpred, int scheme)
+// One block: the exact total of the M raw residuals (from the last tile's
+// exact start: the steps in its binade, else the exact fold) into
+// xstart[ntiles]; the sum of the integer parts, R = N - that sum, the total's
+// reciprocal (transform.hpp:159-165) and the draw parameters for R draws
+// (u01_sequence.hpp: delta = 1 / R, systematic u0 = U delta); then the raw
+// residuals' tile prefix rpre scaled to the normalised residuals (into
+// tpre) and the tiles predicted to cross a binade.
+struct FeResidShared {
+    double tile[kTileLds];
+    fold::ExactShared ex;
+    double tab[128];
+    double dlds[kWaves];
+    long long il[kWaves];
+    double total;
+    int cnt;
+};
+
+template <class B>
+__global__ void __launch_bounds__(kBlock) k_fe_resid_params(WResidRaw<B> src,
+    long long M, ResampleDev *prm, double *xstart, const long long *tint,
+    const double *rpre, double *tpre, int *fpred, int scheme)
 {
     if (!prm->active)
         return;
     const long long nb = (M + kTile - 1) / kTile;
-    __shared__ long long il[kFeScanThreads / kWave];
-    __shared__ double inv_sh;
-    __shared__ int cnt;
+    const long long t = nb - 1;
+    const long long base = t * kTile;
+    __shared__ FeResidShared sh;
+    const int nvalid = static_cast<int>(M - base);
+    const double x0 = xstart[t];
+    // the integer parts' tile sums load with the last tile
     long long is = 0;
-    for (long long b = threadIdx.x; b < nb; b += kFeScanThreads)
+    for (long long b = threadIdx.x; b < nb; b += kBlock)
         is += tint[b];
-    long long tot_is;
-    block_exclusive_scan_n<kFeScanThreads>(is, OpAddL(), 0LL, il, &tot_is);
-    is = tot_is;
+    double w[kItems];
+    fe_load_blocked(src.loaded(), base, M, sh.tile, sh.tab, w);
+    const bool xpos = x0 > 0.0 && x0 < INFINITY;
+    const FeSteps st(xpos ? fold::binade(x0) : 0);
+    double dsum = 0.0;
+    bool tie = false;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        const double q = st.q(w[j]);
+        const double r = FeSteps::round(q);
+        tie = tie || FeSteps::half(q, r);
+        dsum += r;
+    }
+    const double D = block_reduce(dsum, OpAddD(), sh.dlds);
+    const bool anytie = __syncthreads_or(tie);
+    const double a0d = xpos ? static_cast<double>(fold::mant(x0)) : 0.0;
+    if (xpos && st.ok && !anytie && a0d + D < static_cast<double>(fold::kTwo53)) {
+        if (threadIdx.x == 0)
+            sh.total = st.value(a0d + D);
+    } else {
+        fold::tile_exact(sh.tile, x0, sh.ex);
+        if (threadIdx.x == 0)
+            sh.total = sh.tile[lds_pad(nvalid - 1)];
+    }
+    is = block_reduce(is, OpAddL(), sh.il);   // (its barriers publish total)
+    const double tot = sh.total;
     if (threadIdx.x == 0) {
+        xstart[nb] = tot;
         prm->integ_sum = is;
         const long long nd = prm->nsample - is;
         prm->ndraw = nd;
-        const double tot = *total;
         prm->resid_total = tot;
         prm->inv_resid = 1 / tot;
         const double delta = 1 / static_cast<double>(nd);
         prm->delta = delta;
         prm->u0 = scheme == kResidualSystematic ?
             u01_u32d(philox_word(prm->key, prm->word0)) * delta : 0.0;
-        inv_sh = 1 / tot;
     }
-    __syncthreads();
-    const double inv = inv_sh;
-    for (long long b = threadIdx.x; b < nb; b += kFeScanThreads)
-        tpre[b] = tpre[b] * inv;
-    list_predicted(tpre, nb, fpred, &cnt);
+    const double inv = 1 / tot;
+    for (long long b = threadIdx.x; b < nb; b += kBlock)
+        tpre[b] = rpre[b] * inv;
+    list_predicted(tpre, nb, fpred, &sh.cnt);
 }

@@ -196,11 +196,6 @@ inline bool fast_event_scheme(int scheme)
     return scheme >= kMultinomial && scheme <= kResidualSystematic;
 }
 
-inline bool residual_scheme(int scheme)
-{
-    return scheme == kResidual || scheme == kResidualStratified ||
-        scheme == kResidualSystematic;
-}
 
 // Replication counts of the transform with explicit uniforms:
 // resample_trans_u01_rep(M, N, W, u01, rep) semantics. Device pointers.

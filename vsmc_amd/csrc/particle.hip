@@ -1574,7 +1574,7 @@ static int resample_event(ParticleSet *p, int scheme, bool prep, int *rep_out,
     end.rep = rep;
     end.n = p->n;
     end.scheme = scheme;
-    end.words = prep && !residual_scheme(scheme) ? 1 : 0;
+    end.words = 0;   // counted by the counting pass (or k_words_used)
     end.deferred = defer ? 1 : 0;
     // a prepared event counted with the fast event, which leaves every
     // tile's sources in its own region

@@ -3054,23 +3054,28 @@ static int scheme_counts(Workspace &ws, int scheme, long long M, long long N,
     if constexpr (std::is_same<Base, WLog>::value) {
         if (prepared && M == N) {
             // a particle set's event: the raw residuals' tile sums with the
-            // integer parts', their fold for the exact total, the residual
+            // integer parts', their exact total from the fast event's
+            // summaries and resolver and the last tile's fold, the residual
             // parameters and the normalised residuals' prefix, then the fast
             // event's counting passes (R sorted draws first for the
             // multinomial residual scheme)
-            const long long ntiles = (M + kTile - 1) / kTile;
             const unsigned int gt = grid_tiles(M);
+            const long long *none = nullptr;
             hipLaunchKernelGGL(k_fe_resid_tiles, dim3(gt), dim3(kBlock), 0, st, wb,
                 M, coeff, ws.prm, ws.tsum, ws.tagg);
-            hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kBlock), 0, st, ws.tsum,
-                M, static_cast<const long long *>(nullptr), ws.prm, ws.tpre);
+            // raw residuals: prefix in ws.tsum, predicted crossings in ws.tq
+            hipLaunchKernelGGL(k_fe_sorted_scan, dim3(1), dim3(kFeScanThreads), 0,
+                st, M, none, ws.prm, ws.tsum, ws.tq);
             WResidRaw<Base> raw{wb, coeff};
-            rc = run_fold(ws, raw, M, nullptr, true);
-            if (rc != kOk)
-                return rc;
-            hipLaunchKernelGGL(k_fe_resid_params, dim3(1), dim3(kFeScanThreads), 0,
-                st, M, ws.prm, ws.xstart + ntiles, ws.tagg, ws.tpre, ws.fpred,
-                scheme);
+            hipLaunchKernelGGL(k_fe_summ<WResidRaw<Base>>, dim3(kFePred + gt),
+                dim3(kBlock), 0, st, raw, M, none, ws.prm, ws.tsum, ws.tq,
+                ws.segs, ws.fsum, ws.fcross);
+            hipLaunchKernelGGL(k_fe_resolve<WResidRaw<Base>>, dim3(1),
+                dim3(kFeThreads), 0, st, raw, M, none, ws.prm, ws.segs, ws.fsum,
+                ws.fcross, ws.groups, ws.xstart, ws.xgstart, ws.gmode);
+            hipLaunchKernelGGL(k_fe_resid_params<Base>, dim3(1), dim3(kBlock), 0,
+                st, raw, M, ws.prm, ws.xstart, ws.tagg, ws.tsum, ws.tpre,
+                ws.fpred, scheme);
             VSMCH_HIP(hipGetLastError());
             WResid<Base> res{wb, coeff, ws.prm};
             if (scheme == kResidual) {
@@ -3149,9 +3154,8 @@ int resample_scheme(Workspace &ws, int scheme, long long M, long long N,
         return rc;
     const bool resid = scheme == kResidual || scheme == kResidualStratified ||
         scheme == kResidualSystematic;
-    // prepared events count their stream words in the caller's closing kernel
-    // (except residual schemes, whose count depends on the integer parts)
-    if (!prep || resid)
+    // the fast event's counting pass counts the stream words itself
+    if (!prep)
         hipLaunchKernelGGL(k_words_used, dim3(1), dim3(1), 0, ws.stream, ws.prm,
             rep, scheme, resid ? 1 : 0,
             weight_kind == kWeightArray ? weights : nullptr, weight_kind,
