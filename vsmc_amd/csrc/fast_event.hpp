@@ -1141,8 +1141,8 @@ __global__ void __launch_bounds__(kBlock, 4) k_fe_count(WSrc src, USrc us,
 // cross a binade, k_fe_summ / k_fe_resolve the exact starts, and
 // k_fe_sorted_prefix every u_k from its exact sum (the steps in the start's
 // binade, else the exact fold), recording whether u ever decreases: the
-// running maximum (transform.hpp's merge) is then the identity, and the
-// maximum-scan passes run only otherwise.
+// running maximum (transform.hpp's merge) is then the identity, and
+// k_fe_sorted_max applies it only otherwise.
 
 // Terms a_k for k in [0, n) of the stream (key, word0 + k) (n: *ndev when
 // given, the grid covering nh; as in the next two), four words of a
@@ -1242,7 +1242,7 @@ struct FeSortedShared {
 // prm->nonmono when some u_k < u_{k-1}.
 __global__ void __launch_bounds__(kBlock) k_fe_sorted_prefix(long long nh,
     const long long *ndev, ResampleDev *prm, const double *terms,
-    const double *xstart, double *out)
+    const double *xstart, double *out, double *tmax)
 {
     if (!prm->active)
         return;
@@ -1327,8 +1327,55 @@ __global__ void __launch_bounds__(kBlock) k_fe_sorted_prefix(long long nh,
             if (k0 + j < nvalid)
                 out[base + k0 + j] = u[j];
     }
+    // the tile's maximum, for the running maximum should u decrease
+    double mx = -INFINITY;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+        if (k0 + j < nvalid)
+            mx = u[j] > mx ? u[j] : mx;
+    mx = block_reduce(mx, OpMaxD(), sh.dlds);
     if (__syncthreads_or(dec) && tid == 0)
         atomicOr(&prm->nonmono, 1);
+    if (tid == 0)
+        tmax[t] = mx;
+}
+
+// The running maximum v_k = max(u_0 .. u_k) in place (transform.hpp's merge
+// reads the draws through it), only when k_fe_sorted_prefix saw u decrease:
+// tile t's carry is the maximum of the earlier tiles' maxima tmax.
+__global__ void __launch_bounds__(kBlock) k_fe_sorted_max(long long nh,
+    const long long *ndev, const ResampleDev *prm, const double *tmax,
+    double *u)
+{
+    if (!prm->active || !prm->nonmono)
+        return;
+    const long long n = ndev ? *ndev : nh;
+    const long long t = blockIdx.x;
+    const long long base = t * kTile;
+    if (base >= n)
+        return;
+    __shared__ double tile[kTileLds];
+    __shared__ double lds[kWaves];
+    double carry = -INFINITY;
+    for (long long b = threadIdx.x; b < t; b += kBlock)
+        carry = tmax[b] > carry ? tmax[b] : carry;
+    carry = block_reduce(carry, OpMaxD(), lds);
+    double w[kItems];
+    load_tile_blocked(WArray{u}, base, n, w, tile, -INFINITY);
+    double m = -INFINITY;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j)
+        m = w[j] > m ? w[j] : m;
+    double tot;
+    double run = block_exclusive_scan(m, OpMaxD(), -__builtin_huge_val(), lds, &tot);
+    run = run > carry ? run : carry;
+    const long long ib = base + static_cast<long long>(threadIdx.x) * kItems;
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+        run = w[j] > run ? w[j] : run;
+        if (ib + j < n)
+            u[ib + j] = run;
+    }
 }
 
 // ---------------------------------------------------- residual schemes
@@ -1433,6 +1480,7 @@ __global__ void __launch_bounds__(kBlock) k_fe_resid_params(WResidRaw<B> src,
     is = block_reduce(is, OpAddL(), sh.il);   // (its barriers publish total)
     const double tot = sh.total;
     if (threadIdx.x == 0) {
+        sh.cnt = 0;
         xstart[nb] = tot;
         prm->integ_sum = is;
         const long long nd = prm->nsample - is;
@@ -1444,8 +1492,41 @@ __global__ void __launch_bounds__(kBlock) k_fe_resid_params(WResidRaw<B> src,
         prm->u0 = scheme == kResidualSystematic ?
             u01_u32d(philox_word(prm->key, prm->word0)) * delta : 0.0;
     }
+    // the scaled prefix of a thread's consecutive tiles (and the next
+    // one's), all in registers, and their predicted crossings
+    // (fold::predicted_crossing)
     const double inv = 1 / tot;
-    for (long long b = threadIdx.x; b < nb; b += kBlock)
-        tpre[b] = rpre[b] * inv;
-    list_predicted(tpre, nb, fpred, &sh.cnt);
+    constexpr int kPer = 16;
+    const long long b0 = static_cast<long long>(threadIdx.x) * kPer;
+    for (long long c0 = 0; c0 < nb; c0 += static_cast<long long>(kBlock) * kPer) {
+        double tp[kPer + 1];
+#pragma unroll
+        for (int k = 0; k <= kPer; ++k) {
+            const long long b = c0 + b0 + k;
+            tp[k] = rpre[b < nb ? b : nb - 1];
+        }
+#pragma unroll
+        for (int k = 0; k <= kPer; ++k) {
+            keep_loaded(tp[k]);
+            tp[k] = tp[k] * inv;
+        }
+        __syncthreads();   // sh.cnt cleared
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const long long b = c0 + b0 + k;
+            if (b < nb) {
+                tpre[b] = tp[k];
+                const bool cross = b == 0 || b + 1 >= nb ||
+                    fold::sum_tag(tp[k]) != fold::sum_tag(tp[k + 1]);
+                if (cross) {
+                    const int q = atomicAdd(&sh.cnt, 1);
+                    if (q < kFePred)
+                        fpred[1 + q] = static_cast<int>(b);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        fpred[0] = sh.cnt;
 }

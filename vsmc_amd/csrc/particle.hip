@@ -67,6 +67,28 @@ __device__ inline void block_partials(Part3 p, Part3 *out)
     block_partials_at(p, out + blockIdx.x);
 }
 
+// Partials (max, sum exp, sum exp^2) of k log-weights held by one thread:
+// the maximum first, then one exp per value. -inf and NaN contribute
+// nothing, as in part_of / part_merge.
+template <int K>
+__device__ inline Part3 part_of_array(const double (&v)[K])
+{
+    double mx = -INFINITY;
+    for (int k = 0; k < K; ++k)
+        mx = v[k] > mx ? v[k] : mx;
+    Part3 p{mx, 0.0, 0.0};
+    if (mx > -INFINITY) {
+        for (int k = 0; k < K; ++k) {
+            if (v[k] > -INFINITY) {
+                const double r = exp(v[k] - mx);
+                p.s += r;
+                p.q += r * r;
+            }
+        }
+    }
+    return p;
+}
+
 // A log-weight that came out NaN (a non-finite state: a float uniform of
 // exactly 0 gives an infinite Box-Muller radius, probability 2^-24 per
 // draw) becomes -inf: the particle carries no weight and the next
@@ -91,18 +113,34 @@ __global__ void __launch_bounds__(kPB) k_weight_pass(double *l,
     Part3 *part)
 {
     const long long base = static_cast<long long>(blockIdx.x) * kPTile;
-    Part3 acc{-INFINITY, 0.0, 0.0};
+    const bool old = mode == 1 || mode == 4;   // the update reads log W
+    const int equal = pd->equal;
+    const double log_equal = pd->log_equal;
+    const double c = pd->c;
+    // clamped, unconditional loads, all in flight before their first use
+    double a[kPItems], lo[kPItems];
+#pragma unroll
     for (int j = 0; j < kPItems; ++j) {
         const long long i = base + j * kPB + threadIdx.x;
-        if (i < n) {
-            const double a = src[i * stride];
-            const double in = mode >= 3 ? log(a) : a;
-            const double v = (mode == 0 || mode == 3) ? in : logw_of(l, i, pd) + in;
-            l[i] = v;
-            acc = part_merge(acc, part_of(v));
-        }
+        const long long ic = i < n ? i : n - 1;
+        a[j] = src[ic * stride];
+        lo[j] = old && !equal ? l[ic] : 0.0;
     }
-    block_partials(acc, part);
+    double v[kPItems];
+#pragma unroll
+    for (int j = 0; j < kPItems; ++j) {
+        keep_loaded(a[j]);
+        keep_loaded(lo[j]);
+        const long long i = base + j * kPB + threadIdx.x;
+        const double in = mode >= 3 ? log(a[j]) : a[j];
+        const double lw = equal ? log_equal : lo[j] - c;
+        v[j] = old ? lw + in : in;
+        if (i < n)
+            l[i] = v[j];
+        else
+            v[j] = -INFINITY;
+    }
+    block_partials(part_of_array(v), part);
 }
 
 // Merges block partials; mode 0: set_log, 1: add_log, 2: set_equal.
@@ -735,28 +773,6 @@ __device__ inline void cv_advance(const CvModelDev &m, uint32_t key,
         vx = radd(vx, rmul(static_cast<Real>(m.sd_vel), p1.z0));
         vy = radd(vy, rmul(static_cast<Real>(m.sd_vel), p1.z1));
     }
-}
-
-// Partials (max, sum exp, sum exp^2) of k log-weights held by one thread:
-// the maximum first, then one exp per value. -inf and NaN contribute
-// nothing, as in part_of / part_merge.
-template <int K>
-__device__ inline Part3 part_of_array(const double (&v)[K])
-{
-    double mx = -INFINITY;
-    for (int k = 0; k < K; ++k)
-        mx = v[k] > mx ? v[k] : mx;
-    Part3 p{mx, 0.0, 0.0};
-    if (mx > -INFINITY) {
-        for (int k = 0; k < K; ++k) {
-            if (v[k] > -INFINITY) {
-                const double r = exp(v[k] - mx);
-                p.s += r;
-                p.q += r * r;
-            }
-        }
-    }
-    return p;
 }
 
 // Tile `tile` (kPTile particles) of a filter of n particles whose moved

@@ -1235,12 +1235,10 @@ __global__ void __launch_bounds__(kBlock) k_prefix(WSrc src, long long nh,
 
 // ------------------------------------------------- running maximum (v)
 
-// (gate: when non-null, the running maximum is needed only if *gate is set)
 __global__ void __launch_bounds__(kBlock) k_tile_max(const double *u,
-    long long nh, const long long *ndev, const ResampleDev *prm, double *tmax,
-    const int *gate)
+    long long nh, const long long *ndev, const ResampleDev *prm, double *tmax)
 {
-    if (!prm->active || (gate && !*gate))
+    if (!prm->active)
         return;
     const long long n = eff_n(ndev, nh);
     const long long base = static_cast<long long>(blockIdx.x) * kTile;
@@ -1262,10 +1260,9 @@ __global__ void __launch_bounds__(kBlock) k_tile_max(const double *u,
 }
 
 __global__ void __launch_bounds__(kBlock) k_scan_max(const double *tmax,
-    long long nh, const long long *ndev, const ResampleDev *prm, double *tpre,
-    const int *gate)
+    long long nh, const long long *ndev, const ResampleDev *prm, double *tpre)
 {
-    if (!prm->active || (gate && !*gate))
+    if (!prm->active)
         return;
     const long long n = eff_n(ndev, nh);
     const long long ntiles = ceil_div(n, kTile);
@@ -1284,9 +1281,9 @@ __global__ void __launch_bounds__(kBlock) k_scan_max(const double *tmax,
 
 __global__ void __launch_bounds__(kBlock) k_apply_max(const double *u,
     long long nh, const long long *ndev, const ResampleDev *prm,
-    const double *tpre, double *v, const int *gate)
+    const double *tpre, double *v)
 {
-    if (!prm->active || (gate && !*gate))
+    if (!prm->active)
         return;
     const long long n = eff_n(ndev, nh);
     const long long base = static_cast<long long>(blockIdx.x) * kTile;
@@ -2785,18 +2782,17 @@ static int setup_host(Workspace &ws, long long M, long long N,
     return kOk;
 }
 
-// Running maximum of u (n entries) into v (ws.v when null; v == u runs in
-// place); with gate, only when the device flag *gate is set.
+// Running maximum of u (n entries) into ws.v.
 static int run_maxscan(Workspace &ws, const double *u, long long n,
-    const long long *ndev, double *v = nullptr, const int *gate = nullptr)
+    const long long *ndev)
 {
     const unsigned int gt = grid_tiles(n);
     hipLaunchKernelGGL(k_tile_max, dim3(gt), dim3(kBlock), 0, ws.stream, u, n,
-        ndev, ws.prm, ws.tsum, gate);
+        ndev, ws.prm, ws.tsum);
     hipLaunchKernelGGL(k_scan_max, dim3(1), dim3(kBlock), 0, ws.stream,
-        ws.tsum, n, ndev, ws.prm, ws.tpre, gate);
+        ws.tsum, n, ndev, ws.prm, ws.tpre);
     hipLaunchKernelGGL(k_apply_max, dim3(gt), dim3(kBlock), 0, ws.stream, u,
-        n, ndev, ws.prm, ws.tpre, v ? v : ws.v, gate);
+        n, ndev, ws.prm, ws.tpre, ws.v);
     VSMCH_HIP(hipGetLastError());
     return kOk;
 }
@@ -2822,9 +2818,11 @@ static int run_sorted_fast(Workspace &ws, long long n, const long long *ndev,
         WArray{ws.v}, n, ndev, ws.prm, ws.segs, ws.fsum, ws.fcross, ws.groups,
         ws.xstart, ws.xgstart, ws.gmode);
     hipLaunchKernelGGL(k_fe_sorted_prefix, dim3(gt), dim3(kBlock), 0, st, n, ndev,
-        ws.prm, ws.v, ws.xstart, out);
+        ws.prm, ws.v, ws.xstart, out, ws.tsum);
+    hipLaunchKernelGGL(k_fe_sorted_max, dim3(gt), dim3(kBlock), 0, st, n, ndev,
+        ws.prm, ws.tsum, out);
     VSMCH_HIP(hipGetLastError());
-    return run_maxscan(ws, out, n, ndev, out, &ws.prm->nonmono);
+    return kOk;
 }
 
 int trans_u01_rep(Workspace &ws, long long M, long long N, const double *W,
