@@ -413,12 +413,13 @@ __global__ void __launch_bounds__(kMergeThreads) k_prepare(PDev *pd, ResampleDev
 // (pos_prev, when given), the ESS before resampling and the decision
 // (sampler.hpp:772-778), and for systematic / stratified events (prep) the
 // draw parameters and the approximate tile prefix. The move has read the
-// ancestors a previous event left, so anc_valid is cleared.
+// ancestors a previous event left, so anc_valid is cleared (and with swap
+// the current buffer pd->cur changes when it was set).
 __global__ void __launch_bounds__(kMergeThreads) k_step_finalize(
     const Part3 *part, long long nb, long long n, int mode, PDev *pd,
     ResampleDev *prm, double threshold, long long h, double *ess_hist,
     int *rs_hist, double *logz_hist, const double *mpart, double *pos_prev,
-    int scheme, int prep, double *tpre, int *fpred)
+    int scheme, int prep, double *tpre, int *fpred, int swap)
 {
     __shared__ int act;
     __shared__ int sh_equal;
@@ -496,6 +497,10 @@ __global__ void __launch_bounds__(kMergeThreads) k_step_finalize(
         }
         if (mode == 0)
             pd->fallback = 0;
+        // swap: the move could read through the ancestors (cv_iterate's two
+        // buffers); when it did, it wrote the other buffer
+        if (swap && pd->anc_valid)
+            pd->cur ^= 1;
         pd->anc_valid = 0;
         if (logz_hist)
             logz_hist[h] = logz;
@@ -566,18 +571,27 @@ __global__ void __launch_bounds__(kMergeThreads) k_step_finalize(
     list_predicted(tpre, nb, fpred, &npred);
 }
 
+// cv_iterate starts from the particle set's own buffer (a kernel, not a
+// memset, so that a captured sequence replays it).
+__global__ void k_own_buffer(PDev *pd)
+{
+    pd->cur = 0;
+}
+
 // The state after the last move of cv_iterate, into the particle set's own
-// buffer home: particle i from cur at anc[i] when the last event left its
-// ancestors (pd->anc_valid), else at i. With cur == home it runs in place
-// and writes only the slots whose ancestor is another particle: such an
-// ancestor has copies, so it keeps its own slot and is never written.
+// buffer home: particle i from the current buffer (pd->cur: home or alt) at
+// anc[i] when the last event left its ancestors (pd->anc_valid), else at i.
+// With the current buffer home it runs in place and writes only the slots
+// whose ancestor is another particle: such an ancestor has copies, so it
+// keeps its own slot and is never written.
 template <typename Real>
-__global__ void __launch_bounds__(kPB) k_settle(const Real *cur, Real *home,
+__global__ void __launch_bounds__(kPB) k_settle(const Real *alt, Real *home,
     const int *anc, const PDev *pd, long long n, int dim)
 {
     const long long i = static_cast<long long>(blockIdx.x) * kPB + threadIdx.x;
     if (i >= n)
         return;
+    const Real *const cur = pd->cur ? alt : home;
     const long long a = pd->anc_valid ? static_cast<long long>(anc[i]) : i;
     if (cur == home && a == i)
         return;
@@ -714,6 +728,19 @@ struct alignas(4 * sizeof(Real)) Vec4 {
 // made opaque first: the optimiser would otherwise split a store of a
 // vector it sees assembled from scalars into scalar stores and drop the
 // nontemporal flag.
+// Four log-weights loaded nontemporally (read once a step, stored
+// nontemporally by the previous move).
+__device__ inline void load_nt(const double *p, double (&a)[4])
+{
+    typedef double R2 __attribute__((ext_vector_type(2)));
+    const R2 v0 = __builtin_nontemporal_load(reinterpret_cast<const R2 *>(p));
+    const R2 v1 = __builtin_nontemporal_load(reinterpret_cast<const R2 *>(p) + 1);
+    a[0] = v0.x;
+    a[1] = v0.y;
+    a[2] = v1.x;
+    a[3] = v1.y;
+}
+
 __device__ inline void store_nt(double *p, const double (&a)[4])
 {
     typedef double R2 __attribute__((ext_vector_type(2)));
@@ -831,10 +858,11 @@ __device__ inline void cv_tile(const Real *SX, const Real *SY, const Real *SVX,
                         vys[r] = *reinterpret_cast<const Vec4<Real> *>(SVY + i0);
                     }
                     if (!equal) {
-                        const Vec4<double> lq = *reinterpret_cast<const Vec4<double> *>(l + i0);
+                        double lq[4];
+                        load_nt(l + i0, lq);
 #pragma unroll
                         for (int q = 0; q < 4; ++q)
-                            lv[(r0 + r) * 4 + q] = lq.v[q] - c;
+                            lv[(r0 + r) * 4 + q] = lq[q] - c;
                     } else {
 #pragma unroll
                         for (int q = 0; q < 4; ++q)
@@ -932,12 +960,24 @@ __device__ inline void cv_tile(const Real *SX, const Real *SY, const Real *SVX,
 
 // One move over all tiles: from src (through the ancestors anc when the
 // last event left them, see cv_tile) into state; src == state moves in place.
+// With alt (two buffers, state and alt, pd->cur naming the current one):
+// in place in the current buffer, except for a move through the ancestors,
+// which reads the current buffer and writes the other (the step's finalize
+// then swaps pd->cur). A move in place streams read and written lines of
+// the same addresses, measured 20 us (12 %) faster at N = 2^24 than one
+// between two buffers.
 template <typename Real, bool kInit, int kB>
 __global__ void __launch_bounds__(kPB) k_cv_step(Real *state, const Real *src,
-    const int *anc, double *__restrict__ l, long long n, uint32_t seed,
-    long long t, Real ox, Real oy, CvModelDev m, const PDev *pd, Part3 *part,
-    double *mon)
+    Real *alt, const int *anc, double *__restrict__ l, long long n,
+    uint32_t seed, long long t, Real ox, Real oy, CvModelDev m, const PDev *pd,
+    Part3 *part, double *mon)
 {
+    if (alt) {
+        const bool gather = anc != nullptr && pd->anc_valid;
+        Real *const now = pd->cur ? alt : state;
+        src = now;
+        state = gather ? (now == alt ? state : alt) : now;
+    }
     cv_tile<Real, kInit, kB>(src, src + n, src + 2 * n, src + 3 * n, anc,
         state, state + n, state + 2 * n, state + 3 * n, l, n, blockIdx.x, seed,
         t, ox, oy, m, pd,
@@ -1685,17 +1725,17 @@ template <typename Real, bool kInit>
 static void launch_cv(int batch, long long nb, hipStream_t st, Real *s,
     const Real *src, const int *anc, double *l, long long n, uint32_t seed,
     long long t, Real ox, Real oy, const CvModelDev &m, const PDev *pd,
-    Part3 *part, double *mon)
+    Part3 *part, double *mon, Real *alt = nullptr)
 {
     if (batch == 4)
         hipLaunchKernelGGL((k_cv_step<Real, kInit, 4>), dim3(nb), dim3(kPB),
-            0, st, s, src, anc, l, n, seed, t, ox, oy, m, pd, part, mon);
+            0, st, s, src, alt, anc, l, n, seed, t, ox, oy, m, pd, part, mon);
     else if (batch == 2)
         hipLaunchKernelGGL((k_cv_step<Real, kInit, 2>), dim3(nb), dim3(kPB),
-            0, st, s, src, anc, l, n, seed, t, ox, oy, m, pd, part, mon);
+            0, st, s, src, alt, anc, l, n, seed, t, ox, oy, m, pd, part, mon);
     else
         hipLaunchKernelGGL((k_cv_step<Real, kInit, 1>), dim3(nb), dim3(kPB),
-            0, st, s, src, anc, l, n, seed, t, ox, oy, m, pd, part, mon);
+            0, st, s, src, alt, anc, l, n, seed, t, ox, oy, m, pd, part, mon);
 }
 
 int cv_step(ParticleSet *p, const CvModelDev &m, long long t, double ox,
@@ -1824,8 +1864,11 @@ static int cv_iterate_impl(ParticleSet *p, const CvModelDev &m, long long t0,
     if (!p->anc)
         VSMCH_HIP(hipMalloc(&p->anc, sizeof(int) * p->n));
     Real *const home = static_cast<Real *>(p->state);
-    Real *cur = home;
-    Real *nxt = static_cast<Real *>(p->state_alt);
+    Real *const alt = static_cast<Real *>(p->state_alt);
+    // the state starts in the particle set's own buffer; each move runs in
+    // place in the current one, a move through the ancestors writes the
+    // other (pd->cur on the device)
+    hipLaunchKernelGGL(k_own_buffer, dim3(1), dim3(1), 0, st, p->pd);
     const bool prep = p->n >= 2 && fast_event_scheme(scheme);
     const int batch = move_batch(p->bytes);
     Part3 *part = static_cast<Part3 *>(p->part);
@@ -1834,27 +1877,26 @@ static int cv_iterate_impl(ParticleSet *p, const CvModelDev &m, long long t0,
         const Real fx = static_cast<Real>(ox[s]), fy = static_cast<Real>(oy[s]);
         // the move of step s also gives the monitor record of step s - 1
         double *mon = (pos_d && s > 0) ? p->mpart : nullptr;
+        const int *anc = (t > 0 && s > 0) ? p->anc : nullptr;
         if (t == 0)
-            launch_cv<Real, true>(batch, nb, st, nxt, cur, nullptr, p->logw,
-                p->n, p->seed, t, fx, fy, m, p->pd, part, nullptr);
+            launch_cv<Real, true>(batch, nb, st, home, home, nullptr, p->logw,
+                p->n, p->seed, t, fx, fy, m, p->pd, part, nullptr, alt);
         else
-            launch_cv<Real, false>(batch, nb, st, nxt, cur,
-                s > 0 ? p->anc : nullptr, p->logw, p->n, p->seed, t, fx, fy, m,
-                p->pd, part, mon);
-        std::swap(cur, nxt);
+            launch_cv<Real, false>(batch, nb, st, home, home, anc, p->logw, p->n,
+                p->seed, t, fx, fy, m, p->pd, part, mon, alt);
         hipLaunchKernelGGL(k_step_finalize, dim3(1), dim3(kMergeThreads), 0, st,
             static_cast<const Part3 *>(part), nb, p->n, t == 0 ? 0 : 1, p->pd,
             p->ws.prm, threshold, s, ess_d, rs_d, logz_d,
             static_cast<const double *>(p->mpart),
             mon ? pos_d + 2 * (s - 1) : nullptr, scheme, prep ? 1 : 0,
-            p->ws.tpre, p->ws.fpred);
+            p->ws.tpre, p->ws.fpred, anc != nullptr ? 1 : 0);
         VSMCH_HIP(hipGetLastError());
         const int rc = resample_event(p, scheme, prep, nullptr, nullptr, true);
         if (rc != kOk)
             return rc;
     }
     hipLaunchKernelGGL(k_settle<Real>, dim3((p->n + kPB - 1) / kPB), dim3(kPB),
-        0, st, static_cast<const Real *>(cur), home,
+        0, st, static_cast<const Real *>(alt), home,
         static_cast<const int *>(p->anc), static_cast<const PDev *>(p->pd),
         p->n, p->dim);
     VSMCH_HIP(hipGetLastError());

@@ -22,6 +22,10 @@ struct PDev {
     // the last event left its ancestors in ParticleSet::anc instead of
     // gathering the state: the next move reads particle i from anc[i]
     int anc_valid;
+    // cv_iterate: the buffer that holds the current state (0: the particle
+    // set's own, 1: the alternate); a move runs in place unless it gathers
+    // through the ancestors, which writes the other buffer
+    int cur;
     // learned proposal: a learned step's ESS collapsed below
     // kLearnedCollapse N, and the filter's remaining moves take the bootstrap
     // proposal (cleared by set_log / the initialisation)
