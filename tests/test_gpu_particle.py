@@ -127,6 +127,32 @@ def test_particle_events_full_size(dev, s):
         p.close()
 
 
+def test_weight_exponential(dev):
+    """The weights the resampler uses, W_i = e^(l_i - lmax) / sum with the
+    library's table-based exponential: against math.exp over [-700, 0]
+    (every 64th of an octave, both signs of the remainder, integers and
+    random points), W_i / W_max within 3 ulp of e^(l_i - lmax)."""
+    import vsmc_amd as va
+    rs = np.random.default_rng(7)
+    ln2 = math.log(2.0)
+    x = np.concatenate([
+        -np.arange(0, 700, dtype=np.float64),
+        -(np.arange(64 * 40) / 64.0) * ln2,
+        -(np.arange(64 * 40) / 64.0 + 1.0 / 128.0) * ln2,
+        -rs.uniform(0, 700, 20000),
+        -rs.uniform(0, 1, 20000)])
+    l = 3.25 + x                       # lmax = 3.25 (x = 0 is present)
+    p = va.Particle(len(l), 1, "float64", seed=3)
+    p.set_log(_t(l))
+    W = _np(p.weights())
+    imax = int(np.argmax(l))
+    ratio = W / W[imax]
+    ref = np.array([math.exp(v) for v in x])
+    err = np.abs(ratio - ref) / (ref * 2.0 ** -52)
+    assert err.max() <= 3.0, (float(err.max()), float(x[np.argmax(err)]))
+    p.close()
+
+
 def test_monitor_moments(dev):
     import torch
     import vsmc_amd as va
