@@ -751,12 +751,103 @@ template <> struct FeCounter<UStrat> { using type = FeStrat; };
 // searched in place by every sum, so that no block walks a long slice
 // alone. g[j] = G(sk[j]) and gp = G of the sum before the thread's first
 // element (glo for the first thread).
+// The draws of a count kernel's slice: materialised sorted uniforms
+// (multinomial; loaded) or stratified draws (generated from their Philox
+// words, v_j = max(u_{j-1}, u_j) as UStrat). at(j) is draw j; fill(buf, j0,
+// cn) puts draws j0 .. j0 + cn - 1 at buf[0 ..] and +inf in the chunk's
+// other slots (block-wide; the caller synchronises after it).
+template <class U> struct FeDraws;
+
+template <> struct FeDraws<UArr> {
+    const double *v;
+    __device__ explicit FeDraws(const UArr &us) : v(us.v) {}
+    __device__ double at(long long j) const { return v[j]; }
+    __device__ void fill(double *buf, long long j0, int cn) const;
+};
+
+template <> struct FeDraws<UStrat> {
+    UStrat us;
+    __device__ explicit FeDraws(const UStrat &u) : us(u) {}
+    __device__ double at(long long j) const { return us.vi(static_cast<int>(j)); }
+    __device__ void fill(double *buf, long long j0, int cn) const;
+};
+
 constexpr int kFeProbes = 8;                        // draws probed at once
 constexpr int kFeSliceItems = kTileLds / kBlock;     // chunk slots per thread
 constexpr int kFeSlice = kFeSliceItems * kBlock - kFeProbes;   // draws per chunk
 static_assert(kFeSliceItems * kBlock <= kTileLds, "a chunk fits the tile buffer");
 
-__device__ inline void fe_arr_counts(const UArr &us, const double (&sk)[kItems],
+// clamped, unconditional loads, all in flight together
+__device__ inline void FeDraws<UArr>::fill(double *buf, long long j0, int cn) const
+{
+    const int tid = static_cast<int>(threadIdx.x);
+    const double *const vc = v + j0;
+    double x[kFeSliceItems];
+#pragma unroll
+    for (int j = 0; j < kFeSliceItems; ++j) {
+        const int q = j * kBlock + tid;
+        x[j] = vc[q < cn ? q : cn - 1];
+    }
+#pragma unroll
+    for (int j = 0; j < kFeSliceItems; ++j) {
+        keep_loaded(x[j]);
+        buf[j * kBlock + tid] = j * kBlock + tid < cn ? x[j] : INFINITY;
+    }
+}
+
+// u_w for the stream words of draws j0 - 1 .. j0 + cn - 1 (a Philox block,
+// four words, per thread at a time, from the block holding draw j0 - 1's
+// word), then v_j = max(u_{j-1}, u_j) (v_0 = u_0) in place.
+__device__ inline void FeDraws<UStrat>::fill(double *buf, long long j0, int cn) const
+{
+    const int tid = static_cast<int>(threadIdx.x);
+    const ResampleDev *const p = us.p;
+    const unsigned int key = p->key;
+    const double delta = p->delta;
+    const long long jf = j0 > 0 ? j0 - 1 : 0;        // first draw generated
+    const unsigned long long wf = p->word0 + static_cast<unsigned long long>(jf);
+    const unsigned long long b0 = wf / 4;             // its Philox block - 1
+    const long long lead = static_cast<long long>(wf - 4 * b0);   // slot of draw jf
+    const long long nslot = lead + (j0 + cn - jf);    // slots 0 .. nslot - 1 used
+    const long long nblk = (nslot + 3) / 4;
+    for (long long b = tid; b < nblk; b += kBlock) {
+        const u32x4 w = philox_block(key, b0 + b + 1);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const long long slot = 4 * b + q;
+            const long long j = jf + (slot - lead);   // draw of this word
+            if (slot >= lead && slot < nslot)
+                buf[slot] = rn_add(rn_mul(u01_u32d(w.v[q]), delta),
+                    rn_mul(static_cast<double>(j), delta));
+        }
+    }
+    __syncthreads();
+    // draw j0 + q at slot lead + (j0 - jf) + q; v from the slot before
+    const long long s0 = lead + (j0 - jf);
+    double x[kFeSliceItems];
+#pragma unroll
+    for (int j = 0; j < kFeSliceItems; ++j) {
+        const int q = j * kBlock + tid;
+        double a = 0.0;
+        if (q < cn) {
+            a = buf[s0 + q];
+            if (j0 + q > 0) {
+                const double b = buf[s0 + q - 1];
+                a = b > a ? b : a;
+            }
+        }
+        x[j] = a;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kFeSliceItems; ++j) {
+        const int q = j * kBlock + tid;
+        buf[q] = q < cn ? x[j] : INFINITY;
+    }
+}
+
+template <class U>
+__device__ inline void fe_arr_counts(const FeDraws<U> &us, const double (&sk)[kItems],
     long long glo, long long ghi, int nvalid, double *buf, int (&g)[kItems],
     int &gp)
 {
@@ -778,7 +869,7 @@ __device__ inline void fe_arr_counts(const UArr &us, const double (&sk)[kItems],
                 long long lo = 0, hi = m;   // first slice index with v > sk
                 while (lo < hi) {
                     const long long mid = (lo + hi) >> 1;
-                    if (us.v[glo + mid] <= sk[j])
+                    if (us.at(glo + mid) <= sk[j])
                         lo = mid + 1;
                     else
                         hi = mid;
@@ -790,22 +881,7 @@ __device__ inline void fe_arr_counts(const UArr &us, const double (&sk)[kItems],
     }
     for (long long c0 = 0; c0 < (in_lds ? m : 0); c0 += kFeSlice) {
         const int cn = m - c0 < kFeSlice ? static_cast<int>(m - c0) : kFeSlice;
-        {
-            // clamped, unconditional loads, all in flight together
-            const double *const vc = us.v + glo + c0;
-            double x[kFeSliceItems];
-#pragma unroll
-            for (int j = 0; j < kFeSliceItems; ++j) {
-                const int q = j * kBlock + tid;
-                x[j] = vc[q < cn ? q : cn - 1];
-            }
-            // slots past the chunk hold +inf, above every sum
-#pragma unroll
-            for (int j = 0; j < kFeSliceItems; ++j) {
-                keep_loaded(x[j]);
-                buf[j * kBlock + tid] = j * kBlock + tid < cn ? x[j] : INFINITY;
-            }
-        }
+        us.fill(buf, glo + c0, cn);
         __syncthreads();
         // a thread's sums are sorted: a binary search for the first one
         // resolved in this chunk, then from the previous answer a (the
@@ -877,6 +953,20 @@ __global__ void __launch_bounds__(kBlock) k_fe_arr_bounds(UArr us, long long M,
         gb[t] = static_cast<int>(g);
 }
 
+// The same for stratified draws, by UStrat's own search: one thread per
+// tile.
+__global__ void __launch_bounds__(kBlock) k_fe_strat_bounds(UStrat us, long long M,
+    const ResampleDev *prm, const double *xstart, int *gb)
+{
+    if (!prm->active)
+        return;
+    const long long ntiles = (M + kTile - 1) / kTile;
+    const long long t = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= ntiles)
+        return;
+    gb[t] = t == 0 ? 0 : static_cast<int>(us.count(xstart[t]));
+}
+
 // Relative bound between an FP64 block-scan prefix of a tile (exact start
 // plus a blocked scan: at most 16 + 6 + 3 + 18 roundings along any path)
 // and the sequential fold over at most 4096 weights from the same exact
@@ -912,10 +1002,12 @@ __global__ void __launch_bounds__(kBlock, 4) k_fe_count(WSrc src, USrc us,
     const int nvalid = rem < kTile ? static_cast<int>(rem) : kTile;
     const bool last_tile = base + kTile >= M;
     const double x0 = xstart[t];
-    // materialised draws: the tile's slice bounds (the last tile's reaches
-    // the end)
+    // sorted or stratified draws: the tile's slice bounds (the last tile's
+    // reaches the end)
+    constexpr bool kSlice = std::is_same<USrc, UArr>::value ||
+        std::is_same<USrc, UStrat>::value;
     long long glo = 0, ghi = 0;
-    if constexpr (std::is_same<USrc, UArr>::value) {
+    if constexpr (kSlice) {
         glo = gb[t];
         ghi = last_tile ? us.ndraw() : gb[t + 1];
     }
@@ -943,9 +1035,9 @@ __global__ void __launch_bounds__(kBlock, 4) k_fe_count(WSrc src, USrc us,
     // g[j] = G(s_k) for element k = k0 + j, gp = G before the thread's first
     int g[kItems];
     int gp;
-    if constexpr (std::is_same<USrc, UArr>::value) {
-        // materialised draws: the exact sums first (from the steps, or the
-        // exact fold), then their counts from the tile's slice of draws
+    if constexpr (kSlice) {
+        // sorted or stratified draws: the exact sums first (from the steps,
+        // or the exact fold), then their counts from the tile's slice
         double sk[kItems];
         if (fast) {
 #pragma unroll
@@ -960,7 +1052,7 @@ __global__ void __launch_bounds__(kBlock, 4) k_fe_count(WSrc src, USrc us,
                 sk[j] = sh.tile[lds_pad(k0 + j)];
         }
         __syncthreads();   // every thread's sums read from the tile
-        fe_arr_counts(us, sk, glo, ghi, nvalid, sh.tile, g, gp);
+        fe_arr_counts(FeDraws<USrc>(us), sk, glo, ghi, nvalid, sh.tile, g, gp);
     } else {
         const typename FeCounter<USrc>::type cnt(us);
         if (fast) {

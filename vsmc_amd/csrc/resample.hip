@@ -2753,6 +2753,9 @@ static int fast_count(Workspace &ws, WSrc wb, USrc us, long long M, int *rep)
     if constexpr (std::is_same<USrc, UArr>::value)
         hipLaunchKernelGGL(k_fe_arr_bounds, dim3((gt + kWaves - 1) / kWaves),
             dim3(kBlock), 0, st, us, M, ws.prm, ws.xstart, ws.hard);
+    if constexpr (std::is_same<USrc, UStrat>::value)
+        hipLaunchKernelGGL(k_fe_strat_bounds, dim3((gt + kBlock - 1) / kBlock),
+            dim3(kBlock), 0, st, us, M, ws.prm, ws.xstart, ws.hard);
     hipLaunchKernelGGL((k_fe_count<WSrc, USrc>), dim3(gt), dim3(kBlock), 0, st,
         wb, us, M, ws.prm, ws.xstart, rep, ws.tagg, ws.csrc, ws.cE,
         static_cast<const int *>(ws.hard));
