@@ -745,8 +745,8 @@ template <> struct FeCounter<UStrat> { using type = FeStrat; };
 // only the slice v[glo, ghi) decides them. The slice goes through LDS (buf,
 // the tile buffer) in chunks of kFeSlice draws (a tile's share of the draws
 // and some margin), every open sum searched there (a binary search for a
-// thread's first, then kFeProbes probes past the previous answer, since a
-// thread's sums are sorted); a
+// thread's first, then rounds of kFeProbes probes past the previous answer,
+// since a thread's sums are sorted); a
 // slice of more than kVChunks chunks (weight concentrated in the tile) is
 // searched in place by every sum, so that no block walks a long slice
 // alone. g[j] = G(sk[j]) and gp = G of the sum before the thread's first
@@ -909,14 +909,25 @@ __device__ inline void fe_arr_counts(const FeDraws<U> &us, const double (&sk)[kI
                 } else {
                     // a < cn, so the probes stay in the buffer (slots past
                     // the chunk: +inf); the draws at or below sk[j] are a
-                    // prefix of them
-                    int n = 0;
+                    // prefix of them. A second round when all lie below
+                    // (then a + kFeProbes <= the answer < cn), a binary
+                    // search after that (a wave runs a round when any lane
+                    // needs it)
+                    auto probe = [&](int at) {
+                        int c = 0;
 #pragma unroll
-                    for (int q = 0; q < kFeProbes; ++q)
-                        n += buf[a + q] <= sk[j] ? 1 : 0;
+                        for (int q = 0; q < kFeProbes; ++q)
+                            c += buf[at + q] <= sk[j] ? 1 : 0;
+                        return c;
+                    };
+                    int n = probe(a);
                     a += n;
-                    if (n == kFeProbes)
-                        a = search(a, sk[j]);
+                    if (n == kFeProbes) {
+                        n = probe(a);
+                        a += n;
+                        if (n == kFeProbes)
+                            a = search(a, sk[j]);
+                    }
                 }
                 cnt[j] = static_cast<int>(c0) + a;
                 open &= ~(1u << j);
