@@ -183,6 +183,34 @@ __device__ inline T block_reduce_n(T v, Op op, T *lds)
     return r;
 }
 
+// Sums of K values per thread over kThreads threads in one reduction (one
+// pair of barriers for all K), broadcast; lds: K entries per wave.
+template <int kThreads, int K>
+__device__ inline void block_sum_n(double (&v)[K], double *lds)
+{
+    constexpr int nw = kThreads / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        v[k] = wave_reduce(v[k], OpAddD());
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            lds[wave * K + k] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double r = lds[k];
+#pragma unroll
+        for (int w = 1; w < nw; ++w)
+            r += lds[w * K + k];
+        v[k] = r;
+    }
+    __syncthreads();
+}
+
 // Partials a merge thread holds in registers: thread t has partials
 // t kMergePer .. t kMergePer + kMergePer - 1, loaded together.
 constexpr int kMergePer = 4;
@@ -193,9 +221,11 @@ constexpr int kMergePer = 4;
 // <= kThreads kMergePer every thread loads its partials at once (clamped,
 // unconditional loads) and keeps them: held is set and sb[k] = s_b e^{m_b -
 // max} of its k-th partial (0 past nblocks). Otherwise strided loops.
+// extra (when non-null): two more per-thread sums, reduced in the same
+// pass (in place). dlds: four entries per wave.
 template <int kThreads>
 __device__ inline Part3 merge_partials(const Part3 *part, long long nblocks,
-    double (&sb)[kMergePer], bool &held, double *dlds)
+    double (&sb)[kMergePer], bool &held, double *dlds, double *extra = nullptr)
 {
     if (nblocks <= static_cast<long long>(kThreads) * kMergePer) {
         held = true;
@@ -228,9 +258,13 @@ __device__ inline Part3 merge_partials(const Part3 *part, long long nblocks,
             S += sb[k];
             Q += q[k] * r * r;
         }
-        S = block_reduce_n<kThreads>(S, OpAddD(), dlds);
-        Q = block_reduce_n<kThreads>(Q, OpAddD(), dlds);
-        return Part3{mx, S, Q};
+        double v[4] = {S, Q, extra ? extra[0] : 0.0, extra ? extra[1] : 0.0};
+        block_sum_n<kThreads, 4>(v, dlds);
+        if (extra) {
+            extra[0] = v[2];
+            extra[1] = v[3];
+        }
+        return Part3{mx, v[0], v[1]};
     }
     held = false;
     double mx = -INFINITY;
@@ -250,9 +284,13 @@ __device__ inline Part3 merge_partials(const Part3 *part, long long nblocks,
             }
         }
     }
-    S = block_reduce_n<kThreads>(S, OpAddD(), dlds);
-    Q = block_reduce_n<kThreads>(Q, OpAddD(), dlds);
-    return Part3{mx, S, Q};
+    double v[4] = {S, Q, extra ? extra[0] : 0.0, extra ? extra[1] : 0.0};
+    block_sum_n<kThreads, 4>(v, dlds);
+    if (extra) {
+        extra[0] = v[2];
+        extra[1] = v[3];
+    }
+    return Part3{mx, v[0], v[1]};
 }
 
 // The weight scalars from a merged (max, S, Q): thread 0 only.
@@ -264,7 +302,7 @@ template <int kThreads>
 __device__ inline void weight_merge(const Part3 *part, long long nblocks,
     long long n, int mode, PDev *pd, int guard)
 {
-    __shared__ double dlds[kThreads / kWave];
+    __shared__ double dlds[4 * (kThreads / kWave)];
     Part3 acc{-INFINITY, 0.0, 0.0};
     if (mode != 2) {
         double sb[kMergePer];
@@ -424,7 +462,7 @@ __global__ void __launch_bounds__(kMergeThreads) k_step_finalize(
     __shared__ int act;
     __shared__ int sh_equal;
     __shared__ double sh_lmax, sh_inv;
-    __shared__ double lds[kMergeThreads / kWave];
+    __shared__ double lds[4 * (kMergeThreads / kWave)];
     // the "pos" monitor's partials load with the weight partials (a thread's
     // four consecutive blocks, clamped, unconditional loads)
     const bool mheld = pos_prev && nb <= static_cast<long long>(kMergeThreads) * kMergePer;
@@ -448,29 +486,34 @@ __global__ void __launch_bounds__(kMergeThreads) k_step_finalize(
         rkey = pd->rkey;
         rword = pd->rword;
     }
-    Part3 acc{-INFINITY, 0.0, 0.0};
-    double sb[kMergePer];
-    bool held = false;
-    if (mode != 2)
-        acc = merge_partials<kMergeThreads>(part, nb, sb, held, lds);
+    // the monitor's per-thread sums, reduced with the weight sums
+    double ms[2] = {0.0, 0.0};
     if (pos_prev) {
         for (int d = 0; d < 2; ++d) {
-            double a = 0.0;
             if (mheld) {
                 const long long b0 = static_cast<long long>(threadIdx.x) * kMergePer;
 #pragma unroll
                 for (int k = 0; k < kMergePer; ++k) {
                     keep_loaded(mp[d][k]);
-                    a += b0 + k < nb ? mp[d][k] : 0.0;
+                    ms[d] += b0 + k < nb ? mp[d][k] : 0.0;
                 }
             } else {
                 for (long long b = threadIdx.x; b < nb; b += kMergeThreads)
-                    a += mpart[b * 8 + d];
+                    ms[d] += mpart[b * 8 + d];
             }
-            a = block_reduce_n<kMergeThreads>(a, OpAddD(), lds);
-            if (threadIdx.x == 0)
-                pos_prev[d] = a;
         }
+    }
+    Part3 acc{-INFINITY, 0.0, 0.0};
+    double sb[kMergePer];
+    bool held = false;
+    if (mode != 2) {
+        acc = merge_partials<kMergeThreads>(part, nb, sb, held, lds, ms);
+    } else if (pos_prev) {
+        block_sum_n<kMergeThreads, 2>(ms, lds);
+    }
+    if (pos_prev && threadIdx.x == 0) {
+        pos_prev[0] = ms[0];
+        pos_prev[1] = ms[1];
     }
     if (threadIdx.x == 0) {
         // weight_merge_store and decide_event on registers: the same
@@ -537,8 +580,13 @@ __global__ void __launch_bounds__(kMergeThreads) k_step_finalize(
     const double lmax = sh_lmax;
     const double inv = sh_inv;
     const double inv_n = 1 / static_cast<double>(n);
+    __shared__ int npred;
     if (held && !equal) {
-        // the approximate tile prefix from the scaled sums already held
+        // the approximate tile prefix from the scaled sums already held, and
+        // the predicted crossing tiles (fold::predicted_crossing, on the
+        // same values: a tile's start as written, the next thread's first
+        // through LDS)
+        __shared__ double first[kMergeThreads];
         const long long b0 = static_cast<long long>(threadIdx.x) * kMergePer;
         double v[kMergePer];
         double loc = 0.0;
@@ -547,14 +595,35 @@ __global__ void __launch_bounds__(kMergeThreads) k_step_finalize(
             v[k] = sb[k] * inv;
             loc += v[k];
         }
+        if (threadIdx.x == 0)
+            npred = 0;
         double tot;
         double run = block_exclusive_scan_n<kMergeThreads>(loc, OpAddD(), 0.0, lds, &tot);
+        double pre[kMergePer + 1];
 #pragma unroll
         for (int k = 0; k < kMergePer; ++k) {
+            pre[k] = run;
             if (b0 + k < nb)
                 tpre[b0 + k] = run;
             run += v[k];
         }
+        first[threadIdx.x] = pre[0];
+        __syncthreads();
+        pre[kMergePer] = threadIdx.x + 1 < kMergeThreads ? first[threadIdx.x + 1] : run;
+#pragma unroll
+        for (int k = 0; k < kMergePer; ++k) {
+            const long long b = b0 + k;
+            if (b < nb && (b == 0 || b + 1 >= nb ||
+                    fold::sum_tag(pre[k]) != fold::sum_tag(pre[k + 1]))) {
+                const int q = atomicAdd(&npred, 1);
+                if (q < kFePred)
+                    fpred[1 + q] = static_cast<int>(b);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            fpred[0] = npred;
+        return;
     } else {
         block_scan_chunked_n<kMergeThreads, 4>(nb,
             [&](long long b) {
@@ -567,7 +636,6 @@ __global__ void __launch_bounds__(kMergeThreads) k_step_finalize(
             },
             [&](long long b, double ex) { tpre[b] = ex; }, OpAddD(), 0.0, lds);
     }
-    __shared__ int npred;
     list_predicted(tpre, nb, fpred, &npred);
 }
 
