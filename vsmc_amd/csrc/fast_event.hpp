@@ -808,41 +808,67 @@ __device__ inline void FeDraws<UStrat>::fill(double *buf, long long j0, int cn) 
     const unsigned long long wf = p->word0 + static_cast<unsigned long long>(jf);
     const unsigned long long b0 = wf / 4;             // its Philox block - 1
     const long long lead = static_cast<long long>(wf - 4 * b0);   // slot of draw jf
-    const long long nslot = lead + (j0 + cn - jf);    // slots 0 .. nslot - 1 used
-    const long long nblk = (nslot + 3) / 4;
-    for (long long b = tid; b < nblk; b += kBlock) {
-        const u32x4 w = philox_block(key, b0 + b + 1);
+    // slots 0 .. nslot - 1 used (nslot <= 4 + kFeSlice: 32-bit slot indices;
+    // draw jf + s - lead as jf + (s - lead) in FP64, both exact integers)
+    const int nslot = static_cast<int>(lead + (j0 + cn - jf));
+    const int nblk = (nslot + 3) / 4;
+    const int ilead = static_cast<int>(lead);
+    const double djf = static_cast<double>(jf);
+    for (int b = tid; b < nblk; b += kBlock) {
+        const u32x4 w = philox_block(key, b0 + static_cast<unsigned long long>(b) + 1);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const long long slot = 4 * b + q;
-            const long long j = jf + (slot - lead);   // draw of this word
-            if (slot >= lead && slot < nslot)
-                buf[slot] = rn_add(rn_mul(u01_u32d(w.v[q]), delta),
-                    rn_mul(static_cast<double>(j), delta));
+            const int slot = 4 * b + q;
+            const double j = djf + static_cast<double>(slot - ilead);   // draw of this word
+            if (slot >= ilead && slot < nslot)
+                buf[slot] = rn_add(rn_mul(u01_u32d(w.v[q]), delta), rn_mul(j, delta));
         }
     }
     __syncthreads();
-    // draw j0 + q at slot lead + (j0 - jf) + q; v from the slot before
+    // draw j0 + q at slot lead + (j0 - jf) + q; v from the slot before. In
+    // groups of kG items, each group's reads before its writes, so that only
+    // kG values are live (the caller's sums stay in registers). A group's
+    // writes reach the slots the next group reads only at the slot before
+    // its first draw, and only when s0 == 0: that raw value is kept in
+    // edge[] (one per group parity: a group reads one while it writes the
+    // other).
     const long long s0 = lead + (j0 - jf);
-    double x[kFeSliceItems];
+    constexpr int kG = 6;
+    __shared__ double edge[2];
 #pragma unroll
-    for (int j = 0; j < kFeSliceItems; ++j) {
-        const int q = j * kBlock + tid;
-        double a = 0.0;
-        if (q < cn) {
-            a = buf[s0 + q];
-            if (j0 + q > 0) {
-                const double b = buf[s0 + q - 1];
-                a = b > a ? b : a;
+    for (int g0 = 0; g0 < kFeSliceItems; g0 += kG) {
+        const int par = (g0 / kG) & 1;
+        double x[kG];
+#pragma unroll
+        for (int jj = 0; jj < kG; ++jj) {
+            const int j = g0 + jj;
+            if (j >= kFeSliceItems)
+                break;
+            const int q = j * kBlock + tid;
+            double a = 0.0;
+            if (q < cn) {
+                a = buf[s0 + q];
+                if ((jj == kG - 1 || j == kFeSliceItems - 1) && tid == kBlock - 1)
+                    edge[par] = a;
+                if (j0 + q > 0) {
+                    const double b = (g0 > 0 && jj == 0 && tid == 0 && s0 == 0)
+                        ? edge[par ^ 1] : buf[s0 + q - 1];
+                    a = b > a ? b : a;
+                }
             }
+            x[jj] = a;
         }
-        x[j] = a;
-    }
-    __syncthreads();
+        __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kFeSliceItems; ++j) {
-        const int q = j * kBlock + tid;
-        buf[q] = q < cn ? x[j] : INFINITY;
+        for (int jj = 0; jj < kG; ++jj) {
+            const int j = g0 + jj;
+            if (j >= kFeSliceItems)
+                break;
+            const int q = j * kBlock + tid;
+            buf[q] = q < cn ? x[jj] : INFINITY;
+        }
+        if (g0 + kG < kFeSliceItems)
+            __syncthreads();
     }
 }
 
