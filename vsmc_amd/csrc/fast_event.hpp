@@ -586,19 +586,31 @@ __global__ void __launch_bounds__(kFeThreads) k_fe_resolve(WSrc src, long long n
             const bool last_chunk = c0 + kFeChunk >= ntiles;
             double x = sh.carry;
             int ok = 1;
+            // each item's data is read one item ahead, so that the walk waits
+            // on its own chain, not on LDS (an item reads only the fields its
+            // kind set; the others are in-bounds values it ignores)
+            int ce_n = sh.ce[0];
+            FMap T_n = sh.cT[0];
+            FeCross c2_n = sh.cx2[0];
             for (int r = 0; r < tot_head; ++r) {
                 const bool final_item = last_chunk && r == tot_head - 1;
+                const int ce = ce_n;
+                const FMap T = T_n;
+                const FeCross c2 = c2_n;
+                if (r + 1 < tot_head) {
+                    ce_n = sh.ce[r + 1];
+                    T_n = sh.cT[r + 1];
+                    c2_n = sh.cx2[r + 1];
+                }
                 sh.cx[r] = x;
-                const int ce = sh.ce[r];
                 bool good;
                 if (ce > kFeListTag) {
-                    good = fe_run(x, ce, sh.cT[r]);
+                    good = fe_run(x, ce, T);
                 } else if (ce > kFeCrossTag) {
                     good = fe_apply_segments(sh.seg[kFeListTag - ce], x);
                 } else {
                     // one crossing: a run, the real add, a run
-                    const FeCross c2 = sh.cx2[r];
-                    good = fe_run(x, kFeCrossTag - ce - 1100, sh.cT[r]);
+                    good = fe_run(x, kFeCrossTag - ce - 1100, T);
                     if (good) {
                         x = x + c2.wc;
                         good = fe_run(x, c2.e2, FMap{c2.d2, c2.d2});
